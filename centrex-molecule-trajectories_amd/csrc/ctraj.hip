@@ -1,0 +1,716 @@
+// ctraj.hip -- kernels and C ABI of libctraj.so (gfx950).
+//
+// Kernels:
+//   k_propagate  fused beamline walker: one thread per molecule, grid-stride over a SoA store,
+//                lens tables in LDS, wave-aggregated tallies and aoi compaction.
+//   k_record     the same walker re-run for selected molecules with a row / z-grid sink.
+//   k_draw       Philox4x32-10 + Box-Muller Gaussian source.
+//   k_batch      persistent walker over (configuration, chunk) work items (batch.hip).
+//   k_smooth_*   differentiable flux and its adjoint (adjoint.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctraj_device.hpp"
+#include "ctraj_internal.hpp"
+#include "ctraj_sinks.hpp"
+
+using namespace ctraj;
+
+// ---------------------------------------------------------------------------------------------
+// LDS staging of the lens tables
+// ---------------------------------------------------------------------------------------------
+namespace ctraj {
+
+__device__ Tables stage_tables(double *lds, const KTab &kt) {
+  Tables T;
+  const int nws = kt.ws_states * kt.ws_nseg * 4;
+  const int nrt = kt.rt_total * 4;
+  double2 *dst = reinterpret_cast<double2 *>(lds);
+  const double2 *src_ws = reinterpret_cast<const double2 *>(kt.ws);
+  for (int i = threadIdx.x; i < nws / 2; i += blockDim.x) dst[i] = src_ws[i];
+  const double2 *src_rt = reinterpret_cast<const double2 *>(kt.rt);
+  for (int i = threadIdx.x; i < nrt / 2; i += blockDim.x) dst[nws / 2 + i] = src_rt[i];
+  T.ws = lds;
+  T.nseg = kt.ws_nseg;
+  T.umax = (double)kt.ws_nseg;
+  T.rt = lds + nws;
+  return T;
+}
+
+__device__ __forceinline__ int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Wave-aggregated histogram: one LDS atomic per distinct key in the wave.  The leader of each
+// key group is its lowest lane, which also holds the group's lowest molecule index.
+__device__ __forceinline__ void wave_hist(uint32_t *s_hist, unsigned long long *s_first, int key, int label,
+                                          int64_t gidx) {
+  uint64_t todo = __ballot(1);
+  const int lane = lane_id();
+  while (todo) {
+    const int leader = __ffsll((unsigned long long)todo) - 1;
+    const int lk = __shfl(key, leader);
+    const uint64_t same = __ballot(key == lk) & todo;
+    if (lane == leader) {
+      atomicAdd(&s_hist[lk], (uint32_t)__popcll(same));
+      if (s_first) atomicMin(&s_first[label], (unsigned long long)gidx);
+    }
+    todo &= ~same;
+  }
+}
+
+}  // namespace ctraj
+
+// ---------------------------------------------------------------------------------------------
+// k_propagate
+// ---------------------------------------------------------------------------------------------
+template <int FORCE, int INTEG>
+__global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R) {
+  extern __shared__ double lds[];
+  __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
+  __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
+  __shared__ unsigned long long s_work[4];
+  const int nbins = B.n_states * B.n_labels;
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0;
+  for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x) s_first[b] = ~0ull >> 1;
+  if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
+  const Tables T = stage_tables(lds, KT);
+  __syncthreads();
+
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  WalkCounters wc{0u, 0u};
+  uint32_t n_oot = 0, n_cap = 0;
+  const int lane = lane_id();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < R.n; i += stride) {
+    MState m;
+    m.x = R.in.x[i];
+    m.y = R.in.y[i];
+    m.z = R.in.z ? R.in.z[i] : R.z_const;
+    m.vx = R.in.vx[i];
+    m.vy = R.in.vy[i];
+    m.vz = R.in.vz[i];
+    m.t = R.in.t ? R.in.t[i] : R.t_const;
+    m.ax = R.a0[0] ? R.a0[0][i] : 0.0;
+    m.ay = R.a0[1] ? R.a0[1][i] : -B.g;
+    m.az = R.a0[2] ? R.a0[2][i] : 0.0;
+    const int st = R.state ? min((int)R.state[i], B.n_states - 1) : 0;
+    int label;
+    uint32_t status = 0;
+    NullSink sk;
+    walk<FORCE, INTEG>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status, wc, sk);
+
+    if (R.out.x) R.out.x[i] = m.x;
+    if (R.out.y) R.out.y[i] = m.y;
+    if (R.out.z) R.out.z[i] = m.z;
+    if (R.out.vx) R.out.vx[i] = m.vx;
+    if (R.out.vy) R.out.vy[i] = m.vy;
+    if (R.out.vz) R.out.vz[i] = m.vz;
+    if (R.out.t) R.out.t[i] = m.t;
+    if (R.out_a[0]) R.out_a[0][i] = m.ax;
+    if (R.out_a[1]) R.out_a[1][i] = m.ay;
+    if (R.out_a[2]) R.out_a[2][i] = m.az;
+    if (R.label) R.label[i] = label;
+    if (R.status) R.status[i] = (R.flags & CTRAJ_RUN_STATUS_OR) ? (uint8_t)(R.status[i] | status) : (uint8_t)status;
+    n_oot += (status & CTRAJ_ST_OOT) ? 1u : 0u;
+    n_cap += (status & CTRAJ_ST_CAPPED) ? 1u : 0u;
+
+    const int64_t gidx = R.index_base + i;
+    wave_hist(s_hist, R.first_index ? s_first : nullptr, st * B.n_labels + label, label, gidx);
+
+    if (R.aoi_mask) {
+      const bool keep = (R.aoi_mask >> label) & 1ull;
+      const uint64_t mask = __ballot(keep);
+      if (mask) {
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd((unsigned long long *)R.aoi_count, (unsigned long long)__popcll(mask));
+        base = __shfl(base, leader);
+        if (keep) {
+          const int64_t slot = (int64_t)base + lanes_below(mask);
+          if (slot < R.aoi_cap) {
+            R.aoi_index[slot] = gidx;
+            if (R.aoi_label) R.aoi_label[slot] = label;
+            if (R.aoi_out.x) R.aoi_out.x[slot] = m.x;
+            if (R.aoi_out.y) R.aoi_out.y[slot] = m.y;
+            if (R.aoi_out.z) R.aoi_out.z[slot] = m.z;
+            if (R.aoi_out.vx) R.aoi_out.vx[slot] = m.vx;
+            if (R.aoi_out.vy) R.aoi_out.vy[slot] = m.vy;
+            if (R.aoi_out.vz) R.aoi_out.vz[slot] = m.vz;
+            if (R.aoi_out.t) R.aoi_out.t[slot] = m.t;
+          }
+        }
+      }
+    }
+  }
+  if (R.work) {
+    if (wc.faces) atomicAdd(&s_work[0], (unsigned long long)wc.faces);
+    if (wc.steps) atomicAdd(&s_work[1], (unsigned long long)wc.steps);
+    if (n_oot) atomicAdd(&s_work[2], (unsigned long long)n_oot);
+    if (n_cap) atomicAdd(&s_work[3], (unsigned long long)n_cap);
+  }
+  __syncthreads();
+  if (R.tally)
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+      if (s_hist[b]) atomicAdd((unsigned long long *)&R.tally[b], (unsigned long long)s_hist[b]);
+  if (R.first_index)
+    for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x)
+      if (s_first[b] != (~0ull >> 1)) atomicMin((unsigned long long *)&R.first_index[b], s_first[b]);
+  if (R.work && threadIdx.x < 4 && s_work[threadIdx.x])
+    atomicAdd((unsigned long long *)&R.work[threadIdx.x], s_work[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_record: re-run selected molecules and store their rows (or z-grid samples)
+// ---------------------------------------------------------------------------------------------
+template <int FORCE, int INTEG, int MODE>
+__global__ void __launch_bounds__(256) k_record(KBeam B, KTab KT, ctraj_record RC) {
+  extern __shared__ double lds[];
+  const Tables T = stage_tables(lds, KT);
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < RC.n; s += stride) {
+    const int64_t i = RC.index ? RC.index[s] : s;
+    MState m;
+    m.x = RC.in.x[i];
+    m.y = RC.in.y[i];
+    m.z = RC.in.z ? RC.in.z[i] : RC.z_const;
+    m.vx = RC.in.vx[i];
+    m.vy = RC.in.vy[i];
+    m.vz = RC.in.vz[i];
+    m.t = RC.in.t ? RC.in.t[i] : RC.t_const;
+    m.ax = RC.a0[0] ? RC.a0[0][i] : 0.0;
+    m.ay = RC.a0[1] ? RC.a0[1][i] : -B.g;
+    m.az = RC.a0[2] ? RC.a0[2][i] : 0.0;
+    const int st = RC.state ? min((int)RC.state[i], B.n_states - 1) : 0;
+    int label;
+    uint32_t status = 0;
+    WalkCounters wc{0u, 0u};
+    if constexpr (MODE == CTRAJ_REC_ROWS) {
+      RowSink sk{RC.buf, RC.n, s, RC.cap, 0};
+      walk<FORCE, INTEG>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status, wc, sk);
+      sk.finish(m);
+      if (RC.n_rows) RC.n_rows[s] = sk.count;
+    } else {
+      ZGridSink sk;
+      sk.buf = RC.buf; sk.n = RC.n; sk.slot = s; sk.cap = RC.cap;
+      sk.z_start = RC.z_start; sk.z_step = RC.z_step;
+      sk.init();
+      walk<FORCE, INTEG>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status, wc, sk);
+      sk.finish(m);
+      if (RC.n_rows) RC.n_rows[s] = sk.written;
+    }
+    if (RC.label) RC.label[s] = label;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Philox4x32-10 Gaussian source
+// ---------------------------------------------------------------------------------------------
+struct DrawArgs {
+  double mean[8], sigma[8];
+  double *out[8];
+};
+
+__global__ void __launch_bounds__(256) k_draw(uint64_t seed, uint64_t offset, int64_t n, int nd, DrawArgs A) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    for (int p = 0; 2 * p < nd; ++p) {
+      double z0, z1;
+      gaussian_pair(seed, (uint64_t)i + offset, (uint32_t)p, z0, z1);
+      if (A.out[2 * p]) A.out[2 * p][i] = A.mean[2 * p] + A.sigma[2 * p] * z0;
+      if (2 * p + 1 < nd && A.out[2 * p + 1]) A.out[2 * p + 1][i] = A.mean[2 * p + 1] + A.sigma[2 * p + 1] * z1;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fill_i64(int64_t *dst, int64_t v, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = v;
+}
+
+__global__ void __launch_bounds__(256) k_fill_u8(uint8_t *dst, uint8_t v, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------------------
+namespace ctraj {
+
+static thread_local std::string g_err;
+
+int fail(ctraj_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  g_err = msg;
+  return code;
+}
+
+int hipfail(ctraj_ctx *ctx, hipError_t e, const char *what) {
+  return fail(ctx, CTRAJ_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+KTab ctx_ktab(const ctraj_ctx *ctx) {
+  KTab k{};
+  k.ws = ctx->d_ws;
+  k.ws_nseg = ctx->ws_nseg;
+  k.ws_states = ctx->d_ws ? ctx->ws_states : 0;
+  k.rt = ctx->d_rt;
+  k.rt_total = ctx->d_rt ? ctx->rt_total : 0;
+  return k;
+}
+
+size_t ktab_lds_bytes(const KTab &k) { return ((size_t)k.ws_states * k.ws_nseg + (size_t)k.rt_total) * 4 * sizeof(double); }
+
+// Validate an element table and find the (force, integrator) pair shared by its lenses.
+int check_elems(ctraj_ctx *ctx, const ctraj_elem *e, int n, const ctraj_phys *p, int *force, int *integ) {
+  *force = CTRAJ_FORCE_WSPLINE_HEX;
+  *integ = CTRAJ_INTEG_RK4;
+  bool have_lens = false;
+  for (int i = 0; i < n; ++i) {
+    const ctraj_elem &E = e[i];
+    if (E.kind < CTRAJ_CIRC || E.kind > CTRAJ_LENS) return fail(ctx, CTRAJ_EINVAL, "unknown element kind");
+    if (E.label_hit < 0 || E.label_hit >= p->n_labels) return fail(ctx, CTRAJ_EINVAL, "element label out of range");
+    if (i > 0 && E.z0 < e[i - 1].z0) return fail(ctx, CTRAJ_EINVAL, "elements must be sorted by z0");
+    if (E.kind == CTRAJ_LENS) {
+      if (E.label_inside < 0 || E.label_inside >= p->n_labels)
+        return fail(ctx, CTRAJ_EINVAL, "lens label_inside out of range");
+      if (E.force < 0 || E.force > 2 || E.integrator < 0 || E.integrator > 1)
+        return fail(ctx, CTRAJ_EINVAL, "bad lens force/integrator");
+      if (have_lens && (E.force != *force || E.integrator != *integ))
+        return fail(ctx, CTRAJ_EINVAL, "all lenses of a beamline must share force model and integrator");
+      if (E.integrator == CTRAJ_INTEG_RK4 && !(E.p[2] > 0.0)) return fail(ctx, CTRAJ_EINVAL, "RK4 lens needs dt > 0");
+      if (E.force != CTRAJ_FORCE_RTABLE && !(E.p[5] >= 0.0))
+        return fail(ctx, CTRAJ_EINVAL, "lens spline scale p[5] must be >= 0 (use |V|)");
+      if (!(E.p[0] >= 0.0)) return fail(ctx, CTRAJ_EINVAL, "lens bore radius must be >= 0");
+      if (E.integrator == CTRAJ_INTEG_REF && E.n_steps < 0) return fail(ctx, CTRAJ_EINVAL, "negative lens step count");
+      if (E.force == CTRAJ_FORCE_RTABLE && (E.table < 0 || E.table >= 8))
+        return fail(ctx, CTRAJ_EINVAL, "lens r-table id out of range (0..7)");
+      *force = E.force;
+      *integ = E.integrator;
+      have_lens = true;
+    }
+  }
+  if (p->label_detected < 0 || p->label_detected >= p->n_labels)
+    return fail(ctx, CTRAJ_EINVAL, "label_detected out of range");
+  return CTRAJ_OK;
+}
+
+// Tables a launch needs must be present: r tables of RTABLE lenses, the W spline (with enough
+// states) for WSPLINE lenses.
+int check_tables(ctraj_ctx *ctx, const ctraj_elem *e, int n, int n_states) {
+  for (int i = 0; i < n; ++i) {
+    if (e[i].kind != CTRAJ_LENS) continue;
+    if (e[i].force == CTRAJ_FORCE_RTABLE) {
+      if (ctx->rt_n[e[i].table] < 2) return fail(ctx, CTRAJ_ESTATE, "lens r-table not set");
+    } else {
+      if (!ctx->d_ws) return fail(ctx, CTRAJ_ESTATE, "lens W spline not set");
+      if (n_states > ctx->ws_states) return fail(ctx, CTRAJ_ESTATE, "n_states exceeds the W spline states set");
+    }
+  }
+  return CTRAJ_OK;
+}
+
+// Device copy of an element table: an RTABLE lens carries its table's knot offset (table),
+// knot count (reserved) and inverse knot spacing (p[6]) instead of the table id.
+void resolve_elems(const ctraj_ctx *ctx, const ctraj_elem *e, int n, ctraj_elem *out) {
+  for (int i = 0; i < n; ++i) {
+    out[i] = e[i];
+    if (e[i].kind == CTRAJ_LENS && e[i].force == CTRAJ_FORCE_RTABLE) {
+      const int id = e[i].table;
+      out[i].table = ctx->rt_off[id];
+      out[i].reserved = ctx->rt_n[id];
+      out[i].p[6] = ctx->rt_invh[id];
+    }
+  }
+}
+
+static int upload_elems(ctraj_ctx *c) {
+  if (!c->have_beamline) return CTRAJ_OK;
+  std::vector<ctraj_elem> res(c->h_elems.size());
+  resolve_elems(c, c->h_elems.data(), (int)c->h_elems.size(), res.data());
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->d_elems) HIPCHK(c, hipMalloc((void **)&c->d_elems, sizeof(ctraj_elem) * CTRAJ_MAX_ELEMS));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!res.empty()) HIPCHK(c, hipMemcpy(c->d_elems, res.data(), sizeof(ctraj_elem) * res.size(), hipMemcpyHostToDevice));
+  return CTRAJ_OK;
+}
+
+int check_phys(ctraj_ctx *ctx, const ctraj_phys *p) {
+  if (p->n_labels < 1 || p->n_labels > CTRAJ_MAX_LABELS) return fail(ctx, CTRAJ_EINVAL, "n_labels out of range");
+  if (p->n_states < 1 || p->n_states > CTRAJ_MAX_STATES) return fail(ctx, CTRAJ_EINVAL, "n_states out of range");
+  if (p->max_lens_steps < 0) return fail(ctx, CTRAJ_EINVAL, "max_lens_steps < 0");
+  return CTRAJ_OK;
+}
+
+KBeam make_kbeam(const ctraj_elem *d_elems, int n, const ctraj_phys &p) {
+  KBeam b;
+  b.elems = d_elems;
+  b.n_elems = n;
+  b.g = p.g;
+  b.n_labels = p.n_labels;
+  b.label_detected = p.label_detected;
+  b.n_states = p.n_states;
+  b.cap = p.max_lens_steps > 0 ? p.max_lens_steps : (1 << 20);
+  return b;
+}
+
+int lds_limit(ctraj_ctx *ctx, const void *kern, size_t bytes) {
+  if (bytes > (size_t)ctx->lds_max) return fail(ctx, CTRAJ_EINVAL, "lens tables exceed the LDS of one workgroup");
+  if (bytes > 65536) HIPCHK(ctx, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return CTRAJ_OK;
+}
+
+int grid_for(ctraj_ctx *ctx, const void *kern, int block, size_t lds, int64_t n, int req) {
+  if (req > 0) return req;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+  const int64_t need = (n + block - 1) / block;
+  const int64_t full = (int64_t)per_cu * ctx->n_cu;
+  return (int)std::max<int64_t>(1, std::min(need, full));
+}
+
+}  // namespace ctraj
+
+// kernel tables over (force, integrator)
+#define CT_FI_LIST(X)                                 \
+  X(CTRAJ_FORCE_RTABLE, CTRAJ_INTEG_REF)             \
+  X(CTRAJ_FORCE_RTABLE, CTRAJ_INTEG_RK4)             \
+  X(CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_REF)        \
+  X(CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_RK4)        \
+  X(CTRAJ_FORCE_WSPLINE_LIN, CTRAJ_INTEG_REF)        \
+  X(CTRAJ_FORCE_WSPLINE_LIN, CTRAJ_INTEG_RK4)
+
+using PropFn = void (*)(KBeam, KTab, ctraj_run);
+using RecFn = void (*)(KBeam, KTab, ctraj_record);
+
+static PropFn prop_kernel(int f, int i) {
+#define X(F, I) \
+  if (f == F && i == I) return k_propagate<F, I>;
+  CT_FI_LIST(X)
+#undef X
+  return nullptr;
+}
+
+static RecFn rec_kernel(int f, int i, int mode) {
+#define X(F, I)                                                             \
+  if (f == F && i == I) {                                                  \
+    return mode == CTRAJ_REC_ROWS ? k_record<F, I, CTRAJ_REC_ROWS> : k_record<F, I, CTRAJ_REC_ZGRID>; \
+  }
+  CT_FI_LIST(X)
+#undef X
+  return nullptr;
+}
+
+extern "C" {
+
+int32_t ctraj_abi_version(void) { return CTRAJ_ABI_VERSION; }
+
+int32_t ctraj_ctx_create(int32_t device, ctraj_ctx **out) {
+  if (!out) return fail(nullptr, CTRAJ_EINVAL, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess) return hipfail(nullptr, e, "hipGetDeviceCount");
+  if (device < 0 || device >= ndev) return fail(nullptr, CTRAJ_EINVAL, "no such device");
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return hipfail(nullptr, e, "hipSetDevice");
+  ctraj_ctx *c = new ctraj_ctx();
+  c->device = device;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return hipfail(nullptr, e, "hipStreamCreate");
+  }
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return hipfail(nullptr, e, "hipGetDeviceProperties");
+  }
+  c->n_cu = prop.multiProcessorCount;
+  c->lds_max = (int)prop.sharedMemPerBlock;
+  int lds_optin = 0;
+  if (hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, device) == hipSuccess &&
+      lds_optin > c->lds_max)
+    c->lds_max = lds_optin;
+  c->arch = 0;
+  if (std::strncmp(prop.gcnArchName, "gfx", 3) == 0) c->arch = (int)std::strtol(prop.gcnArchName + 3, nullptr, 10);
+  c->wave = prop.warpSize;
+  for (int i = 0; i < 64; ++i) c->ev[i] = nullptr;
+  *out = c;
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_ctx_destroy(ctraj_ctx *c) {
+  if (!c) return CTRAJ_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->d_elems) (void)hipFree(c->d_elems);
+  if (c->d_ws) (void)hipFree(c->d_ws);
+  if (c->d_rt) (void)hipFree(c->d_rt);
+  for (int i = 0; i < 64; ++i)
+    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CTRAJ_OK;
+}
+
+const char *ctraj_last_error(const ctraj_ctx *c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+int32_t ctraj_device_info(ctraj_ctx *c, int32_t *n_cu, int32_t *lds, int32_t *wave, int32_t *arch) {
+  if (!c) return fail(nullptr, CTRAJ_EINVAL, "ctx is NULL");
+  if (n_cu) *n_cu = c->n_cu;
+  if (lds) *lds = c->lds_max;
+  if (wave) *wave = c->wave;
+  if (arch) *arch = c->arch;
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_dev_alloc(ctraj_ctx *c, uint64_t bytes, void **ptr) {
+  if (!c || !ptr) return fail(c, CTRAJ_EINVAL, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  *ptr = nullptr;
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+  if (e != hipSuccess) return fail(c, CTRAJ_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_dev_free(ctraj_ctx *c, void *ptr) {
+  if (!c) return fail(nullptr, CTRAJ_EINVAL, "ctx is NULL");
+  if (ptr) HIPCHK(c, hipFree(ptr));
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_host_alloc(ctraj_ctx *c, uint64_t bytes, void **ptr) {
+  if (!c || !ptr) return fail(c, CTRAJ_EINVAL, "null argument");
+  *ptr = nullptr;
+  hipError_t e = hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault);
+  if (e != hipSuccess) return fail(c, CTRAJ_ENOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_host_free(ctraj_ctx *c, void *ptr) {
+  if (!c) return fail(nullptr, CTRAJ_EINVAL, "ctx is NULL");
+  if (ptr) HIPCHK(c, hipHostFree(ptr));
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_memcpy(ctraj_ctx *c, void *dst, const void *src, uint64_t bytes, int32_t kind) {
+  if (!c) return fail(nullptr, CTRAJ_EINVAL, "ctx is NULL");
+  if (bytes == 0) return CTRAJ_OK;
+  if (!dst || !src) return fail(c, CTRAJ_EINVAL, "null pointer");
+  hipMemcpyKind k = kind == CTRAJ_H2D ? hipMemcpyHostToDevice
+                    : kind == CTRAJ_D2H ? hipMemcpyDeviceToHost
+                    : kind == CTRAJ_D2D ? hipMemcpyDeviceToDevice
+                                        : hipMemcpyDefault;
+  if (kind < 1 || kind > 3) return fail(c, CTRAJ_EINVAL, "bad copy kind");
+  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, k, c->stream));
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_fill_u8(ctraj_ctx *c, void *dst, uint8_t v, uint64_t bytes) {
+  if (!c) return fail(nullptr, CTRAJ_EINVAL, "ctx is NULL");
+  if (bytes == 0) return CTRAJ_OK;
+  if (!dst) return fail(c, CTRAJ_EINVAL, "null pointer");
+  const int grid = (int)std::min<uint64_t>((bytes + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u8, dim3(grid), dim3(256), 0, c->stream, (uint8_t *)dst, v, (int64_t)bytes);
+  HIPCHK(c, hipGetLastError());
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_fill_i64(ctraj_ctx *c, int64_t *dst, int64_t v, int64_t n) {
+  if (!c) return fail(nullptr, CTRAJ_EINVAL, "ctx is NULL");
+  if (n <= 0) return CTRAJ_OK;
+  if (!dst) return fail(c, CTRAJ_EINVAL, "null pointer");
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_i64, dim3(grid), dim3(256), 0, c->stream, dst, v, n);
+  HIPCHK(c, hipGetLastError());
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_sync(ctraj_ctx *c) {
+  if (!c) return fail(nullptr, CTRAJ_EINVAL, "ctx is NULL");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_event_record(ctraj_ctx *c, int32_t slot) {
+  if (!c || slot < 0 || slot >= 64) return fail(c, CTRAJ_EINVAL, "bad event slot");
+  if (!c->ev[slot]) HIPCHK(c, hipEventCreate(&c->ev[slot]));
+  HIPCHK(c, hipEventRecord(c->ev[slot], c->stream));
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_event_elapsed(ctraj_ctx *c, int32_t a, int32_t b, float *ms) {
+  if (!c || !ms || a < 0 || a >= 64 || b < 0 || b >= 64 || !c->ev[a] || !c->ev[b])
+    return fail(c, CTRAJ_EINVAL, "bad event slots");
+  HIPCHK(c, hipEventSynchronize(c->ev[b]));
+  HIPCHK(c, hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_set_beamline(ctraj_ctx *c, const ctraj_elem *elems, int32_t n, const ctraj_phys *phys) {
+  if (!c || !phys || (n > 0 && !elems)) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (n < 0 || n > CTRAJ_MAX_ELEMS) return fail(c, CTRAJ_EINVAL, "n_elems out of range");
+  int rc = check_phys(c, phys);
+  if (rc) return rc;
+  int f, in;
+  rc = check_elems(c, elems, n, phys, &f, &in);
+  if (rc) return rc;
+  c->h_elems.assign(elems, elems + n);
+  c->n_elems = n;
+  c->phys = *phys;
+  c->force = f;
+  c->integ = in;
+  c->have_beamline = true;
+  return upload_elems(c);
+}
+
+int32_t ctraj_set_rtable(ctraj_ctx *c, int32_t id, const double *r, const double *a, int32_t n) {
+  if (!c || !r || !a) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (id < 0 || id >= 8) return fail(c, CTRAJ_EINVAL, "r-table id out of range (0..7)");
+  if (n < 2) return fail(c, CTRAJ_EINVAL, "r-table needs at least 2 knots");
+  for (int j = 0; j < n; ++j) {
+    if (!std::isfinite(r[j]) || !std::isfinite(a[j])) return fail(c, CTRAJ_EINVAL, "r-table has non-finite entries");
+    if (j > 0 && !(r[j] > r[j - 1])) return fail(c, CTRAJ_EINVAL, "r-table knots must be strictly increasing");
+  }
+  std::vector<double> tb((size_t)n * 4, 0.0);
+  for (int j = 0; j < n; ++j) {
+    tb[4 * j] = r[j];
+    tb[4 * j + 1] = a[j];
+    tb[4 * j + 2] = j + 1 < n ? (a[j + 1] - a[j]) / (r[j + 1] - r[j]) : 0.0;
+  }
+  c->rt_host[id] = tb;
+  // rebuild the concatenated device table
+  int total = 0;
+  for (int k = 0; k < 8; ++k) {
+    c->rt_off[k] = total;
+    c->rt_n[k] = (int)(c->rt_host[k].size() / 4);
+    c->rt_invh[k] = c->rt_n[k] >= 2 ? 1.0 / (c->rt_host[k][4] - c->rt_host[k][0]) : 0.0;
+    total += c->rt_n[k];
+  }
+  std::vector<double> all((size_t)total * 4);
+  for (int k = 0; k < 8; ++k)
+    std::copy(c->rt_host[k].begin(), c->rt_host[k].end(), all.begin() + (size_t)c->rt_off[k] * 4);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->d_rt) HIPCHK(c, hipFree(c->d_rt));
+  c->d_rt = nullptr;
+  HIPCHK(c, hipMalloc((void **)&c->d_rt, all.size() * sizeof(double)));
+  HIPCHK(c, hipMemcpy(c->d_rt, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->rt_total = total;
+  return upload_elems(c);
+}
+
+int32_t ctraj_set_wspline(ctraj_ctx *c, int32_t n_states, int32_t n_seg, const double *coef) {
+  if (!c || !coef) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (n_states < 1 || n_states > CTRAJ_MAX_STATES) return fail(c, CTRAJ_EINVAL, "n_states out of range");
+  if (n_seg < 1) return fail(c, CTRAJ_EINVAL, "n_seg < 1");
+  const size_t cnt = (size_t)n_states * n_seg * 4;
+  for (size_t i = 0; i < cnt; ++i)
+    if (!std::isfinite(coef[i])) return fail(c, CTRAJ_EINVAL, "W spline has non-finite coefficients");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->d_ws) HIPCHK(c, hipFree(c->d_ws));
+  c->d_ws = nullptr;
+  HIPCHK(c, hipMalloc((void **)&c->d_ws, cnt * sizeof(double)));
+  HIPCHK(c, hipMemcpy(c->d_ws, coef, cnt * sizeof(double), hipMemcpyHostToDevice));
+  c->ws_states = n_states;
+  c->ws_nseg = n_seg;
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
+  if (!c || !R) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (!c->have_beamline) return fail(c, CTRAJ_ESTATE, "no beamline set");
+  if (R->n < 0) return fail(c, CTRAJ_EINVAL, "n < 0");
+  if (R->n == 0) return CTRAJ_OK;
+  if (!R->in.x || !R->in.y || !R->in.vx || !R->in.vy || !R->in.vz) return fail(c, CTRAJ_EINVAL, "missing inputs");
+  if (R->aoi_mask && (!R->aoi_count || !R->aoi_index || R->aoi_cap < 0))
+    return fail(c, CTRAJ_EINVAL, "aoi compaction needs aoi_count, aoi_index and aoi_cap");
+  if (c->phys.n_labels < 64 && (R->aoi_mask >> c->phys.n_labels))
+    return fail(c, CTRAJ_EINVAL, "aoi_mask names labels beyond n_labels");
+  const int block = R->block ? R->block : 256;
+  if (block < 64 || block > 256 || block % 64) return fail(c, CTRAJ_EINVAL, "block must be 64, 128, 192 or 256");
+  int rc0 = check_tables(c, c->h_elems.data(), c->n_elems, c->phys.n_states);
+  if (rc0) return rc0;
+  PropFn k = prop_kernel(c->force, c->integ);
+  if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
+  KTab kt = ctx_ktab(c);
+  if (c->force == CTRAJ_FORCE_RTABLE) kt.ws_states = 0;
+  const size_t lds = ktab_lds_bytes(kt);
+  int rc = lds_limit(c, (const void *)k, lds);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int grid = grid_for(c, (const void *)k, block, lds, R->n, R->grid);
+  KBeam B = make_kbeam(c->d_elems, c->n_elems, c->phys);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, *R);
+  HIPCHK(c, hipGetLastError());
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_record_trajectories(ctraj_ctx *c, const ctraj_record *RC) {
+  if (!c || !RC) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (!c->have_beamline) return fail(c, CTRAJ_ESTATE, "no beamline set");
+  if (RC->n < 0 || RC->cap < 0) return fail(c, CTRAJ_EINVAL, "negative size");
+  if (RC->n == 0) return CTRAJ_OK;
+  if (!RC->in.x || !RC->in.y || !RC->in.vx || !RC->in.vy || !RC->in.vz || !RC->buf)
+    return fail(c, CTRAJ_EINVAL, "missing inputs or buffer");
+  if (RC->mode != CTRAJ_REC_ROWS && RC->mode != CTRAJ_REC_ZGRID) return fail(c, CTRAJ_EINVAL, "bad record mode");
+  if (RC->mode == CTRAJ_REC_ZGRID && !(RC->z_step > 0.0)) return fail(c, CTRAJ_EINVAL, "z_step must be > 0");
+  const int block = RC->block ? RC->block : 128;
+  if (block < 64 || block > 256 || block % 64) return fail(c, CTRAJ_EINVAL, "block must be 64..256, multiple of 64");
+  int rc0 = check_tables(c, c->h_elems.data(), c->n_elems, c->phys.n_states);
+  if (rc0) return rc0;
+  RecFn k = rec_kernel(c->force, c->integ, RC->mode);
+  KTab kt = ctx_ktab(c);
+  if (c->force == CTRAJ_FORCE_RTABLE) kt.ws_states = 0;
+  const size_t lds = ktab_lds_bytes(kt);
+  int rc = lds_limit(c, (const void *)k, lds);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  // NaN prefill of the whole buffer (rows past a molecule's end stay NaN, like Trajectory's np.full(nan))
+  const int nf = RC->mode == CTRAJ_REC_ROWS ? 10 : 6;
+  int64_t nan_bits;
+  const double qnan = std::nan("");
+  std::memcpy(&nan_bits, &qnan, 8);
+  rc = ctraj_fill_i64(c, (int64_t *)RC->buf, nan_bits, (int64_t)nf * RC->cap * RC->n);
+  if (rc) return rc;
+  const int grid = grid_for(c, (const void *)k, block, lds, RC->n, 0);
+  KBeam B = make_kbeam(c->d_elems, c->n_elems, c->phys);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, *RC);
+  HIPCHK(c, hipGetLastError());
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_draw_gaussian(ctraj_ctx *c, uint64_t seed, uint64_t offset, int64_t n, int32_t nd, const double *mean,
+                            const double *sigma, double *const *out) {
+  if (!c || !mean || !sigma || !out) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (nd < 1 || nd > 8) return fail(c, CTRAJ_EINVAL, "n_dims must be 1..8");
+  if (n < 0) return fail(c, CTRAJ_EINVAL, "n < 0");
+  if (n == 0) return CTRAJ_OK;
+  DrawArgs A{};
+  for (int d = 0; d < nd; ++d) {
+    A.mean[d] = mean[d];
+    A.sigma[d] = sigma[d];
+    A.out[d] = out[d];
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->n_cu * 16);
+  hipLaunchKernelGGL(k_draw, dim3(grid), dim3(256), 0, c->stream, seed, offset, n, nd, A);
+  HIPCHK(c, hipGetLastError());
+  return CTRAJ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,452 @@
+// ctraj_device.hpp -- device-side building blocks of the gfx950 trajectory engine.
+//
+// One thread owns one molecule.  Its phase space lives in VGPRs as FP64; the element table is
+// wave-uniform (scalar loads); lens tables are staged once per workgroup into LDS.
+//
+// Floating-point contract: the library is compiled with -ffp-contract=off, so every a*b+c below
+// that is not written as __builtin_fma rounds twice, exactly as numpy does in the reference.
+// The explicit FMAs (config-mode lens only) are mirrored one for one by the CPU oracle, which
+// makes the device results reproducible bit for bit on the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ctraj.h"
+
+namespace ctraj {
+
+// Molecule state plus the acceleration stored with its current trajectory row (molecule.py:46-50).
+struct MState {
+  double x, y, z, vx, vy, vz, t;
+  double ax, ay, az;
+};
+
+// Lens tables as seen from one workgroup (LDS) or one host loop.
+struct Tables {
+  const double *ws;  // W' spline: [state][nseg][4] = {A, B, C, 0}
+  int nseg;
+  double umax;       // = nseg; |E| beyond the table is clamped to its last knot
+  const double *rt;  // r tables: [knot][4] = {r, a, slope, 0}, all tables back to back
+};
+
+// Where one r table sits in Tables::rt.  The host resolves a lens's table id into these before
+// launch: ctraj_elem.table = knot offset, .reserved = knot count, .p[6] = 1/(r_1 - r_0).
+struct RDesc {
+  int off, n;
+  double invh;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Constant-acceleration drift: Molecule.update_trajectory(delta_t) (molecule.py:58-68) using
+// Molecule.x(dt) = x + v*dt + a*dt**2/2 and v(dt) = v + a*dt (molecule.py:26-44), with `a` the
+// acceleration stored on the current row.  `if not delta_t` returns the current state (:31,41).
+// The new row stores a = (0, -g, 0) (:58).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ void drift_dt(MState &m, double dt, double g) {
+  if (dt != 0.0) {
+    const double dt2 = dt * dt;
+    const double nx = (m.x + m.vx * dt) + (m.ax * dt2) * 0.5;
+    const double ny = (m.y + m.vy * dt) + (m.ay * dt2) * 0.5;
+    const double nz = (m.z + m.vz * dt) + (m.az * dt2) * 0.5;
+    m.vx = m.vx + m.ax * dt;
+    m.vy = m.vy + m.ay * dt;
+    m.vz = m.vz + m.az * dt;
+    m.x = nx;
+    m.y = ny;
+    m.z = nz;
+  }
+  m.t = m.t + dt;
+  m.ax = 0.0;
+  m.ay = -g;
+  m.az = 0.0;
+}
+
+// time to the plane z = zf: delta_t = (z - x_z) / v_z (apertures.py:103)
+__device__ __forceinline__ double time_to(const MState &m, double zf) { return (zf - m.z) / m.vz; }
+
+__device__ __forceinline__ double rho_of(double x, double y) { return __builtin_sqrt(x * x + y * y); }
+
+// ---------------------------------------------------------------------------------------------
+// Lens forces.  Each returns the acceleration (ax, ay) at transverse position (x, y); a_z is 0.
+// ---------------------------------------------------------------------------------------------
+
+// np.interp on an ascending knot table (the interp1d of electrostatic_lens.py:209, evaluated at
+// :217): j with r_j <= r < r_{j+1}; r == r_j returns a_j; r == r_last returns a_last; r above the
+// table is flagged (the reference raises ValueError there) and returns a_last.
+__device__ __forceinline__ double rtable_eval(const Tables &T, const RDesc &D, double r, uint32_t &status) {
+  const double *tb = T.rt + 4 * D.off;
+  const int n = D.n;
+  const double rlast = tb[4 * (n - 1)];
+  if (r < tb[0]) {
+    status |= CTRAJ_ST_OOT;
+    return tb[1];
+  }
+  if (!(r < rlast)) {
+    if (r == rlast) return tb[4 * (n - 1) + 1];
+    if (r != r) return r;  // NaN propagates
+    status |= CTRAJ_ST_OOT;
+    return tb[4 * (n - 1) + 1];
+  }
+  int j = (int)((r - tb[0]) * D.invh);
+  j = j < 0 ? 0 : (j > n - 2 ? n - 2 : j);
+  while (j > 0 && r < tb[4 * j]) --j;
+  while (j < n - 2 && r >= tb[4 * (j + 1)]) ++j;
+  const double rj = tb[4 * j];
+  const double aj = tb[4 * j + 1];
+  if (r == rj) return aj;
+  return tb[4 * j + 2] * (r - rj) + aj;
+}
+
+// ElectrostaticLens.lens_acceleration (electrostatic_lens.py:168-228) with a tabulated a_r(r):
+// a = (a_r x / r, a_r y / r - g, 0), and (0, -g, 0) on axis.
+__device__ __forceinline__ void force_rtable(const Tables &T, const RDesc &D, double g, double x, double y,
+                                             double &ax, double &ay, uint32_t &status) {
+  const double r = rho_of(x, y);
+  const double ar = rtable_eval(T, D, r, status);
+  double fx = 0.0, fy = 0.0;
+  if (r != 0.0) {
+    fx = (ar * x) / r;
+    fy = (ar * y) / r;
+  }
+  ax = fx;
+  ay = fy - g;
+}
+
+// W'(u) from the LDS spline of one state; u = |E|/dE.  Segment k = trunc(u) clamped to the last
+// segment, s = u - k (u is first clamped to the table end, so the force saturates beyond it).
+__device__ __forceinline__ double wspline_eval(const Tables &T, int state, double u) {
+  u = __builtin_fmin(u, T.umax);
+  int k = (int)u;
+  k = k < T.nseg - 1 ? k : T.nseg - 1;
+  const double s = u - (double)k;
+  const double *c = T.ws + 4 * ((size_t)state * T.nseg + k);
+  const double2 ab = *reinterpret_cast<const double2 *>(c);
+  const double cc = c[2];
+  return __builtin_fma(__builtin_fma(ab.x, s, ab.y), s, cc);
+}
+
+// Hexapole: |E| = cE r^2, a = -W'(|E|) 2 cE (x, y) / m - g y_hat.  r2 = x^2 + y^2 is passed in
+// so that the bore test of the previous step can share it.
+__device__ __forceinline__ void force_hex_r2(const Tables &T, int state, double ks, double cu, double g,
+                                             double x, double y, double r2, double &ax, double &ay) {
+  const double f = wspline_eval(T, state, r2 * cu) * ks;
+  ax = f * x;
+  ay = __builtin_fma(f, y, -g);
+}
+
+// Linear-field model |E| = cE r (the reference's field assumption, electrostatic_lens.py:199-200):
+// a = -W'(|E|) cE (x, y) / (m r) - g y_hat, and (0, -g) on axis.
+__device__ __forceinline__ void force_lin(const Tables &T, int state, double ks, double cu, double g,
+                                          double x, double y, double &ax, double &ay) {
+  const double r = rho_of(x, y);
+  double fx = 0.0, fy = 0.0;
+  if (r != 0.0) {
+    const double f = (wspline_eval(T, state, r * cu) * ks) / r;
+    fx = f * x;
+    fy = f * y;
+  }
+  ax = fx;
+  ay = fy - g;
+}
+
+template <int FORCE>
+struct Force;
+
+template <>
+struct Force<CTRAJ_FORCE_RTABLE> {
+  RDesc d;
+  double g;
+  __device__ __forceinline__ void operator()(const Tables &T, int /*state*/, double x, double y, double &ax,
+                                             double &ay, uint32_t &st) const {
+    force_rtable(T, d, g, x, y, ax, ay, st);
+  }
+};
+
+template <>
+struct Force<CTRAJ_FORCE_WSPLINE_HEX> {
+  double ks, cu, g;
+  __device__ __forceinline__ void operator()(const Tables &T, int state, double x, double y, double &ax,
+                                             double &ay, uint32_t &) const {
+    force_hex_r2(T, state, ks, cu, g, x, y, __builtin_fma(x, x, y * y), ax, ay);
+  }
+};
+
+template <>
+struct Force<CTRAJ_FORCE_WSPLINE_LIN> {
+  double ks, cu, g;
+  __device__ __forceinline__ void operator()(const Tables &T, int state, double x, double y, double &ax,
+                                             double &ay, uint32_t &) const {
+    force_lin(T, state, ks, cu, g, x, y, ax, ay);
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Trajectory sinks.  The walker reports every row the reference would append to its Trajectory
+// arrays (molecule.py:133-144, electrostatic_lens.py:106-111).  NullSink compiles to nothing.
+// ---------------------------------------------------------------------------------------------
+struct NullSink {
+  static constexpr bool kRows = false;
+  __device__ __forceinline__ void row(const MState &) {}
+};
+
+// ---------------------------------------------------------------------------------------------
+// Element walk: Beamline.propagate_through (beamline.py:20-35) over the z0-sorted element table.
+// Returns with `label` set (label_detected if the molecule survives) and m at its final row.
+// ---------------------------------------------------------------------------------------------
+struct WalkCounters {
+  uint32_t faces;
+  uint32_t steps;
+};
+
+template <int FORCE>
+__device__ __forceinline__ Force<FORCE> make_force(const ctraj_elem &E, double g);
+
+template <>
+__device__ __forceinline__ Force<CTRAJ_FORCE_RTABLE> make_force<CTRAJ_FORCE_RTABLE>(const ctraj_elem &E, double g) {
+  return Force<CTRAJ_FORCE_RTABLE>{RDesc{E.table, E.reserved, E.p[6]}, g};
+}
+template <>
+__device__ __forceinline__ Force<CTRAJ_FORCE_WSPLINE_HEX> make_force<CTRAJ_FORCE_WSPLINE_HEX>(const ctraj_elem &E,
+                                                                                           double g) {
+  return Force<CTRAJ_FORCE_WSPLINE_HEX>{E.p[4], E.p[5], g};
+}
+template <>
+__device__ __forceinline__ Force<CTRAJ_FORCE_WSPLINE_LIN> make_force<CTRAJ_FORCE_WSPLINE_LIN>(const ctraj_elem &E,
+                                                                                           double g) {
+  return Force<CTRAJ_FORCE_WSPLINE_LIN>{E.p[4], E.p[5], g};
+}
+
+// The reference's lens step (electrostatic_lens.py:91-118): RK4 with the second stage at the
+// full Euler point, state stored with a = l1.  Returns false when the molecule leaves the bore.
+template <int FORCE, class Sink>
+__device__ __forceinline__ bool lens_ref(const ctraj_elem &E, const Tables &T, int state, double g, MState &m,
+                                         uint32_t &status, WalkCounters &wc, Sink &sink) {
+  const Force<FORCE> F = make_force<FORCE>(E, g);
+  const double half_d = E.p[0];
+  const int N = E.n_steps;
+  const double dt = E.p[2] / m.vz;
+  for (int i = 0; i < N; ++i) {
+    const double x = m.x, y = m.y, z = m.z;
+    const double k1x = m.vx, k1y = m.vy, k1z = m.vz;
+    double l1x, l1y;
+    F(T, state, x, y, l1x, l1y, status);
+    const double k2x = k1x + (dt * l1x) / 2.0;
+    const double k2y = k1y + (dt * l1y) / 2.0;
+    const double k2z = k1z + (dt * 0.0) / 2.0;
+    double l2x, l2y;
+    F(T, state, x + dt * k1x, y + dt * k1y, l2x, l2y, status);
+    const double k3x = k1x + (dt * l2x) / 2.0;
+    const double k3y = k1y + (dt * l2y) / 2.0;
+    const double k3z = k1z + (dt * 0.0) / 2.0;
+    double l3x, l3y;
+    F(T, state, x + (dt * k2x) / 2.0, y + (dt * k2y) / 2.0, l3x, l3y, status);
+    const double k4x = k1x + dt * l3x;
+    const double k4y = k1y + dt * l3y;
+    const double k4z = k1z + dt * 0.0;
+    double l4x, l4y;
+    F(T, state, x + dt * k3x, y + dt * k3y, l4x, l4y, status);
+    m.x = x + (dt * (((k1x + 2.0 * k2x) + 2.0 * k3x) + k4x)) / 6.0;
+    m.y = y + (dt * (((k1y + 2.0 * k2y) + 2.0 * k3y) + k4y)) / 6.0;
+    m.z = z + (dt * (((k1z + 2.0 * k2z) + 2.0 * k3z) + k4z)) / 6.0;
+    m.vx = k1x + (dt * (((l1x + 2.0 * l2x) + 2.0 * l3x) + l4x)) / 6.0;
+    m.vy = k1y + (dt * (((l1y + 2.0 * l2y) + 2.0 * l3y) + l4y)) / 6.0;
+    m.vz = k1z + (dt * (((0.0 + 2.0 * 0.0) + 2.0 * 0.0) + 0.0)) / 6.0;
+    m.t = m.t + dt;
+    m.ax = l1x;
+    m.ay = l1y;
+    m.az = 0.0;
+    ++wc.steps;
+    if constexpr (Sink::kRows) sink.row(m);
+    if (rho_of(m.x, m.y) > half_d) return false;
+  }
+  return true;
+}
+
+// Classical RK4 on the transverse state (x, y, vx, vy) with step h; vz is constant in the lens.
+// hh = h/2 and h6 = h/6 are passed in.  The stage slopes are folded into running sums
+//   S_x = ((v1 + 2 v2) + 2 v3) + v4,   S_v = ((a1 + 2 a2) + 2 a3) + a4
+// as soon as each stage is known, which keeps few FP64 values live.  Every FMA here is
+// mirrored by the CPU oracle.
+template <int FORCE>
+__device__ __forceinline__ void rk4_step(const Force<FORCE> &F, const Tables &T, int state, double h, double hh,
+                                         double h6, double &x, double &y, double &vx, double &vy, double r2,
+                                         uint32_t &status) {
+  double ax, ay;
+  if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX) {
+    force_hex_r2(T, state, F.ks, F.cu, F.g, x, y, r2, ax, ay);
+  } else {
+    F(T, state, x, y, ax, ay, status);
+  }
+  // stage 2 point
+  double xs = __builtin_fma(hh, vx, x), ys = __builtin_fma(hh, vy, y);
+  double ux = __builtin_fma(hh, ax, vx), uy = __builtin_fma(hh, ay, vy);
+  double svx = ax, svy = ay;
+  F(T, state, xs, ys, ax, ay, status);
+  double sx = __builtin_fma(2.0, ux, vx), sy = __builtin_fma(2.0, uy, vy);
+  svx = __builtin_fma(2.0, ax, svx);
+  svy = __builtin_fma(2.0, ay, svy);
+  // stage 3 point
+  xs = __builtin_fma(hh, ux, x);
+  ys = __builtin_fma(hh, uy, y);
+  ux = __builtin_fma(hh, ax, vx);
+  uy = __builtin_fma(hh, ay, vy);
+  F(T, state, xs, ys, ax, ay, status);
+  sx = __builtin_fma(2.0, ux, sx);
+  sy = __builtin_fma(2.0, uy, sy);
+  svx = __builtin_fma(2.0, ax, svx);
+  svy = __builtin_fma(2.0, ay, svy);
+  // stage 4 point
+  xs = __builtin_fma(h, ux, x);
+  ys = __builtin_fma(h, uy, y);
+  ux = __builtin_fma(h, ax, vx);
+  uy = __builtin_fma(h, ay, vy);
+  F(T, state, xs, ys, ax, ay, status);
+  sx = sx + ux;
+  sy = sy + uy;
+  svx = svx + ax;
+  svy = svy + ay;
+  x = __builtin_fma(h6, sx, x);
+  y = __builtin_fma(h6, sy, y);
+  vx = __builtin_fma(h6, svx, vx);
+  vy = __builtin_fma(h6, svy, vy);
+}
+
+// Number of full RK4 steps in a lens of length L at axial speed vz: floor(L / (vz dt)), 0 for a
+// non-finite or negative quotient, clipped to cap (CTRAJ_ST_CAPPED).
+__device__ __forceinline__ int rk4_full_steps(double L, double vz, double dt, int cap, uint32_t &status) {
+  const double q = __builtin_floor(L / (vz * dt));
+  if (!(q >= 0.0)) return 0;
+  if (q > (double)cap) {
+    status |= CTRAJ_ST_CAPPED;
+    return cap;
+  }
+  return (int)q;
+}
+
+// Config-mode lens interior: n = floor(L/(vz dt)) steps of dt, then one step of L/vz - n dt, so
+// the molecule leaves at z1.  Bore test x^2 + y^2 > (d/2)^2 after every step.  Rows store the
+// force at the row's position; the exit row stores (0, -g, 0).
+template <int FORCE, class Sink>
+__device__ __forceinline__ bool lens_rk4(const ctraj_elem &E, const Tables &T, int state, double g, int cap,
+                                         MState &m, uint32_t &status, WalkCounters &wc, Sink &sink) {
+  const Force<FORCE> F = make_force<FORCE>(E, g);
+  const double R2 = E.p[1];
+  const double h = E.p[2];
+  const double L = E.p[3];
+  const double hh = 0.5 * h, h6 = h / 6.0;
+  const double dzs = m.vz * h;
+  const int nfull = rk4_full_steps(L, m.vz, h, cap, status);
+  double x = m.x, y = m.y, vx = m.vx, vy = m.vy;
+  double z = m.z, t = m.t;
+  double r2 = __builtin_fma(x, x, y * y);
+  bool alive = true;
+  for (int i = 0; i <= nfull; ++i) {
+    if (i < nfull) {
+      rk4_step<FORCE>(F, T, state, h, hh, h6, x, y, vx, vy, r2, status);
+      z = z + dzs;
+      t = t + h;
+    } else {
+      const double hp = L / m.vz - (double)nfull * h;
+      rk4_step<FORCE>(F, T, state, hp, 0.5 * hp, hp / 6.0, x, y, vx, vy, r2, status);
+      z = E.z1;
+      t = t + hp;
+    }
+    ++wc.steps;
+    r2 = __builtin_fma(x, x, y * y);
+    if constexpr (Sink::kRows) {
+      MState r = m;
+      r.x = x; r.y = y; r.z = z; r.vx = vx; r.vy = vy; r.t = t;
+      if (i < nfull || r2 > R2) {
+        F(T, state, x, y, r.ax, r.ay, status);
+      } else {
+        r.ax = 0.0; r.ay = -g;
+      }
+      r.az = 0.0;
+      sink.row(r);
+    }
+    if (r2 > R2) {
+      alive = false;
+      break;
+    }
+  }
+  m.x = x; m.y = y; m.vx = vx; m.vy = vy; m.z = z; m.t = t;
+  if (alive) {
+    m.ax = 0.0; m.ay = -g; m.az = 0.0;
+  } else {
+    uint32_t st = 0;
+    F(T, state, x, y, m.ax, m.ay, st);
+    m.az = 0.0;
+  }
+  return alive;
+}
+
+template <int FORCE, int INTEG, class Sink>
+__device__ __forceinline__ void walk(const ctraj_elem *__restrict__ elems, int n_elems, const Tables &T, double g,
+                                     int cap, int state, int label_detected, MState &m, int &label, uint32_t &status,
+                                     WalkCounters &wc, Sink &sink) {
+  if constexpr (Sink::kRows) sink.row(m);
+  label = label_detected;
+  for (int e = 0; e < n_elems; ++e) {
+    const ctraj_elem E = elems[e];
+    if (E.kind == CTRAJ_CIRC) {
+      // CircularAperture.propagate_through (apertures.py:92-115): test at z0 and z1; x0, y0 unused
+      const double half_d = E.p[0];
+      drift_dt(m, time_to(m, E.z0), g);
+      ++wc.faces;
+      if constexpr (Sink::kRows) sink.row(m);
+      if (rho_of(m.x, m.y) > half_d) { label = E.label_hit; return; }
+      drift_dt(m, time_to(m, E.z1), g);
+      ++wc.faces;
+      if constexpr (Sink::kRows) sink.row(m);
+      if (rho_of(m.x, m.y) > half_d) { label = E.label_hit; return; }
+    } else if (E.kind == CTRAJ_RECT) {
+      // RectangularAperture.propagate_through (apertures.py:165-189): strict box test
+      const double x1 = E.p[0], x2 = E.p[1], y1 = E.p[2], y2 = E.p[3];
+      drift_dt(m, time_to(m, E.z0), g);
+      ++wc.faces;
+      if constexpr (Sink::kRows) sink.row(m);
+      if (!((x1 < m.x && m.x < x2) && (y1 < m.y && m.y < y2))) { label = E.label_hit; return; }
+      drift_dt(m, time_to(m, E.z1), g);
+      ++wc.faces;
+      if constexpr (Sink::kRows) sink.row(m);
+      if (!((x1 < m.x && m.x < x2) && (y1 < m.y && m.y < y2))) { label = E.label_hit; return; }
+    } else if (E.kind == CTRAJ_PLATES) {
+      // FieldPlates.propagate_through (apertures.py:227-270): x-bounded plates with wall-hit time
+      const double x1 = E.p[0], x2 = E.p[1];
+      drift_dt(m, time_to(m, E.z0), g);
+      ++wc.faces;
+      if constexpr (Sink::kRows) sink.row(m);
+      if (!(x1 < m.x && m.x < x2)) { label = E.label_hit; return; }
+      double dt = time_to(m, E.z1);
+      double xe = m.x;
+      if (dt != 0.0) xe = (m.x + m.vx * dt) + (m.ax * (dt * dt)) * 0.5;
+      const bool inside = (x1 < xe && xe < x2);
+      if (!inside) {
+        if (m.vx < 0.0) dt = (x1 - m.x) / m.vx;
+        else if (m.vx > 0.0) dt = (x2 - m.x) / m.vx;
+      }
+      drift_dt(m, dt, g);
+      ++wc.faces;
+      if constexpr (Sink::kRows) sink.row(m);
+      if (!inside) { label = E.label_hit; return; }
+    } else if (E.kind == CTRAJ_LENS) {
+      // ElectrostaticLens.propagate_through (electrostatic_lens.py:48-77)
+      drift_dt(m, time_to(m, E.z0), g);
+      ++wc.faces;
+      if constexpr (Sink::kRows) sink.row(m);
+      if (rho_of(m.x, m.y) > E.p[0]) { label = E.label_hit; return; }
+      if constexpr (INTEG == CTRAJ_INTEG_REF) {
+        if (!lens_ref<FORCE>(E, T, state, g, m, status, wc, sink)) { label = E.label_inside; return; }
+        // exit hop with the stored a = l1 of the last step (electrostatic_lens.py:74-77)
+        drift_dt(m, time_to(m, E.z1), g);
+        ++wc.faces;
+        if constexpr (Sink::kRows) sink.row(m);
+      } else {
+        if (!lens_rk4<FORCE>(E, T, state, g, cap, m, status, wc, sink)) { label = E.label_inside; return; }
+      }
+    }
+  }
+}
+
+}  // namespace ctraj

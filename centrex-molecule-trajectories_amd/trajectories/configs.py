@@ -1,0 +1,113 @@
+"""The benchmark beamlines, with every constant the workload descriptions leave open pinned here.
+
+Source (all workloads): x, y ~ N(0, 1.5 mm), z = 0; vx, vy ~ N(0, 1.5 m/s); vz ~ N(184, 16) m/s,
+drawn in the reference's order (per chunk: vx, vy, vz, then x, y).  Gravity 9.81 m/s^2 along -Y.
+Aperture thickness L = 0 (thin).  Lens: hexapole |E| = 3 V0 r^2 / R^3, R = 1.75 cm, molecular mass
+= the reference default (204.38 + 19.00) * 1.67e-27 kg, W(|E|) two-level with nu = 6.67 GHz,
+mu = 4.23 D, tabulated at 1001 knots over 0..100 kV/cm and splined (not-a-knot), classical RK4 at
+dt = 10 us with a final partial step.  Aperture sizes below are radii.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .beamline import Beamline
+from .beamline_elements import CircularAperture, ElectrostaticLens
+from .distributions import CeNTREXVelocityDistribution, GaussianPositionDistribution
+from .stark import TwoLevelStark, WSpline
+
+G = 9.81
+SIGMA_X = 1.5e-3
+SIGMA_V = 1.5
+VZ_MEAN, VZ_SIGMA = 184.0, 16.0
+LENS_R = 1.75e-2
+LENS_L = 0.6
+LENS_DT = 10e-6
+NU = 6.67e9
+MU_D = 4.23
+
+_TABLES: dict = {}
+
+
+def source():
+    """(xdist, vdist) of the benchmark source."""
+    return (GaussianPositionDistribution(SIGMA_X, SIGMA_X, 0.0),
+            CeNTREXVelocityDistribution(0.0, SIGMA_V, 0.0, SIGMA_V, VZ_MEAN, VZ_SIGMA))
+
+
+def draw_source(n: int, seed: int, n_jobs: int = 1):
+    """Host draw of n molecules with np.random.RandomState(seed) in the reference's order.
+    Returns X (3, n), V (3, n).  n must be a multiple of 100*n_jobs (the reference's chunking)."""
+    xdist, vdist = source()
+    rs = np.random.RandomState(seed)
+    loops = 100 * n_jobs
+    N = n // loops
+    if N * loops != n:
+        raise ValueError("n must be a multiple of 100*n_jobs")
+    X = np.empty((3, n))
+    V = np.empty((3, n))
+    for c in range(loops):
+        s = slice(c * N, (c + 1) * N)
+        V[0, s] = vdist.vx + vdist.sigmax * rs.standard_normal(N)
+        V[1, s] = vdist.vy + vdist.sigmay * rs.standard_normal(N)
+        V[2, s] = vdist.vz + vdist.sigmaz * rs.standard_normal(N)
+        X[0, s] = 0.0 + xdist.sigmax * rs.standard_normal(N)
+        X[1, s] = 0.0 + xdist.sigmay * rs.standard_normal(N)
+        X[2, s] = xdist.z
+    return X, V
+
+
+def stark_table(sign: float = -1.0, dipole_scale: float = 1.0) -> WSpline:
+    """W(|E|) spline of the two-level state (cached)."""
+    key = (float(sign), float(dipole_scale))
+    if key not in _TABLES:
+        _TABLES[key] = WSpline.from_stark(TwoLevelStark(NU, MU_D, sign, dipole_scale), 100e3, 1001)
+    return _TABLES[key]
+
+
+def config0() -> Beamline:
+    """Ballistic beamline: apertures r = 6, 4, 2.5 mm at z = 0.3, 0.8, 1.2 m; detector r = 5 mm at 2.0 m."""
+    return Beamline([
+        CircularAperture("Aperture 1", 0.3, 0.0, d=2 * 6e-3),
+        CircularAperture("Aperture 2", 0.8, 0.0, d=2 * 4e-3),
+        CircularAperture("Aperture 3", 1.2, 0.0, d=2 * 2.5e-3),
+        CircularAperture("Detector", 2.0, 0.0, d=2 * 5e-3),
+    ], G)
+
+
+def lens(V0: float = 30e3, z0: float = 0.8, stark=None, integrator: str = "rk4", dt: float = LENS_DT):
+    return ElectrostaticLens("ES lens", z0, LENS_L, d=2 * LENS_R, V=V0, stark=stark or stark_table(),
+                             field="hexapole", integrator=integrator, dt=dt)
+
+
+def config1(V0: float = 30e3, lens_z0: float = 0.8, r1: float = 6e-3, r2: float = 3e-3, r3: float = 4e-3,
+            stark=None) -> Beamline:
+    """ES-lens beamline: aperture r1 at 0.3 m; hexapole lens at z0 (L = 0.6 m); aperture r2 at 1.6 m;
+    detector r3 at 2.5 m."""
+    return Beamline([
+        CircularAperture("Aperture 1", 0.3, 0.0, d=2 * r1),
+        lens(V0, lens_z0, stark),
+        CircularAperture("Aperture 2", 1.6, 0.0, d=2 * r2),
+        CircularAperture("Detector", 2.5, 0.0, d=2 * r3),
+    ], G)
+
+
+def config4_tables():
+    """Four states: the two-level form with sign -1 and +1, and sign -1 with the dipole scaled by 0.5, 0.8."""
+    return [stark_table(-1.0, 1.0), stark_table(+1.0, 1.0), stark_table(-1.0, 0.5), stark_table(-1.0, 0.8)]
+
+
+def config4() -> Beamline:
+    return config1(stark=config4_tables())
+
+
+def config2_params(n_cfg: int = 256, seed: int = 2):
+    """Per-configuration (V0, lens z0, final aperture radius, molecule count), drawn from
+    RandomState(seed) in that order: V0 ~ U[10, 40] kV, z0 ~ U[0.7, 0.9] m, r ~ U[2, 5] mm,
+    n = round(10**U[3, 6])."""
+    rs = np.random.RandomState(seed)
+    V0 = rs.uniform(10e3, 40e3, n_cfg)
+    z0 = rs.uniform(0.7, 0.9, n_cfg)
+    r = rs.uniform(2e-3, 5e-3, n_cfg)
+    n = np.round(10.0 ** rs.uniform(3.0, 6.0, n_cfg)).astype(np.int64)
+    return V0, z0, r, n

@@ -1,0 +1,233 @@
+/*
+ * ctraj.h -- C ABI of libctraj.so, the MI355X (gfx950) FP64 molecular-trajectory engine.
+ *
+ * The reference (otimgren/centrex-molecule-trajectories) is pure Python with no FFI; its hot path is
+ * TrajectorySimulator.run_simulation -> Beamline.propagate_through -> BeamlineElement.propagate_through
+ * -> Molecule.update_trajectory / ElectrostaticLens.propagate_inside_lens.  Every entry point below
+ * replaces one piece of that path and is bound from Python with ctypes (see the `trajectories`
+ * package next to csrc/).  Conventions:
+ *   - every function returns CTRAJ_OK (0) or a negative CTRAJ_E* code and never aborts;
+ *     ctraj_last_error() gives the text of the last failure on that context;
+ *   - every array argument is a DEVICE pointer unless its comment says "host";
+ *   - all work is queued on the context's one HIP stream; ctraj_sync() waits for it;
+ *   - one context per host thread; nothing is retained after a call returns except the
+ *     beamline/tables set on the context and the batch objects.
+ */
+#ifndef CTRAJ_H
+#define CTRAJ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CTRAJ_ABI_VERSION 1
+
+/* return codes */
+#define CTRAJ_OK 0
+#define CTRAJ_EINVAL (-1) /* bad argument (null pointer, size, unsupported element combination) */
+#define CTRAJ_EHIP (-2)   /* HIP runtime error; text in ctraj_last_error */
+#define CTRAJ_ENOMEM (-3) /* device or pinned allocation failed */
+#define CTRAJ_ESTATE (-4) /* call out of order (no beamline / tables set) */
+
+#define CTRAJ_MAX_ELEMS 64
+#define CTRAJ_MAX_LABELS 64
+#define CTRAJ_MAX_STATES 4
+
+/* ---- beamline elements (apertures.py:22-286, electrostatic_lens.py:23-125) ---- */
+#define CTRAJ_CIRC 1   /* CircularAperture   apertures.py:84-122  p[0] = d/2                      */
+#define CTRAJ_RECT 2   /* RectangularAperture apertures.py:148-196 p[0..3] = x1, x2, y1, y2        */
+#define CTRAJ_PLATES 3 /* FieldPlates        apertures.py:214-277 p[0..1] = x1, x2                */
+#define CTRAJ_LENS 4   /* ElectrostaticLens  electrostatic_lens.py:24-125, p[] below              */
+
+/* lens force models */
+#define CTRAJ_FORCE_RTABLE 0      /* radial acceleration a_r(r) from a linear table (electrostatic_lens.py:168-228) */
+#define CTRAJ_FORCE_WSPLINE_HEX 1 /* a = -grad W(|E|)/m with |E| = cE r^2 (hexapole); W'(|E|) from a cubic spline */
+#define CTRAJ_FORCE_WSPLINE_LIN 2 /* same, with |E| = cE r (the reference's linear field model) */
+
+/* lens integrators */
+#define CTRAJ_INTEG_REF 0 /* N = rint(L/dz) steps of dt = dz/vz, the reference's RK4 variant (electrostatic_lens.py:79-118) */
+#define CTRAJ_INTEG_RK4 1 /* classical RK4 at fixed dt, then one partial step of L/vz - n*dt so the lens ends at z1 */
+
+/*
+ * LENS parameters:
+ *   p[0] = d/2 (bore radius)      p[1] = (d/2)^2             p[2] = dz (REF) or dt (RK4)
+ *   p[3] = L                      p[4] = ks force scale      p[5] = cu = cE/dE (|E| -> spline coordinate)
+ *   p[6] = unused                 p[7] = V (electrode voltage; used by the adjoint for d/dV)
+ * WSPLINE_HEX: a = ks * W'(E) * (x, y) with ks = -2 cE / m;  u = r^2 * cu
+ * WSPLINE_LIN: a = ks * W'(E) * (x, y) / r with ks = -cE / m; u = r * cu
+ */
+typedef struct ctraj_elem {
+  int32_t kind;         /* CTRAJ_CIRC .. CTRAJ_LENS */
+  int32_t label_hit;    /* label of a molecule stopped here: element name, or "Lens entrance" for a lens */
+  int32_t label_inside; /* LENS: label "Inside lens" */
+  int32_t force;        /* LENS: CTRAJ_FORCE_* */
+  int32_t integrator;   /* LENS: CTRAJ_INTEG_* */
+  int32_t table;        /* LENS with FORCE_RTABLE: r-table id */
+  int32_t n_steps;      /* LENS with INTEG_REF: int(rint(L/dz)) */
+  int32_t reserved;
+  double z0, z1;
+  double p[8];
+} ctraj_elem;
+
+typedef struct ctraj_phys {
+  double g;               /* gravitational acceleration along -y (reference: scipy.constants.g) */
+  int32_t n_labels;       /* distinct labels, <= CTRAJ_MAX_LABELS */
+  int32_t label_detected; /* label of survivors (beamline.py:34-35) */
+  int32_t n_states;       /* W-spline state tables (1..CTRAJ_MAX_STATES); molecule state ids must be < n_states */
+  int32_t max_lens_steps; /* safety cap on RK4 steps per lens per molecule (0 = 1<<20) */
+} ctraj_phys;
+
+/* structure-of-arrays phase space (device pointers) */
+typedef struct ctraj_soa {
+  double *x, *y, *z, *vx, *vy, *vz, *t;
+} ctraj_soa;
+
+/* per-molecule status bits */
+#define CTRAJ_ST_OOT 1u    /* a lens force evaluation fell outside its table (reference: interp1d ValueError) */
+#define CTRAJ_ST_CAPPED 2u /* RK4 step count clipped to max_lens_steps */
+
+/* work counters (uint64, accumulated) */
+#define CTRAJ_WORK_FACES 0  /* element faces reached (drift + test) */
+#define CTRAJ_WORK_STEPS 1  /* lens integration steps taken */
+#define CTRAJ_WORK_OOT 2    /* molecules with CTRAJ_ST_OOT */
+#define CTRAJ_WORK_CAPPED 3 /* molecules with CTRAJ_ST_CAPPED */
+#define CTRAJ_WORK_N 8
+
+typedef struct ctraj_ctx ctraj_ctx;
+typedef struct ctraj_batch ctraj_batch;
+
+/* ---- context, memory, timing ---- */
+int32_t ctraj_abi_version(void);
+int32_t ctraj_ctx_create(int32_t device, ctraj_ctx **out);
+int32_t ctraj_ctx_destroy(ctraj_ctx *ctx);
+const char *ctraj_last_error(const ctraj_ctx *ctx);
+/* n_cu, max LDS bytes per block, warp size, gcn arch number (950 for gfx950) -- host outputs */
+int32_t ctraj_device_info(ctraj_ctx *ctx, int32_t *n_cu, int32_t *lds_per_block, int32_t *wave, int32_t *arch);
+int32_t ctraj_dev_alloc(ctraj_ctx *ctx, uint64_t bytes, void **ptr);
+int32_t ctraj_dev_free(ctraj_ctx *ctx, void *ptr);
+int32_t ctraj_host_alloc(ctraj_ctx *ctx, uint64_t bytes, void **ptr); /* pinned host memory */
+int32_t ctraj_host_free(ctraj_ctx *ctx, void *ptr);
+#define CTRAJ_H2D 1
+#define CTRAJ_D2H 2
+#define CTRAJ_D2D 3
+int32_t ctraj_memcpy(ctraj_ctx *ctx, void *dst, const void *src, uint64_t bytes, int32_t kind); /* async on the stream */
+int32_t ctraj_fill_u8(ctraj_ctx *ctx, void *dst, uint8_t value, uint64_t bytes);               /* fill kernel */
+int32_t ctraj_fill_i64(ctraj_ctx *ctx, int64_t *dst, int64_t value, int64_t n);                 /* fill kernel */
+int32_t ctraj_sync(ctraj_ctx *ctx);
+/* HIP events on the context stream: record into slot (0..63), elapsed ms between two recorded slots */
+int32_t ctraj_event_record(ctraj_ctx *ctx, int32_t slot);
+int32_t ctraj_event_elapsed(ctraj_ctx *ctx, int32_t slot_a, int32_t slot_b, float *ms);
+
+/* ---- beamline and lens tables ---- */
+/* Beamline(elements) with its stable z0 sort already applied by the caller (beamline.py:17-18,44-45). host arrays */
+int32_t ctraj_set_beamline(ctraj_ctx *ctx, const ctraj_elem *elems, int32_t n_elems, const ctraj_phys *phys);
+/* linear a_r(r) table with np.interp semantics (the interp1d built at electrostatic_lens.py:194-209). host arrays;
+ * r ascending, n >= 2; slopes are formed here as (a[j+1]-a[j])/(r[j+1]-r[j]) */
+int32_t ctraj_set_rtable(ctraj_ctx *ctx, int32_t id, const double *r, const double *a, int32_t n);
+/* W'(|E|) spline for each state: coef[state][seg][4] = {A, B, C, 0}, W'(E) = (A s + B) s + C with
+ * s = u - seg, u = E/dE (host array) */
+int32_t ctraj_set_wspline(ctraj_ctx *ctx, int32_t n_states, int32_t n_seg, const double *coef);
+
+/* ---- forward propagation: replaces run_simulation's per-molecule loop (trajectory_simulator.py:52-78)
+ *      and Beamline.propagate_through (beamline.py:20-38) ---- */
+#define CTRAJ_RUN_STATUS_OR 1 /* OR status bits into run->status instead of overwriting */
+typedef struct ctraj_run {
+  int64_t n;             /* molecules */
+  int64_t index_base;    /* global index of molecule 0 (for first_index / aoi_index in chunked runs) */
+  ctraj_soa in;          /* initial phase space; in.z / in.t may be NULL, then z_const / t_const */
+  double z_const, t_const;
+  const double *a0[3];   /* initial stored acceleration; NULL -> (0, -g, 0) (molecule.py:15-16) */
+  const uint8_t *state;  /* per-molecule state id, NULL -> 0 */
+  ctraj_soa out;         /* final phase space; NULL fields are not written (may alias in) */
+  double *out_a[3];      /* final stored acceleration (optional) */
+  int32_t *label;        /* per-molecule label (optional) */
+  uint8_t *status;       /* per-molecule CTRAJ_ST_* bits (optional) */
+  int64_t *tally;        /* [n_states][n_labels] counts, accumulated (optional) */
+  int64_t *first_index;  /* [n_labels] atomic min of the global index reaching each label (optional) */
+  uint64_t aoi_mask;     /* labels whose molecules are compacted (apertures_of_interest) */
+  int64_t aoi_cap;
+  int64_t *aoi_count;    /* [1] accumulated; may exceed aoi_cap (then entries past the cap are dropped) */
+  int64_t *aoi_index;    /* [aoi_cap] global indices of compacted molecules */
+  int32_t *aoi_label;    /* [aoi_cap] their labels (optional) */
+  ctraj_soa aoi_out;     /* [aoi_cap] their final phase space (optional fields) */
+  uint64_t *work;        /* [CTRAJ_WORK_N] counters, accumulated (optional) */
+  int32_t block;         /* threads per block, multiple of 64 (0 = 256) */
+  int32_t grid;          /* blocks (0 = enough for n, capped at 8 per CU x n_cu x 8) */
+  int32_t flags;
+  int32_t reserved;
+} ctraj_run;
+int32_t ctraj_propagate(ctraj_ctx *ctx, const ctraj_run *run);
+
+/* ---- saved trajectories (molecule.py:111-167 rows; post_processing.py:20-140 resampling) ---- */
+#define CTRAJ_REC_ROWS 1  /* every trajectory row; buf = [10][cap][n]: x,y,z,vx,vy,vz,ax,ay,az,t */
+#define CTRAJ_REC_ZGRID 2 /* samples at z_k = z_start + k*z_step; buf = [6][cap][n]: x,y,vx,vy,vz,t */
+typedef struct ctraj_record {
+  int64_t n;              /* molecules to record */
+  const int64_t *index;   /* indices into `in` (NULL -> 0..n-1) */
+  ctraj_soa in;
+  double z_const, t_const;
+  const double *a0[3];
+  const uint8_t *state;
+  int32_t mode;           /* CTRAJ_REC_* */
+  int32_t cap;            /* rows (ROWS) or samples (ZGRID) per molecule */
+  double z_start, z_step; /* ZGRID */
+  double *buf;            /* see mode; unwritten entries are NaN */
+  int32_t *n_rows;        /* [n] rows produced (ROWS; may exceed cap) or samples written (ZGRID) */
+  int32_t *label;         /* [n] final label (optional) */
+  int32_t block;
+  int32_t reserved;
+} ctraj_record;
+int32_t ctraj_record_trajectories(ctraj_ctx *ctx, const ctraj_record *rec);
+
+/* ---- device source sampling: Distribution.draw (distributions.py:32-162) as Philox4x32-10 + Box-Muller ---- */
+/* out[d][i] = mean[d] + sigma[d] * N(0,1) drawn from counter (i + offset, pair(d)) with key seed; host mean/sigma,
+ * device out[d] (a NULL out[d] skips that dimension) */
+int32_t ctraj_draw_gaussian(ctraj_ctx *ctx, uint64_t seed, uint64_t offset, int64_t n, int32_t n_dims,
+                            const double *mean, const double *sigma, double *const *out);
+
+/* ---- batched scan: many beamline configurations in one persistent, graph-replayed launch ---- */
+typedef struct ctraj_batch_cfg {
+  int32_t elem_begin, n_elems; /* slice of the batch element array (z0-sorted) */
+  int64_t n;                   /* molecules in this configuration */
+  uint64_t seed;               /* device source stream of this configuration */
+  double mean[6], sigma[6];    /* Gaussian source for x, y, z, vx, vy, vz */
+} ctraj_batch_cfg;
+typedef struct ctraj_batch_tune {
+  int32_t block;     /* threads per block */
+  int32_t chunk;     /* molecules per work item (multiple of block) */
+  int32_t grid;      /* persistent blocks (0 = n_cu * blocks_per_cu) */
+  int32_t use_graph; /* replay a captured hipGraph */
+} ctraj_batch_tune;
+/* host arrays; the W spline (ctraj_set_wspline) and r-tables of the context are shared by all configurations */
+int32_t ctraj_batch_create(ctraj_ctx *ctx, const ctraj_elem *elems, int32_t n_elems_total, const ctraj_batch_cfg *cfgs,
+                           int32_t n_cfgs, const ctraj_phys *phys, ctraj_batch **out);
+/* runs every configuration; tally_host [n_cfgs][n_labels] and work_host [CTRAJ_WORK_N] are host outputs (may be NULL);
+ * elapsed_ms (host, may be NULL) is the device time of the launch measured with HIP events */
+int32_t ctraj_batch_run(ctraj_batch *b, const ctraj_batch_tune *tune, int64_t *tally_host, uint64_t *work_host,
+                        float *elapsed_ms);
+int32_t ctraj_batch_destroy(ctraj_batch *b);
+
+/* ---- differentiable detected flux and its adjoint ----
+ * J = (1/n) sum_i prod_{CIRC faces f} sigmoid((r_f - rho_i,f) / softness); no molecule is stopped.
+ * Parameters, in beamline order: each CIRC contributes d/2; each LENS contributes (V, z0); then mean vz.
+ * Only CIRC and LENS(WSPLINE_HEX, INTEG_RK4) elements are supported. */
+typedef struct ctraj_smooth {
+  int64_t n;
+  const double *x, *y, *vx, *vy, *eps; /* source; vz_i = mu_vz + sigma_vz * eps_i; z = z_src, t = 0 */
+  double z_src, mu_vz, sigma_vz;
+  double softness;                     /* sigmoid width in m */
+  int32_t ckpt_every;                  /* lens checkpoint interval K in steps (>= 1) */
+  int32_t block;
+  double *J_host;                      /* host [1] */
+  double *grad_host;                   /* host [n_params] or NULL (objective only) */
+  int32_t n_params;                    /* must equal the parameter count of the beamline */
+  int32_t reserved;
+} ctraj_smooth;
+int32_t ctraj_smooth_objective(ctraj_ctx *ctx, const ctraj_smooth *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CTRAJ_H */
