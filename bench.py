@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--config", choices=["lens", "ballistic"], default="lens")
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--refill", type=int, default=0, help="lane-refill threshold (0 = library default)")
+    ap.add_argument("--simple", action="store_true", help="one molecule per thread (no lane refill)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline wall time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -131,8 +133,9 @@ def main():
     def step():
         aoi_count.fill(0)
         eng.launch(inp, tally=tally, work=work, aoi_mask=1 << det, aoi_cap=cap, aoi_count=aoi_count,
-                   aoi_index=aoi_index, aoi_out=aoi_out, n=n, block=args.block)
+                   aoi_index=aoi_index, aoi_out=aoi_out, n=n, block=args.block, flags=flags, refill=args.refill)
 
+    flags = N.RUN_SIMPLE if args.simple else 0
     for _ in range(args.warmup):
         step()
     ctx.sync()
@@ -148,7 +151,7 @@ def main():
         s = 2 + 2 * (k % 31)
         ctx.event_record(s)
         eng.launch(inp, tally=tally, work=work, aoi_mask=1 << det, aoi_cap=cap, aoi_count=aoi_count,
-                   aoi_index=aoi_index, aoi_out=aoi_out, n=n, block=args.block)
+                   aoi_index=aoi_index, aoi_out=aoi_out, n=n, block=args.block, flags=flags, refill=args.refill)
         ctx.event_record(s + 1)
         if k % 31 == 30:
             kernel_ms += [ctx.event_elapsed_ms(2 + 2 * j, 3 + 2 * j) for j in range(31)]
@@ -194,7 +197,8 @@ def main():
         "config": {"workload": workload, "molecules": n, "outputs": "per-element tallies + compacted detected phase space"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": traffic,
-                     "kernel": "k_propagate<WSPLINE_HEX,RK4>", "kernel_ms_avg": kavg,
+                     "kernel": "k_propagate<WSPLINE_HEX,RK4>" if args.simple or args.config != "lens"
+                     else "k_propagate_lanes<WSPLINE_HEX,RK4>", "kernel_ms_avg": kavg,
                      "flop_per_launch": flop, "faces_per_launch": faces, "rk4_steps_per_launch": steps_},
         "region_ms_per_step": region_ms / K,
         "counts": counts,

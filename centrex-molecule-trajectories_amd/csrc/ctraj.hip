@@ -19,6 +19,7 @@
 
 #include "ctraj_device.hpp"
 #include "ctraj_internal.hpp"
+#include "ctraj_lanes.hpp"
 #include "ctraj_sinks.hpp"
 
 using namespace ctraj;
@@ -151,6 +152,262 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
       }
     }
   }
+  if (R.work) {
+    if (wc.faces) atomicAdd(&s_work[0], (unsigned long long)wc.faces);
+    if (wc.steps) atomicAdd(&s_work[1], (unsigned long long)wc.steps);
+    if (n_oot) atomicAdd(&s_work[2], (unsigned long long)n_oot);
+    if (n_cap) atomicAdd(&s_work[3], (unsigned long long)n_cap);
+  }
+  __syncthreads();
+  if (R.tally)
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+      if (s_hist[b]) atomicAdd((unsigned long long *)&R.tally[b], (unsigned long long)s_hist[b]);
+  if (R.first_index)
+    for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x)
+      if (s_first[b] != (~0ull >> 1)) atomicMin((unsigned long long *)&R.first_index[b], s_first[b]);
+  if (R.work && threadIdx.x < 4 && s_work[threadIdx.x])
+    atomicAdd((unsigned long long *)&R.work[threadIdx.x], s_work[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_propagate_lanes: persistent lane-refill walker (beamlines with exactly one lens)
+// ---------------------------------------------------------------------------------------------
+struct LaneArgs {
+  unsigned long long *counter;  // next unclaimed molecule (zeroed before launch)
+  int lens_e;                   // index of the lens in the element table
+  int refill_min;               // idle lanes that trigger a refill section
+  int claim;                    // molecules claimed per atomic
+};
+
+// Record the molecules of lanes with s.fin set (convergent call).  A molecule stopped by the bore
+// gets its stored acceleration (the force at the stopping point) here, only if it is asked for.
+template <int FORCE>
+__device__ __forceinline__ void record_finished(const KBeam &B, const ctraj_run &R, LaneMol &s, uint32_t *s_hist,
+                                                unsigned long long *s_first, int lane, uint32_t &n_oot,
+                                                uint32_t &n_cap, const Force<FORCE> &F, const Tables &T) {
+  if (!__ballot(s.fin)) return;
+  if (s.fin) {
+    const int64_t i = s.gidx;
+    MState &m = s.m;
+    if (s.fin == 2 && R.out_a[0]) {
+      uint32_t st2 = 0;
+      F(T, s.st, m.x, m.y, m.ax, m.ay, st2);
+      m.az = 0.0;
+    }
+    if (R.out.x) R.out.x[i] = m.x;
+    if (R.out.y) R.out.y[i] = m.y;
+    if (R.out.z) R.out.z[i] = m.z;
+    if (R.out.vx) R.out.vx[i] = m.vx;
+    if (R.out.vy) R.out.vy[i] = m.vy;
+    if (R.out.vz) R.out.vz[i] = m.vz;
+    if (R.out.t) R.out.t[i] = m.t;
+    if (R.out_a[0]) R.out_a[0][i] = m.ax;
+    if (R.out_a[1]) R.out_a[1][i] = m.ay;
+    if (R.out_a[2]) R.out_a[2][i] = m.az;
+    if (R.label) R.label[i] = s.label;
+    if (R.status) R.status[i] = (R.flags & CTRAJ_RUN_STATUS_OR) ? (uint8_t)(R.status[i] | s.status) : (uint8_t)s.status;
+    n_oot += (s.status & CTRAJ_ST_OOT) ? 1u : 0u;
+    n_cap += (s.status & CTRAJ_ST_CAPPED) ? 1u : 0u;
+    const int64_t gidx = R.index_base + i;
+    wave_hist(s_hist, R.first_index ? s_first : nullptr, s.st * B.n_labels + s.label, s.label, gidx);
+    if (R.aoi_mask) {
+      const bool keep = (R.aoi_mask >> s.label) & 1ull;
+      const uint64_t mask = __ballot(keep);
+      if (mask) {
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd((unsigned long long *)R.aoi_count, (unsigned long long)__popcll(mask));
+        base = __shfl(base, leader);
+        if (keep) {
+          const int64_t slot = (int64_t)base + lanes_below(mask);
+          if (slot < R.aoi_cap) {
+            R.aoi_index[slot] = gidx;
+            if (R.aoi_label) R.aoi_label[slot] = s.label;
+            if (R.aoi_out.x) R.aoi_out.x[slot] = m.x;
+            if (R.aoi_out.y) R.aoi_out.y[slot] = m.y;
+            if (R.aoi_out.z) R.aoi_out.z[slot] = m.z;
+            if (R.aoi_out.vx) R.aoi_out.vx[slot] = m.vx;
+            if (R.aoi_out.vy) R.aoi_out.vy[slot] = m.vy;
+            if (R.aoi_out.vz) R.aoi_out.vz[slot] = m.vz;
+            if (R.aoi_out.t) R.aoi_out.t[slot] = m.t;
+          }
+        }
+      }
+    }
+    s.fin = 0;
+  }
+}
+
+// A copy of `p` the compiler cannot see through: loads made through it stay where they are
+// written instead of being hoisted out of the walker loop (which would keep dozens of output
+// pointers live in SGPRs and spill them).
+template <class T>
+__device__ __forceinline__ const T *opaque(const T *p) {
+  uint64_t v = (uint64_t)p;
+  asm volatile("" : "+s"(v));
+  return (const T *)v;
+}
+
+template <int FORCE, int INTEG>
+__global__ void __launch_bounds__(256) k_propagate_lanes(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg,
+                                                         LaneArgs LA) {
+  extern __shared__ double lds[];
+  __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
+  __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
+  __shared__ unsigned long long s_work[4];
+  const int nbins = B.n_states * B.n_labels;
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0;
+  for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x) s_first[b] = ~0ull >> 1;
+  if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
+  const Tables T = stage_tables(lds, KT);
+  __syncthreads();
+
+  const int lane = lane_id();
+  const ctraj_elem L = B.elems[LA.lens_e];
+  const Force<FORCE> F = make_force<FORCE>(L, B.g);
+  const double half_d = L.p[0], R2 = L.p[1], Hs = L.p[2], Ll = L.p[3];
+  WalkCounters wc{0u, 0u};
+  uint32_t n_oot = 0, n_cap = 0;
+  int64_t next = 0, end = 0;  // this wavefront's claimed molecules [next, end)
+  bool exhausted = false;
+  LaneMol s;
+  s.phase = PH_NEED;
+  s.fin = 0;
+  s.gidx = 0;
+  s.st = 0;
+  s.label = 0;
+  s.status = 0;
+
+  for (;;) {
+    const uint64_t lensm = __ballot(s.phase == PH_LENS);
+    const uint64_t idlem = __ballot(s.phase == PH_NEED || s.phase == PH_POST);
+    if (lensm == 0 || __popcll(idlem) >= LA.refill_min) {
+      for (;;) {
+        // lanes that left the lens walk the rest of the beamline
+        if (s.phase == PH_POST) {
+          if constexpr (INTEG == CTRAJ_INTEG_REF) {
+            drift_dt(s.m, time_to(s.m, L.z1), B.g);  // exit hop with the stored a = l1
+            ++wc.faces;
+          }
+          int lab = B.label_detected;
+          walk_plain(B.elems, LA.lens_e + 1, B.n_elems, B.g, s.m, lab, wc);
+          s.label = lab;
+          s.fin = 1;
+          s.phase = PH_NEED;
+        }
+        {
+          const ctraj_run &R = *opaque(Rg);
+          record_finished<FORCE>(B, R, s, s_hist, R.first_index ? s_first : nullptr, lane, n_oot, n_cap, F, T);
+        }
+        const uint64_t need = __ballot(s.phase == PH_NEED);
+        if (!need || exhausted) {
+          if (s.phase == PH_NEED && exhausted) s.phase = PH_DONE;
+          break;
+        }
+        if (next >= end) {
+          unsigned long long c = 0;
+          if (lane == 0) c = atomicAdd(LA.counter, (unsigned long long)LA.claim);
+          c = __shfl(c, 0);
+          next = (int64_t)c;
+          const int64_t n_all = opaque(Rg)->n;
+          end = next + LA.claim < n_all ? next + LA.claim : n_all;
+          if (next >= end) {
+            exhausted = true;
+            continue;
+          }
+        }
+        const int64_t avail = end - next;
+        if (s.phase == PH_NEED) {
+          const int rank = lanes_below(need);
+          if (rank < avail) {
+            const ctraj_run &R = *opaque(Rg);
+            const int64_t i = next + rank;
+            s.gidx = i;
+            MState &m = s.m;
+            m.x = R.in.x[i];
+            m.y = R.in.y[i];
+            m.z = R.in.z ? R.in.z[i] : R.z_const;
+            m.vx = R.in.vx[i];
+            m.vy = R.in.vy[i];
+            m.vz = R.in.vz[i];
+            m.t = R.in.t ? R.in.t[i] : R.t_const;
+            m.ax = R.a0[0] ? R.a0[0][i] : 0.0;
+            m.ay = R.a0[1] ? R.a0[1][i] : -B.g;
+            m.az = R.a0[2] ? R.a0[2][i] : 0.0;
+            s.st = R.state ? min((int)R.state[i], B.n_states - 1) : 0;
+            s.status = 0;
+            int lab = B.label_detected;
+            bool ok = walk_plain(B.elems, 0, LA.lens_e, B.g, m, lab, wc);
+            if (ok) {
+              drift_dt(m, time_to(m, L.z0), B.g);
+              ++wc.faces;
+              if (rho_of(m.x, m.y) > half_d) {
+                ok = false;
+                lab = L.label_hit;
+              }
+            }
+            if (!ok) {
+              s.label = lab;
+              s.fin = 1;
+            } else {
+              s.i = 0;
+              if constexpr (INTEG == CTRAJ_INTEG_RK4) {
+                s.nfull = rk4_full_steps(Ll, m.vz, Hs, B.cap, s.status);
+                s.dzs = m.vz * Hs;
+                s.hp = Ll / m.vz - (double)s.nfull * Hs;
+                s.h = s.nfull == 0 ? s.hp : Hs;
+                s.r2 = __builtin_fma(m.x, m.x, m.y * m.y);
+                s.phase = PH_LENS;
+              } else {
+                s.nfull = L.n_steps;
+                s.h = Hs / m.vz;
+                s.phase = s.nfull > 0 ? PH_LENS : PH_POST;
+              }
+            }
+          }
+        }
+        next += avail < (int64_t)__popcll(need) ? avail : (int64_t)__popcll(need);
+      }
+      if (!__ballot(s.phase == PH_LENS)) break;
+    }
+
+    if (s.phase == PH_LENS) {
+      MState &m = s.m;
+      if constexpr (INTEG == CTRAJ_INTEG_RK4) {
+        const bool last = (s.i == s.nfull);
+        rk4_step<FORCE>(F, T, s.st, s.h, m.x, m.y, m.vx, m.vy, s.r2, s.status);
+        m.z = last ? L.z1 : m.z + s.dzs;
+        m.t = m.t + s.h;
+        ++wc.steps;
+        s.r2 = __builtin_fma(m.x, m.x, m.y * m.y);
+        ++s.i;
+        if (s.i == s.nfull) s.h = s.hp;
+        if (s.r2 > R2) {
+          s.label = L.label_inside;
+          s.fin = 2;
+          s.phase = PH_NEED;
+        } else if (last) {
+          m.ax = 0.0;
+          m.ay = -B.g;
+          m.az = 0.0;
+          s.phase = PH_POST;
+        }
+      } else {
+        ref_step<FORCE>(F, T, s.st, s.h, m, s.status);
+        ++wc.steps;
+        ++s.i;
+        if (rho_of(m.x, m.y) > half_d) {
+          s.label = L.label_inside;
+          s.fin = 1;
+          s.phase = PH_NEED;
+        } else if (s.i == s.nfull) {
+          s.phase = PH_POST;
+        }
+      }
+    }
+  }
+
+  const ctraj_run &R = *Rg;
   if (R.work) {
     if (wc.faces) atomicAdd(&s_work[0], (unsigned long long)wc.faces);
     if (wc.steps) atomicAdd(&s_work[1], (unsigned long long)wc.steps);
@@ -391,11 +648,20 @@ int grid_for(ctraj_ctx *ctx, const void *kern, int block, size_t lds, int64_t n,
   X(CTRAJ_FORCE_WSPLINE_LIN, CTRAJ_INTEG_RK4)
 
 using PropFn = void (*)(KBeam, KTab, ctraj_run);
+using LaneFn = void (*)(KBeam, KTab, const ctraj_run *, LaneArgs);
 using RecFn = void (*)(KBeam, KTab, ctraj_record);
 
 static PropFn prop_kernel(int f, int i) {
 #define X(F, I) \
   if (f == F && i == I) return k_propagate<F, I>;
+  CT_FI_LIST(X)
+#undef X
+  return nullptr;
+}
+
+static LaneFn lane_kernel(int f, int i) {
+#define X(F, I) \
+  if (f == F && i == I) return k_propagate_lanes<F, I>;
   CT_FI_LIST(X)
 #undef X
   return nullptr;
@@ -459,6 +725,8 @@ int32_t ctraj_ctx_destroy(ctraj_ctx *c) {
   if (c->d_elems) (void)hipFree(c->d_elems);
   if (c->d_ws) (void)hipFree(c->d_ws);
   if (c->d_rt) (void)hipFree(c->d_rt);
+  if (c->d_counter) (void)hipFree(c->d_counter);
+  if (c->d_run) (void)hipFree(c->d_run);
   for (int i = 0; i < 64; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   (void)hipStreamDestroy(c->stream);
@@ -645,17 +913,46 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
   if (block < 64 || block > 256 || block % 64) return fail(c, CTRAJ_EINVAL, "block must be 64, 128, 192 or 256");
   int rc0 = check_tables(c, c->h_elems.data(), c->n_elems, c->phys.n_states);
   if (rc0) return rc0;
-  PropFn k = prop_kernel(c->force, c->integ);
-  if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
   KTab kt = ctx_ktab(c);
   if (c->force == CTRAJ_FORCE_RTABLE) kt.ws_states = 0;
   const size_t lds = ktab_lds_bytes(kt);
-  int rc = lds_limit(c, (const void *)k, lds);
-  if (rc) return rc;
-  HIPCHK(c, hipSetDevice(c->device));
-  const int grid = grid_for(c, (const void *)k, block, lds, R->n, R->grid);
   KBeam B = make_kbeam(c->d_elems, c->n_elems, c->phys);
-  hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, *R);
+  int n_lens = 0, lens_e = -1;
+  for (int e = 0; e < c->n_elems; ++e)
+    if (c->h_elems[e].kind == CTRAJ_LENS) {
+      ++n_lens;
+      lens_e = e;
+    }
+  HIPCHK(c, hipSetDevice(c->device));
+  if (n_lens == 1 && !(R->flags & CTRAJ_RUN_SIMPLE)) {
+    LaneFn k = lane_kernel(c->force, c->integ);
+    if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
+    int rc = lds_limit(c, (const void *)k, lds);
+    if (rc) return rc;
+    if (!c->d_counter) HIPCHK(c, hipMalloc((void **)&c->d_counter, 256));
+    rc = ctraj_fill_i64(c, (int64_t *)c->d_counter, 0, 1);
+    if (rc) return rc;
+    if (!c->d_run) HIPCHK(c, hipMalloc(&c->d_run, sizeof(ctraj_run)));
+    if (!c->have_last_run || std::memcmp(&c->last_run, R, sizeof(ctraj_run)) != 0) {
+      HIPCHK(c, hipMemcpyAsync(c->d_run, R, sizeof(ctraj_run), hipMemcpyHostToDevice, c->stream));
+      c->last_run = *R;
+      c->have_last_run = true;
+    }
+    LaneArgs LA;
+    LA.counter = (unsigned long long *)c->d_counter;
+    LA.lens_e = lens_e;
+    LA.refill_min = R->refill > 0 ? std::min(R->refill, 64) : 16;
+    LA.claim = 128;
+    const int grid = grid_for(c, (const void *)k, block, lds, R->n, R->grid);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, (const ctraj_run *)c->d_run, LA);
+  } else {
+    PropFn k = prop_kernel(c->force, c->integ);
+    if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
+    int rc = lds_limit(c, (const void *)k, lds);
+    if (rc) return rc;
+    const int grid = grid_for(c, (const void *)k, block, lds, R->n, R->grid);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, *R);
+  }
   HIPCHK(c, hipGetLastError());
   return CTRAJ_OK;
 }

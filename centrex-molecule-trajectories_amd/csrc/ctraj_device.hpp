@@ -217,8 +217,44 @@ __device__ __forceinline__ Force<CTRAJ_FORCE_WSPLINE_LIN> make_force<CTRAJ_FORCE
   return Force<CTRAJ_FORCE_WSPLINE_LIN>{E.p[4], E.p[5], g};
 }
 
-// The reference's lens step (electrostatic_lens.py:91-118): RK4 with the second stage at the
-// full Euler point, state stored with a = l1.  Returns false when the molecule leaves the bore.
+// One step of the reference's lens integrator (electrostatic_lens.py:91-111): RK4 with the second
+// stage at the full Euler point x + dt k1; the new row stores a = l1.
+template <int FORCE>
+__device__ __forceinline__ void ref_step(const Force<FORCE> &F, const Tables &T, int state, double dt, MState &m,
+                                         uint32_t &status) {
+  const double x = m.x, y = m.y, z = m.z;
+  const double k1x = m.vx, k1y = m.vy, k1z = m.vz;
+  double l1x, l1y;
+  F(T, state, x, y, l1x, l1y, status);
+  const double k2x = k1x + (dt * l1x) / 2.0;
+  const double k2y = k1y + (dt * l1y) / 2.0;
+  const double k2z = k1z + (dt * 0.0) / 2.0;
+  double l2x, l2y;
+  F(T, state, x + dt * k1x, y + dt * k1y, l2x, l2y, status);
+  const double k3x = k1x + (dt * l2x) / 2.0;
+  const double k3y = k1y + (dt * l2y) / 2.0;
+  const double k3z = k1z + (dt * 0.0) / 2.0;
+  double l3x, l3y;
+  F(T, state, x + (dt * k2x) / 2.0, y + (dt * k2y) / 2.0, l3x, l3y, status);
+  const double k4x = k1x + dt * l3x;
+  const double k4y = k1y + dt * l3y;
+  const double k4z = k1z + dt * 0.0;
+  double l4x, l4y;
+  F(T, state, x + dt * k3x, y + dt * k3y, l4x, l4y, status);
+  m.x = x + (dt * (((k1x + 2.0 * k2x) + 2.0 * k3x) + k4x)) / 6.0;
+  m.y = y + (dt * (((k1y + 2.0 * k2y) + 2.0 * k3y) + k4y)) / 6.0;
+  m.z = z + (dt * (((k1z + 2.0 * k2z) + 2.0 * k3z) + k4z)) / 6.0;
+  m.vx = k1x + (dt * (((l1x + 2.0 * l2x) + 2.0 * l3x) + l4x)) / 6.0;
+  m.vy = k1y + (dt * (((l1y + 2.0 * l2y) + 2.0 * l3y) + l4y)) / 6.0;
+  m.vz = k1z + (dt * (((0.0 + 2.0 * 0.0) + 2.0 * 0.0) + 0.0)) / 6.0;
+  m.t = m.t + dt;
+  m.ax = l1x;
+  m.ay = l1y;
+  m.az = 0.0;
+}
+
+// The reference's lens interior (electrostatic_lens.py:86-118): N = rint(L/dz) steps of dt = dz/vz,
+// bore test after each.  Returns false when the molecule leaves the bore.
 template <int FORCE, class Sink>
 __device__ __forceinline__ bool lens_ref(const ctraj_elem &E, const Tables &T, int state, double g, MState &m,
                                          uint32_t &status, WalkCounters &wc, Sink &sink) {
@@ -227,35 +263,7 @@ __device__ __forceinline__ bool lens_ref(const ctraj_elem &E, const Tables &T, i
   const int N = E.n_steps;
   const double dt = E.p[2] / m.vz;
   for (int i = 0; i < N; ++i) {
-    const double x = m.x, y = m.y, z = m.z;
-    const double k1x = m.vx, k1y = m.vy, k1z = m.vz;
-    double l1x, l1y;
-    F(T, state, x, y, l1x, l1y, status);
-    const double k2x = k1x + (dt * l1x) / 2.0;
-    const double k2y = k1y + (dt * l1y) / 2.0;
-    const double k2z = k1z + (dt * 0.0) / 2.0;
-    double l2x, l2y;
-    F(T, state, x + dt * k1x, y + dt * k1y, l2x, l2y, status);
-    const double k3x = k1x + (dt * l2x) / 2.0;
-    const double k3y = k1y + (dt * l2y) / 2.0;
-    const double k3z = k1z + (dt * 0.0) / 2.0;
-    double l3x, l3y;
-    F(T, state, x + (dt * k2x) / 2.0, y + (dt * k2y) / 2.0, l3x, l3y, status);
-    const double k4x = k1x + dt * l3x;
-    const double k4y = k1y + dt * l3y;
-    const double k4z = k1z + dt * 0.0;
-    double l4x, l4y;
-    F(T, state, x + dt * k3x, y + dt * k3y, l4x, l4y, status);
-    m.x = x + (dt * (((k1x + 2.0 * k2x) + 2.0 * k3x) + k4x)) / 6.0;
-    m.y = y + (dt * (((k1y + 2.0 * k2y) + 2.0 * k3y) + k4y)) / 6.0;
-    m.z = z + (dt * (((k1z + 2.0 * k2z) + 2.0 * k3z) + k4z)) / 6.0;
-    m.vx = k1x + (dt * (((l1x + 2.0 * l2x) + 2.0 * l3x) + l4x)) / 6.0;
-    m.vy = k1y + (dt * (((l1y + 2.0 * l2y) + 2.0 * l3y) + l4y)) / 6.0;
-    m.vz = k1z + (dt * (((0.0 + 2.0 * 0.0) + 2.0 * 0.0) + 0.0)) / 6.0;
-    m.t = m.t + dt;
-    m.ax = l1x;
-    m.ay = l1y;
-    m.az = 0.0;
+    ref_step<FORCE>(F, T, state, dt, m, status);
     ++wc.steps;
     if constexpr (Sink::kRows) sink.row(m);
     if (rho_of(m.x, m.y) > half_d) return false;
@@ -263,15 +271,19 @@ __device__ __forceinline__ bool lens_ref(const ctraj_elem &E, const Tables &T, i
   return true;
 }
 
+// h/6 is formed as h * (1/6) (one rounding of the constant, one of the product), the same on host
+// and device.
+constexpr double kSixth = 1.0 / 6.0;
+
 // Classical RK4 on the transverse state (x, y, vx, vy) with step h; vz is constant in the lens.
-// hh = h/2 and h6 = h/6 are passed in.  The stage slopes are folded into running sums
+// hh = h/2 and h6 = h*(1/6).  The stage slopes are folded into running sums
 //   S_x = ((v1 + 2 v2) + 2 v3) + v4,   S_v = ((a1 + 2 a2) + 2 a3) + a4
 // as soon as each stage is known, which keeps few FP64 values live.  Every FMA here is
 // mirrored by the CPU oracle.
 template <int FORCE>
-__device__ __forceinline__ void rk4_step(const Force<FORCE> &F, const Tables &T, int state, double h, double hh,
-                                         double h6, double &x, double &y, double &vx, double &vy, double r2,
-                                         uint32_t &status) {
+__device__ __forceinline__ void rk4_step(const Force<FORCE> &F, const Tables &T, int state, double h, double &x,
+                                         double &y, double &vx, double &vy, double r2, uint32_t &status) {
+  const double hh = 0.5 * h, h6 = h * kSixth;
   double ax, ay;
   if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX) {
     force_hex_r2(T, state, F.ks, F.cu, F.g, x, y, r2, ax, ay);
@@ -334,7 +346,6 @@ __device__ __forceinline__ bool lens_rk4(const ctraj_elem &E, const Tables &T, i
   const double R2 = E.p[1];
   const double h = E.p[2];
   const double L = E.p[3];
-  const double hh = 0.5 * h, h6 = h / 6.0;
   const double dzs = m.vz * h;
   const int nfull = rk4_full_steps(L, m.vz, h, cap, status);
   double x = m.x, y = m.y, vx = m.vx, vy = m.vy;
@@ -343,12 +354,12 @@ __device__ __forceinline__ bool lens_rk4(const ctraj_elem &E, const Tables &T, i
   bool alive = true;
   for (int i = 0; i <= nfull; ++i) {
     if (i < nfull) {
-      rk4_step<FORCE>(F, T, state, h, hh, h6, x, y, vx, vy, r2, status);
+      rk4_step<FORCE>(F, T, state, h, x, y, vx, vy, r2, status);
       z = z + dzs;
       t = t + h;
     } else {
       const double hp = L / m.vz - (double)nfull * h;
-      rk4_step<FORCE>(F, T, state, hp, 0.5 * hp, hp / 6.0, x, y, vx, vy, r2, status);
+      rk4_step<FORCE>(F, T, state, hp, x, y, vx, vy, r2, status);
       z = E.z1;
       t = t + hp;
     }

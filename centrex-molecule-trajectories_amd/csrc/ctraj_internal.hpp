@@ -30,6 +30,10 @@ struct ctraj_ctx {
   int rt_off[8] = {}, rt_n[8] = {};
   double rt_invh[8] = {};
   std::vector<double> rt_host[8];
+  void *d_counter = nullptr;  // work counter of the lane-refill kernel
+  void *d_run = nullptr;      // device copy of the last ctraj_run given to the lane-refill kernel
+  ctraj_run last_run{};
+  bool have_last_run = false;
   hipEvent_t ev[64];
 };
 

@@ -26,7 +26,7 @@ INTEG_REF, INTEG_RK4 = 0, 1
 
 ST_OOT, ST_CAPPED = 1, 2
 WORK_FACES, WORK_STEPS, WORK_OOT, WORK_CAPPED, WORK_N = 0, 1, 2, 3, 8
-RUN_STATUS_OR = 1
+RUN_STATUS_OR, RUN_SIMPLE = 1, 2
 REC_ROWS, REC_ZGRID = 1, 2
 H2D, D2H, D2D = 1, 2, 3
 
@@ -88,7 +88,7 @@ class Run(C.Structure):
         ("block", C.c_int32),
         ("grid", C.c_int32),
         ("flags", C.c_int32),
-        ("reserved", C.c_int32),
+        ("refill", C.c_int32),
     ]
 
 

@@ -63,7 +63,7 @@ class Engine:
                first_index=None, aoi_mask: int = 0, aoi_cap: int = 0, aoi_count=None, aoi_index=None,
                aoi_label=None, aoi_out: dict | None = None, work=None, a0=None, state=None, n: int | None = None,
                index_base: int = 0, z_const: float = 0.0, t_const: float = 0.0, block: int = 0, grid: int = 0,
-               flags: int = 0) -> None:
+               flags: int = 0, refill: int = 0) -> None:
         """Queue one ``ctraj_propagate`` on device buffers (no synchronisation)."""
         bind(self.ctx, self.cb)
         R = N.Run()
@@ -82,7 +82,7 @@ class Engine:
         R.aoi_count, R.aoi_index, R.aoi_label = N.addr(aoi_count), N.addr(aoi_index), N.addr(aoi_label)
         R.aoi_out = _soa(aoi_out)
         R.work = N.addr(work)
-        R.block, R.grid, R.flags = int(block), int(grid), int(flags)
+        R.block, R.grid, R.flags, R.refill = int(block), int(grid), int(flags), int(refill)
         N.check("ctraj_propagate", N.lib().ctraj_propagate(self.ctx.handle, C.byref(R)), self.ctx.handle)
 
     # ------------------------------------------------------------------ host-array form
@@ -104,7 +104,8 @@ class Engine:
         return n, inp, a0d, st
 
     def propagate(self, x, y, z, vx, vy, vz, t=None, a0=None, state=None, *, outputs: bool = True,
-                  want_accel: bool = False, aoi_labels=(), index_base: int = 0, block: int = 0) -> dict:
+                  want_accel: bool = False, aoi_labels=(), index_base: int = 0, block: int = 0, flags: int = 0,
+                  refill: int = 0) -> dict:
         """Propagate host arrays; returns labels, tallies, work counters and (optionally) final states."""
         ctx = self.ctx
         n, inp, a0d, st = self._upload_inputs(x, y, z, vx, vy, vz, t, a0, state)
@@ -132,7 +133,7 @@ class Engine:
                         aoi_mask=mask, aoi_cap=n if mask else 0, aoi_count=aoi.get("count"),
                         aoi_index=aoi.get("index"), aoi_label=aoi.get("label"), aoi_out=aoi.get("out"),
                         work=work, a0=a0d, state=st,
-                        n=n, index_base=index_base, block=block)
+                        n=n, index_base=index_base, block=block, flags=flags, refill=refill)
         ctx.sync()
         res["tally"] = tally.to_host().reshape(ns, nl)
         res["first_index"] = first.to_host()

@@ -195,9 +195,11 @@ static int lens_ref(const ctraj_elem *E, const OTab *T, int state, double g, Mol
   return 1;
 }
 
-/* classical RK4 on (x, y, vx, vy), running slope sums; same operation order as the device */
-static void rk4_step(const OForce *F, const OTab *T, int state, double h, double hh, double h6, double *x,
-                     double *y, double *vx, double *vy, uint32_t *st) {
+/* classical RK4 on (x, y, vx, vy), running slope sums, h/6 formed as h*(1/6); same operation order
+ * as the device */
+static void rk4_step(const OForce *F, const OTab *T, int state, double h, double *x, double *y, double *vx,
+                     double *vy, uint32_t *st) {
+  const double hh = 0.5 * h, h6 = h * (1.0 / 6.0);
   double ax, ay;
   force_eval(F, T, state, *x, *y, &ax, &ay, st);
   double xs = fma(hh, *vx, *x), ys = fma(hh, *vy, *y);
@@ -237,7 +239,6 @@ static int lens_rk4(const ctraj_elem *E, const OTab *T, int state, double g, int
                     uint64_t *steps, Rows *R, double *edge) {
   OForce F = {E->force, E->table, E->p[4], E->p[5], g};
   const double R2 = E->p[1], h = E->p[2], L = E->p[3];
-  const double hh = 0.5 * h, h6 = h / 6.0;
   const double dzs = m->vz * h;
   int nfull;
   {
@@ -250,12 +251,12 @@ static int lens_rk4(const ctraj_elem *E, const OTab *T, int state, double g, int
   int alive = 1;
   for (int i = 0; i <= nfull; ++i) {
     if (i < nfull) {
-      rk4_step(&F, T, state, h, hh, h6, &x, &y, &vx, &vy, st);
+      rk4_step(&F, T, state, h, &x, &y, &vx, &vy, st);
       z = z + dzs;
       t = t + h;
     } else {
       const double hp = L / m->vz - (double)nfull * h;
-      rk4_step(&F, T, state, hp, 0.5 * hp, hp / 6.0, &x, &y, &vx, &vy, st);
+      rk4_step(&F, T, state, hp, &x, &y, &vx, &vy, st);
       z = E->z1;
       t = t + hp;
     }

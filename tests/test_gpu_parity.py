@@ -227,3 +227,44 @@ def test_beamline_propagate_single_molecule_smoke_kat():
     assert m.trajectory.x.shape[0] == 2
     dt = 0.99 / 200
     assert m.trajectory.x[1, 1] == pytest.approx(-9.80665 * dt * dt / 2, rel=1e-15)
+
+
+def test_lane_refill_kernel_matches_simple_kernel():
+    """The persistent lane-refill walker only reorders work: every refill threshold and the
+    one-thread-per-molecule kernel give identical bits."""
+    for bl, seed in ((configs.config1(), 41), (reference_example_beamline(), 42)):
+        X, V = configs.draw_source(50_000, seed=seed)
+        if bl.elements[0].name == "4K shield":
+            X[:2] *= 0.05
+            V[:2] *= 0.05
+        eng = Engine(bl)
+        base = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], flags=N.RUN_SIMPLE, want_accel=True,
+                             aoi_labels=("Detected",))
+        for refill in (1, 16, 64):
+            res = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], refill=refill, want_accel=True,
+                                aoi_labels=("Detected",))
+            np.testing.assert_array_equal(res["label"], base["label"])
+            np.testing.assert_array_equal(res["status"], base["status"])
+            for k in FIELDS:
+                np.testing.assert_array_equal(res["out"][k], base["out"][k])
+            for q in range(3):
+                np.testing.assert_array_equal(res["out_a"][q], base["out_a"][q])
+            np.testing.assert_array_equal(res["tally"], base["tally"])
+            np.testing.assert_array_equal(res["first_index"], base["first_index"])
+            np.testing.assert_array_equal(res["work"], base["work"])
+            np.testing.assert_array_equal(np.sort(res["aoi_index"]), np.sort(base["aoi_index"]))
+
+
+def test_small_and_empty_runs():
+    bl = configs.config1()
+    eng = Engine(bl)
+    for n in (0, 1, 63, 65, 129):
+        X, V = configs.draw_source(200, seed=50 + n)
+        X, V = X[:, :n], V[:, :n]
+        dev = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2])
+        assert dev["tally"].sum() == n
+        if n:
+            orc = O.propagate(eng.cb, X[0], X[1], X[2], V[0], V[1], V[2])
+            np.testing.assert_array_equal(dev["label"], orc["label"])
+            for k in FIELDS:
+                np.testing.assert_array_equal(dev["out"][k], orc["out"][k])
