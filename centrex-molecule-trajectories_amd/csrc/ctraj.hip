@@ -29,19 +29,29 @@ using namespace ctraj;
 // ---------------------------------------------------------------------------------------------
 namespace ctraj {
 
+// The W' spline is kept in LDS as [state][seg][3] = {A, B, C}: a 24-byte segment stride spreads
+// the lanes' random segments over the LDS banks far better than the 32-byte stride of the global
+// table (which puts every segment on one of 8 bank groups).  r tables keep [knot][4].
 __device__ Tables stage_tables(double *lds, const KTab &kt) {
   Tables T;
-  const int nws = kt.ws_states * kt.ws_nseg * 4;
+  const int nseg_all = kt.ws_states * kt.ws_nseg;
+  const int nws = nseg_all * 3;
   const int nrt = kt.rt_total * 4;
-  double2 *dst = reinterpret_cast<double2 *>(lds);
-  const double2 *src_ws = reinterpret_cast<const double2 *>(kt.ws);
-  for (int i = threadIdx.x; i < nws / 2; i += blockDim.x) dst[i] = src_ws[i];
+  for (int i = threadIdx.x; i < nseg_all; i += blockDim.x) {
+    const double2 ab = reinterpret_cast<const double2 *>(kt.ws)[2 * i];
+    const double c = kt.ws[4 * i + 2];
+    lds[3 * i] = ab.x;
+    lds[3 * i + 1] = ab.y;
+    lds[3 * i + 2] = c;
+  }
   const double2 *src_rt = reinterpret_cast<const double2 *>(kt.rt);
-  for (int i = threadIdx.x; i < nrt / 2; i += blockDim.x) dst[nws / 2 + i] = src_rt[i];
-  T.ws = lds;
+  double2 *dst_rt = reinterpret_cast<double2 *>(lds + nws + (nws & 1));
+  for (int i = threadIdx.x; i < nrt / 2; i += blockDim.x) dst_rt[i] = src_rt[i];
+  const lds_double *base = (const lds_double *)lds;
+  T.ws = base;
   T.nseg = kt.ws_nseg;
-  T.umax = (double)kt.ws_nseg;
-  T.rt = lds + nws;
+  T.umax = kt.umax;
+  T.rt = base + nws + (nws & 1);
   return T;
 }
 
@@ -72,85 +82,132 @@ __device__ __forceinline__ void wave_hist(uint32_t *s_hist, unsigned long long *
 }  // namespace ctraj
 
 // ---------------------------------------------------------------------------------------------
-// k_propagate
+// k_propagate: one molecule per thread, grid-stride over the SoA store.  The loop runs in
+// workgroup-uniform chunks so that kept (aoi) molecules can be staged in LDS and flushed to the
+// dense output with one global atomic per ~AOI_BUF/2 molecules instead of one per wavefront pass.
+// INTEG = CTRAJ_INTEG_NONE instantiates the walker without lens code (beamlines with no lens).
 // ---------------------------------------------------------------------------------------------
+constexpr int AOI_BUF = 512;
+
+struct AoiStage {
+  long long *idx;  // [AOI_BUF]
+  int *label;      // [AOI_BUF]
+  double *val;     // [7][AOI_BUF]
+};
+
+__device__ __forceinline__ void aoi_flush(const ctraj_run &R, AoiStage &A, int *s_n, unsigned long long *s_base) {
+  __syncthreads();
+  const int cnt = *s_n;
+  if (threadIdx.x == 0) *s_base = atomicAdd((unsigned long long *)R.aoi_count, (unsigned long long)cnt);
+  __syncthreads();
+  const int64_t base = (int64_t)*s_base;
+  double *outs[7] = {R.aoi_out.x, R.aoi_out.y, R.aoi_out.z, R.aoi_out.vx, R.aoi_out.vy, R.aoi_out.vz, R.aoi_out.t};
+  for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+    const int64_t slot = base + j;
+    if (slot < R.aoi_cap) {
+      R.aoi_index[slot] = A.idx[j];
+      if (R.aoi_label) R.aoi_label[slot] = A.label[j];
+#pragma unroll
+      for (int f = 0; f < 7; ++f)
+        if (outs[f]) outs[f][slot] = A.val[f * AOI_BUF + j];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *s_n = 0;
+  __syncthreads();
+}
+
 template <int FORCE, int INTEG>
-__global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R) {
+__global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R, int aoi_lds_off) {
   extern __shared__ double lds[];
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_work[4];
+  __shared__ int s_aoi_n;
+  __shared__ unsigned long long s_aoi_base;
   const int nbins = B.n_states * B.n_labels;
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0;
   for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x) s_first[b] = ~0ull >> 1;
   if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
-  const Tables T = stage_tables(lds, KT);
+  if (threadIdx.x == 0) s_aoi_n = 0;
+  Tables T{};
+  if constexpr (INTEG != CTRAJ_INTEG_NONE) T = stage_tables(lds, KT);
+  AoiStage A;
+  A.idx = reinterpret_cast<long long *>(lds + aoi_lds_off);
+  A.label = reinterpret_cast<int *>(lds + aoi_lds_off + AOI_BUF);
+  A.val = lds + aoi_lds_off + AOI_BUF + AOI_BUF / 2;
   __syncthreads();
 
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   WalkCounters wc{0u, 0u};
   uint32_t n_oot = 0, n_cap = 0;
   const int lane = lane_id();
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < R.n; i += stride) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < R.n; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    const bool active = i < R.n;
+    int label = 0;
     MState m;
-    m.x = R.in.x[i];
-    m.y = R.in.y[i];
-    m.z = R.in.z ? R.in.z[i] : R.z_const;
-    m.vx = R.in.vx[i];
-    m.vy = R.in.vy[i];
-    m.vz = R.in.vz[i];
-    m.t = R.in.t ? R.in.t[i] : R.t_const;
-    m.ax = R.a0[0] ? R.a0[0][i] : 0.0;
-    m.ay = R.a0[1] ? R.a0[1][i] : -B.g;
-    m.az = R.a0[2] ? R.a0[2][i] : 0.0;
-    const int st = R.state ? min((int)R.state[i], B.n_states - 1) : 0;
-    int label;
-    uint32_t status = 0;
-    NullSink sk;
-    walk<FORCE, INTEG>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status, wc, sk);
+    if (active) {
+      m.x = R.in.x[i];
+      m.y = R.in.y[i];
+      m.z = R.in.z ? R.in.z[i] : R.z_const;
+      m.vx = R.in.vx[i];
+      m.vy = R.in.vy[i];
+      m.vz = R.in.vz[i];
+      m.t = R.in.t ? R.in.t[i] : R.t_const;
+      m.ax = R.a0[0] ? R.a0[0][i] : 0.0;
+      m.ay = R.a0[1] ? R.a0[1][i] : -B.g;
+      m.az = R.a0[2] ? R.a0[2][i] : 0.0;
+      const int st = R.state ? min((int)R.state[i], B.n_states - 1) : 0;
+      uint32_t status = 0;
+      NullSink sk;
+      walk<FORCE, INTEG>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status, wc, sk);
 
-    if (R.out.x) R.out.x[i] = m.x;
-    if (R.out.y) R.out.y[i] = m.y;
-    if (R.out.z) R.out.z[i] = m.z;
-    if (R.out.vx) R.out.vx[i] = m.vx;
-    if (R.out.vy) R.out.vy[i] = m.vy;
-    if (R.out.vz) R.out.vz[i] = m.vz;
-    if (R.out.t) R.out.t[i] = m.t;
-    if (R.out_a[0]) R.out_a[0][i] = m.ax;
-    if (R.out_a[1]) R.out_a[1][i] = m.ay;
-    if (R.out_a[2]) R.out_a[2][i] = m.az;
-    if (R.label) R.label[i] = label;
-    if (R.status) R.status[i] = (R.flags & CTRAJ_RUN_STATUS_OR) ? (uint8_t)(R.status[i] | status) : (uint8_t)status;
-    n_oot += (status & CTRAJ_ST_OOT) ? 1u : 0u;
-    n_cap += (status & CTRAJ_ST_CAPPED) ? 1u : 0u;
-
-    const int64_t gidx = R.index_base + i;
-    wave_hist(s_hist, R.first_index ? s_first : nullptr, st * B.n_labels + label, label, gidx);
-
+      if (R.out.x) R.out.x[i] = m.x;
+      if (R.out.y) R.out.y[i] = m.y;
+      if (R.out.z) R.out.z[i] = m.z;
+      if (R.out.vx) R.out.vx[i] = m.vx;
+      if (R.out.vy) R.out.vy[i] = m.vy;
+      if (R.out.vz) R.out.vz[i] = m.vz;
+      if (R.out.t) R.out.t[i] = m.t;
+      if (R.out_a[0]) R.out_a[0][i] = m.ax;
+      if (R.out_a[1]) R.out_a[1][i] = m.ay;
+      if (R.out_a[2]) R.out_a[2][i] = m.az;
+      if (R.label) R.label[i] = label;
+      if (R.status)
+        R.status[i] = (R.flags & CTRAJ_RUN_STATUS_OR) ? (uint8_t)(R.status[i] | status) : (uint8_t)status;
+      n_oot += (status & CTRAJ_ST_OOT) ? 1u : 0u;
+      n_cap += (status & CTRAJ_ST_CAPPED) ? 1u : 0u;
+      wave_hist(s_hist, R.first_index ? s_first : nullptr, st * B.n_labels + label, label, R.index_base + i);
+    }
     if (R.aoi_mask) {
-      const bool keep = (R.aoi_mask >> label) & 1ull;
+      const bool keep = active && ((R.aoi_mask >> label) & 1ull);
       const uint64_t mask = __ballot(keep);
       if (mask) {
         const int leader = __ffsll((unsigned long long)mask) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd((unsigned long long *)R.aoi_count, (unsigned long long)__popcll(mask));
-        base = __shfl(base, leader);
+        int slot0 = 0;
+        if (lane == leader) slot0 = atomicAdd(&s_aoi_n, __popcll(mask));
+        slot0 = __shfl(slot0, leader);
         if (keep) {
-          const int64_t slot = (int64_t)base + lanes_below(mask);
-          if (slot < R.aoi_cap) {
-            R.aoi_index[slot] = gidx;
-            if (R.aoi_label) R.aoi_label[slot] = label;
-            if (R.aoi_out.x) R.aoi_out.x[slot] = m.x;
-            if (R.aoi_out.y) R.aoi_out.y[slot] = m.y;
-            if (R.aoi_out.z) R.aoi_out.z[slot] = m.z;
-            if (R.aoi_out.vx) R.aoi_out.vx[slot] = m.vx;
-            if (R.aoi_out.vy) R.aoi_out.vy[slot] = m.vy;
-            if (R.aoi_out.vz) R.aoi_out.vz[slot] = m.vz;
-            if (R.aoi_out.t) R.aoi_out.t[slot] = m.t;
-          }
+          const int j = slot0 + lanes_below(mask);
+          A.idx[j] = R.index_base + i;
+          A.label[j] = label;
+          A.val[0 * AOI_BUF + j] = m.x;
+          A.val[1 * AOI_BUF + j] = m.y;
+          A.val[2 * AOI_BUF + j] = m.z;
+          A.val[3 * AOI_BUF + j] = m.vx;
+          A.val[4 * AOI_BUF + j] = m.vy;
+          A.val[5 * AOI_BUF + j] = m.vz;
+          A.val[6 * AOI_BUF + j] = m.t;
         }
       }
+      __syncthreads();
+      if (s_aoi_n > AOI_BUF - (int)blockDim.x) aoi_flush(R, A, &s_aoi_n, &s_aoi_base);
     }
+  }
+  if (R.aoi_mask) {
+    __syncthreads();
+    if (s_aoi_n > 0) aoi_flush(R, A, &s_aoi_n, &s_aoi_base);
   }
   if (R.work) {
     if (wc.faces) atomicAdd(&s_work[0], (unsigned long long)wc.faces);
@@ -191,7 +248,7 @@ __device__ __forceinline__ void record_finished(const KBeam &B, const ctraj_run 
     MState &m = s.m;
     if (s.fin == 2 && R.out_a[0]) {
       uint32_t st2 = 0;
-      F(T, s.st, m.x, m.y, m.ax, m.ay, st2);
+      F(T, ws_of(T, s.st), m.x, m.y, m.ax, m.ay, st2);
       m.az = 0.0;
     }
     if (R.out.x) R.out.x[i] = m.x;
@@ -263,11 +320,14 @@ __global__ void __launch_bounds__(256) k_propagate_lanes(KBeam B, KTab KT, const
   __syncthreads();
 
   const int lane = lane_id();
-  const ctraj_elem L = B.elems[LA.lens_e];
-  const Force<FORCE> F = make_force<FORCE>(L, B.g);
-  const double half_d = L.p[0], R2 = L.p[1], Hs = L.p[2], Ll = L.p[3];
+  // hot lens parameters (live across the walker loop); the rest is re-read where it is used
+  const ctraj_elem &L0 = B.elems[LA.lens_e];
+  const Force<FORCE> F = make_force<FORCE>(L0, B.g);
+  const double R2 = L0.p[1], z1 = L0.z1;
+  const int label_inside = L0.label_inside;
   WalkCounters wc{0u, 0u};
   uint32_t n_oot = 0, n_cap = 0;
+  unsigned long long steps_w = 0;  // lens steps of this wavefront (counted with the exec mask)
   int64_t next = 0, end = 0;  // this wavefront's claimed molecules [next, end)
   bool exhausted = false;
   LaneMol s;
@@ -286,8 +346,12 @@ __global__ void __launch_bounds__(256) k_propagate_lanes(KBeam B, KTab KT, const
         // lanes that left the lens walk the rest of the beamline
         if (s.phase == PH_POST) {
           if constexpr (INTEG == CTRAJ_INTEG_REF) {
-            drift_dt(s.m, time_to(s.m, L.z1), B.g);  // exit hop with the stored a = l1
+            drift_dt(s.m, time_to(s.m, z1), B.g);  // exit hop with the stored a = l1
             ++wc.faces;
+          } else {
+            s.m.ax = 0.0;  // the exit row at z1 stores (0, -g, 0)
+            s.m.ay = -B.g;
+            s.m.az = 0.0;
           }
           int lab = B.label_detected;
           walk_plain(B.elems, LA.lens_e + 1, B.n_elems, B.g, s.m, lab, wc);
@@ -321,6 +385,8 @@ __global__ void __launch_bounds__(256) k_propagate_lanes(KBeam B, KTab KT, const
           const int rank = lanes_below(need);
           if (rank < avail) {
             const ctraj_run &R = *opaque(Rg);
+            const ctraj_elem &L = *opaque(B.elems + LA.lens_e);
+            const double half_d = L.p[0], Hs = L.p[2], Ll = L.p[3];
             const int64_t i = next + rank;
             s.gidx = i;
             MState &m = s.m;
@@ -335,6 +401,7 @@ __global__ void __launch_bounds__(256) k_propagate_lanes(KBeam B, KTab KT, const
             m.ay = R.a0[1] ? R.a0[1][i] : -B.g;
             m.az = R.a0[2] ? R.a0[2][i] : 0.0;
             s.st = R.state ? min((int)R.state[i], B.n_states - 1) : 0;
+            s.wsb = ws_of(T, s.st);
             s.status = 0;
             int lab = B.label_detected;
             bool ok = walk_plain(B.elems, 0, LA.lens_e, B.g, m, lab, wc);
@@ -371,33 +438,27 @@ __global__ void __launch_bounds__(256) k_propagate_lanes(KBeam B, KTab KT, const
       if (!__ballot(s.phase == PH_LENS)) break;
     }
 
+    steps_w += (unsigned long long)__popcll(__ballot(s.phase == PH_LENS));
     if (s.phase == PH_LENS) {
       MState &m = s.m;
       if constexpr (INTEG == CTRAJ_INTEG_RK4) {
+        // branch-free step tail: every lane in the lens runs the same instructions
         const bool last = (s.i == s.nfull);
-        rk4_step<FORCE>(F, T, s.st, s.h, m.x, m.y, m.vx, m.vy, s.r2, s.status);
-        m.z = last ? L.z1 : m.z + s.dzs;
+        rk4_step<FORCE>(F, T, s.wsb, s.h, m.x, m.y, m.vx, m.vy, s.r2, s.status);
+        m.z = last ? z1 : m.z + s.dzs;
         m.t = m.t + s.h;
-        ++wc.steps;
         s.r2 = __builtin_fma(m.x, m.x, m.y * m.y);
         ++s.i;
-        if (s.i == s.nfull) s.h = s.hp;
-        if (s.r2 > R2) {
-          s.label = L.label_inside;
-          s.fin = 2;
-          s.phase = PH_NEED;
-        } else if (last) {
-          m.ax = 0.0;
-          m.ay = -B.g;
-          m.az = 0.0;
-          s.phase = PH_POST;
-        }
+        s.h = (s.i == s.nfull) ? s.hp : s.h;
+        const bool dead = s.r2 > R2;
+        s.label = dead ? label_inside : s.label;
+        s.fin = dead ? 2 : 0;
+        s.phase = dead ? (int)PH_NEED : (last ? (int)PH_POST : (int)PH_LENS);
       } else {
-        ref_step<FORCE>(F, T, s.st, s.h, m, s.status);
-        ++wc.steps;
+        ref_step<FORCE>(F, T, s.wsb, s.h, m, s.status);
         ++s.i;
-        if (rho_of(m.x, m.y) > half_d) {
-          s.label = L.label_inside;
+        if (rho_of(m.x, m.y) > L0.p[0]) {
+          s.label = label_inside;
           s.fin = 1;
           s.phase = PH_NEED;
         } else if (s.i == s.nfull) {
@@ -410,7 +471,7 @@ __global__ void __launch_bounds__(256) k_propagate_lanes(KBeam B, KTab KT, const
   const ctraj_run &R = *Rg;
   if (R.work) {
     if (wc.faces) atomicAdd(&s_work[0], (unsigned long long)wc.faces);
-    if (wc.steps) atomicAdd(&s_work[1], (unsigned long long)wc.steps);
+    if (lane == 0 && steps_w) atomicAdd(&s_work[1], steps_w);
     if (n_oot) atomicAdd(&s_work[2], (unsigned long long)n_oot);
     if (n_cap) atomicAdd(&s_work[3], (unsigned long long)n_cap);
   }
@@ -521,12 +582,16 @@ KTab ctx_ktab(const ctraj_ctx *ctx) {
   k.ws = ctx->d_ws;
   k.ws_nseg = ctx->ws_nseg;
   k.ws_states = ctx->d_ws ? ctx->ws_states : 0;
+  k.umax = ctx->ws_nseg > 0 ? std::nextafter((double)ctx->ws_nseg, 0.0) : 0.0;
   k.rt = ctx->d_rt;
   k.rt_total = ctx->d_rt ? ctx->rt_total : 0;
   return k;
 }
 
-size_t ktab_lds_bytes(const KTab &k) { return ((size_t)k.ws_states * k.ws_nseg + (size_t)k.rt_total) * 4 * sizeof(double); }
+size_t ktab_lds_bytes(const KTab &k) {
+  const size_t nws = (size_t)k.ws_states * k.ws_nseg * 3;
+  return (nws + (nws & 1) + (size_t)k.rt_total * 4) * sizeof(double);
+}
 
 // Validate an element table and find the (force, integrator) pair shared by its lenses.
 int check_elems(ctraj_ctx *ctx, const ctraj_elem *e, int n, const ctraj_phys *p, int *force, int *integ) {
@@ -647,11 +712,12 @@ int grid_for(ctraj_ctx *ctx, const void *kern, int block, size_t lds, int64_t n,
   X(CTRAJ_FORCE_WSPLINE_LIN, CTRAJ_INTEG_REF)        \
   X(CTRAJ_FORCE_WSPLINE_LIN, CTRAJ_INTEG_RK4)
 
-using PropFn = void (*)(KBeam, KTab, ctraj_run);
+using PropFn = void (*)(KBeam, KTab, ctraj_run, int);
 using LaneFn = void (*)(KBeam, KTab, const ctraj_run *, LaneArgs);
 using RecFn = void (*)(KBeam, KTab, ctraj_record);
 
-static PropFn prop_kernel(int f, int i) {
+static PropFn prop_kernel(int f, int i, bool has_lens) {
+  if (!has_lens) return k_propagate<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_NONE>;
 #define X(F, I) \
   if (f == F && i == I) return k_propagate<F, I>;
   CT_FI_LIST(X)
@@ -946,12 +1012,17 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     const int grid = grid_for(c, (const void *)k, block, lds, R->n, R->grid);
     hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, (const ctraj_run *)c->d_run, LA);
   } else {
-    PropFn k = prop_kernel(c->force, c->integ);
+    PropFn k = prop_kernel(c->force, c->integ, n_lens > 0);
     if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
-    int rc = lds_limit(c, (const void *)k, lds);
+    if (n_lens == 0) kt.ws_states = kt.rt_total = 0;
+    size_t lds_tab = n_lens > 0 ? lds : 0;
+    // kept-molecule staging area after the tables: int64 idx, int32 label, 7 doubles per entry
+    const int aoi_off = (int)(lds_tab / sizeof(double));
+    const size_t lds_all = lds_tab + (R->aoi_mask ? (size_t)AOI_BUF * (8 + 4 + 7 * 8) : 0);
+    int rc = lds_limit(c, (const void *)k, lds_all);
     if (rc) return rc;
-    const int grid = grid_for(c, (const void *)k, block, lds, R->n, R->grid);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, *R);
+    const int grid = grid_for(c, (const void *)k, block, lds_all, R->n, R->grid);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds_all, c->stream, B, kt, *R, aoi_off);
   }
   HIPCHK(c, hipGetLastError());
   return CTRAJ_OK;

@@ -16,18 +16,24 @@
 
 namespace ctraj {
 
+// Walker instantiation without lens code (beamlines that contain no lens).
+constexpr int CTRAJ_INTEG_NONE = -1;
+
 // Molecule state plus the acceleration stored with its current trajectory row (molecule.py:46-50).
 struct MState {
   double x, y, z, vx, vy, vz, t;
   double ax, ay, az;
 };
 
-// Lens tables as seen from one workgroup (LDS) or one host loop.
+// LDS-qualified doubles: arithmetic on these pointers stays 32-bit and loads become ds_read.
+typedef __attribute__((address_space(3))) double lds_double;
+
+// Lens tables as seen from one workgroup (staged in LDS).
 struct Tables {
-  const double *ws;  // W' spline: [state][nseg][4] = {A, B, C, 0}
+  const lds_double *ws;  // W' spline: [state][nseg][3] = {A, B, C}
   int nseg;
-  double umax;       // = nseg; |E| beyond the table is clamped to its last knot
-  const double *rt;  // r tables: [knot][4] = {r, a, slope, 0}, all tables back to back
+  double umax;           // largest double below nseg: |E| beyond the table is clamped there
+  const lds_double *rt;  // r tables: [knot][4] = {r, a, slope, 0}, all tables back to back
 };
 
 // Where one r table sits in Tables::rt.  The host resolves a lens's table id into these before
@@ -75,7 +81,7 @@ __device__ __forceinline__ double rho_of(double x, double y) { return __builtin_
 // :217): j with r_j <= r < r_{j+1}; r == r_j returns a_j; r == r_last returns a_last; r above the
 // table is flagged (the reference raises ValueError there) and returns a_last.
 __device__ __forceinline__ double rtable_eval(const Tables &T, const RDesc &D, double r, uint32_t &status) {
-  const double *tb = T.rt + 4 * D.off;
+  const lds_double *tb = T.rt + 4 * D.off;
   const int n = D.n;
   const double rlast = tb[4 * (n - 1)];
   if (r < tb[0]) {
@@ -113,36 +119,38 @@ __device__ __forceinline__ void force_rtable(const Tables &T, const RDesc &D, do
   ay = fy - g;
 }
 
-// W'(u) from the LDS spline of one state; u = |E|/dE.  Segment k = trunc(u) clamped to the last
-// segment, s = u - k (u is first clamped to the table end, so the force saturates beyond it).
-__device__ __forceinline__ double wspline_eval(const Tables &T, int state, double u) {
-  u = __builtin_fmin(u, T.umax);
-  int k = (int)u;
-  k = k < T.nseg - 1 ? k : T.nseg - 1;
-  const double s = u - (double)k;
-  const double *c = T.ws + 4 * ((size_t)state * T.nseg + k);
-  const double2 ab = *reinterpret_cast<const double2 *>(c);
-  const double cc = c[2];
-  return __builtin_fma(__builtin_fma(ab.x, s, ab.y), s, cc);
+// LDS base of one state's W' spline.
+__device__ __forceinline__ const lds_double *ws_of(const Tables &T, int state) { return T.ws + 3 * (state * T.nseg); }
+
+// W'(u) from a state's LDS spline; u = |E|/dE >= 0.  u is clamped to umax, the largest double below
+// nseg, so k = trunc(u) <= nseg - 1 and s = u - k = fract(u) exactly; the force saturates beyond the
+// table end.
+__device__ __forceinline__ double wspline_eval(const lds_double *wsb, double umax, double u) {
+  u = __builtin_fmin(u, umax);
+  const unsigned k = (unsigned)(int)u;
+  const double s = __builtin_amdgcn_fract(u);
+  // 32-bit LDS byte address k*24 + base (segments are far below 2^24)
+  const lds_double *c = (const lds_double *)(size_t)(__umul24(k, 24u) + (unsigned)(size_t)wsb);
+  return __builtin_fma(__builtin_fma(c[0], s, c[1]), s, c[2]);
 }
 
 // Hexapole: |E| = cE r^2, a = -W'(|E|) 2 cE (x, y) / m - g y_hat.  r2 = x^2 + y^2 is passed in
 // so that the bore test of the previous step can share it.
-__device__ __forceinline__ void force_hex_r2(const Tables &T, int state, double ks, double cu, double g,
-                                             double x, double y, double r2, double &ax, double &ay) {
-  const double f = wspline_eval(T, state, r2 * cu) * ks;
+__device__ __forceinline__ void force_hex_r2(const lds_double *wsb, double umax, double ks, double cu, double g, double x,
+                                             double y, double r2, double &ax, double &ay) {
+  const double f = wspline_eval(wsb, umax, r2 * cu) * ks;
   ax = f * x;
   ay = __builtin_fma(f, y, -g);
 }
 
 // Linear-field model |E| = cE r (the reference's field assumption, electrostatic_lens.py:199-200):
 // a = -W'(|E|) cE (x, y) / (m r) - g y_hat, and (0, -g) on axis.
-__device__ __forceinline__ void force_lin(const Tables &T, int state, double ks, double cu, double g,
-                                          double x, double y, double &ax, double &ay) {
+__device__ __forceinline__ void force_lin(const lds_double *wsb, double umax, double ks, double cu, double g, double x,
+                                          double y, double &ax, double &ay) {
   const double r = rho_of(x, y);
   double fx = 0.0, fy = 0.0;
   if (r != 0.0) {
-    const double f = (wspline_eval(T, state, r * cu) * ks) / r;
+    const double f = (wspline_eval(wsb, umax, r * cu) * ks) / r;
     fx = f * x;
     fy = f * y;
   }
@@ -150,6 +158,7 @@ __device__ __forceinline__ void force_lin(const Tables &T, int state, double ks,
   ay = fy - g;
 }
 
+// A lens force; `wsb` is the LDS spline of the molecule's state (ws_of).
 template <int FORCE>
 struct Force;
 
@@ -157,7 +166,7 @@ template <>
 struct Force<CTRAJ_FORCE_RTABLE> {
   RDesc d;
   double g;
-  __device__ __forceinline__ void operator()(const Tables &T, int /*state*/, double x, double y, double &ax,
+  __device__ __forceinline__ void operator()(const Tables &T, const lds_double * /*wsb*/, double x, double y, double &ax,
                                              double &ay, uint32_t &st) const {
     force_rtable(T, d, g, x, y, ax, ay, st);
   }
@@ -166,18 +175,18 @@ struct Force<CTRAJ_FORCE_RTABLE> {
 template <>
 struct Force<CTRAJ_FORCE_WSPLINE_HEX> {
   double ks, cu, g;
-  __device__ __forceinline__ void operator()(const Tables &T, int state, double x, double y, double &ax,
+  __device__ __forceinline__ void operator()(const Tables &T, const lds_double *wsb, double x, double y, double &ax,
                                              double &ay, uint32_t &) const {
-    force_hex_r2(T, state, ks, cu, g, x, y, __builtin_fma(x, x, y * y), ax, ay);
+    force_hex_r2(wsb, T.umax, ks, cu, g, x, y, __builtin_fma(x, x, y * y), ax, ay);
   }
 };
 
 template <>
 struct Force<CTRAJ_FORCE_WSPLINE_LIN> {
   double ks, cu, g;
-  __device__ __forceinline__ void operator()(const Tables &T, int state, double x, double y, double &ax,
+  __device__ __forceinline__ void operator()(const Tables &T, const lds_double *wsb, double x, double y, double &ax,
                                              double &ay, uint32_t &) const {
-    force_lin(T, state, ks, cu, g, x, y, ax, ay);
+    force_lin(wsb, T.umax, ks, cu, g, x, y, ax, ay);
   }
 };
 
@@ -220,27 +229,27 @@ __device__ __forceinline__ Force<CTRAJ_FORCE_WSPLINE_LIN> make_force<CTRAJ_FORCE
 // One step of the reference's lens integrator (electrostatic_lens.py:91-111): RK4 with the second
 // stage at the full Euler point x + dt k1; the new row stores a = l1.
 template <int FORCE>
-__device__ __forceinline__ void ref_step(const Force<FORCE> &F, const Tables &T, int state, double dt, MState &m,
-                                         uint32_t &status) {
+__device__ __forceinline__ void ref_step(const Force<FORCE> &F, const Tables &T, const lds_double *wsb, double dt,
+                                         MState &m, uint32_t &status) {
   const double x = m.x, y = m.y, z = m.z;
   const double k1x = m.vx, k1y = m.vy, k1z = m.vz;
   double l1x, l1y;
-  F(T, state, x, y, l1x, l1y, status);
+  F(T, wsb, x, y, l1x, l1y, status);
   const double k2x = k1x + (dt * l1x) / 2.0;
   const double k2y = k1y + (dt * l1y) / 2.0;
   const double k2z = k1z + (dt * 0.0) / 2.0;
   double l2x, l2y;
-  F(T, state, x + dt * k1x, y + dt * k1y, l2x, l2y, status);
+  F(T, wsb, x + dt * k1x, y + dt * k1y, l2x, l2y, status);
   const double k3x = k1x + (dt * l2x) / 2.0;
   const double k3y = k1y + (dt * l2y) / 2.0;
   const double k3z = k1z + (dt * 0.0) / 2.0;
   double l3x, l3y;
-  F(T, state, x + (dt * k2x) / 2.0, y + (dt * k2y) / 2.0, l3x, l3y, status);
+  F(T, wsb, x + (dt * k2x) / 2.0, y + (dt * k2y) / 2.0, l3x, l3y, status);
   const double k4x = k1x + dt * l3x;
   const double k4y = k1y + dt * l3y;
   const double k4z = k1z + dt * 0.0;
   double l4x, l4y;
-  F(T, state, x + dt * k3x, y + dt * k3y, l4x, l4y, status);
+  F(T, wsb, x + dt * k3x, y + dt * k3y, l4x, l4y, status);
   m.x = x + (dt * (((k1x + 2.0 * k2x) + 2.0 * k3x) + k4x)) / 6.0;
   m.y = y + (dt * (((k1y + 2.0 * k2y) + 2.0 * k3y) + k4y)) / 6.0;
   m.z = z + (dt * (((k1z + 2.0 * k2z) + 2.0 * k3z) + k4z)) / 6.0;
@@ -259,11 +268,12 @@ template <int FORCE, class Sink>
 __device__ __forceinline__ bool lens_ref(const ctraj_elem &E, const Tables &T, int state, double g, MState &m,
                                          uint32_t &status, WalkCounters &wc, Sink &sink) {
   const Force<FORCE> F = make_force<FORCE>(E, g);
+  const lds_double *wsb = ws_of(T, state);
   const double half_d = E.p[0];
   const int N = E.n_steps;
   const double dt = E.p[2] / m.vz;
   for (int i = 0; i < N; ++i) {
-    ref_step<FORCE>(F, T, state, dt, m, status);
+    ref_step<FORCE>(F, T, wsb, dt, m, status);
     ++wc.steps;
     if constexpr (Sink::kRows) sink.row(m);
     if (rho_of(m.x, m.y) > half_d) return false;
@@ -281,20 +291,20 @@ constexpr double kSixth = 1.0 / 6.0;
 // as soon as each stage is known, which keeps few FP64 values live.  Every FMA here is
 // mirrored by the CPU oracle.
 template <int FORCE>
-__device__ __forceinline__ void rk4_step(const Force<FORCE> &F, const Tables &T, int state, double h, double &x,
-                                         double &y, double &vx, double &vy, double r2, uint32_t &status) {
+__device__ __forceinline__ void rk4_step(const Force<FORCE> &F, const Tables &T, const lds_double *wsb, double h,
+                                         double &x, double &y, double &vx, double &vy, double r2, uint32_t &status) {
   const double hh = 0.5 * h, h6 = h * kSixth;
   double ax, ay;
   if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX) {
-    force_hex_r2(T, state, F.ks, F.cu, F.g, x, y, r2, ax, ay);
+    force_hex_r2(wsb, T.umax, F.ks, F.cu, F.g, x, y, r2, ax, ay);
   } else {
-    F(T, state, x, y, ax, ay, status);
+    F(T, wsb, x, y, ax, ay, status);
   }
   // stage 2 point
   double xs = __builtin_fma(hh, vx, x), ys = __builtin_fma(hh, vy, y);
   double ux = __builtin_fma(hh, ax, vx), uy = __builtin_fma(hh, ay, vy);
   double svx = ax, svy = ay;
-  F(T, state, xs, ys, ax, ay, status);
+  F(T, wsb, xs, ys, ax, ay, status);
   double sx = __builtin_fma(2.0, ux, vx), sy = __builtin_fma(2.0, uy, vy);
   svx = __builtin_fma(2.0, ax, svx);
   svy = __builtin_fma(2.0, ay, svy);
@@ -303,7 +313,7 @@ __device__ __forceinline__ void rk4_step(const Force<FORCE> &F, const Tables &T,
   ys = __builtin_fma(hh, uy, y);
   ux = __builtin_fma(hh, ax, vx);
   uy = __builtin_fma(hh, ay, vy);
-  F(T, state, xs, ys, ax, ay, status);
+  F(T, wsb, xs, ys, ax, ay, status);
   sx = __builtin_fma(2.0, ux, sx);
   sy = __builtin_fma(2.0, uy, sy);
   svx = __builtin_fma(2.0, ax, svx);
@@ -313,7 +323,7 @@ __device__ __forceinline__ void rk4_step(const Force<FORCE> &F, const Tables &T,
   ys = __builtin_fma(h, uy, y);
   ux = __builtin_fma(h, ax, vx);
   uy = __builtin_fma(h, ay, vy);
-  F(T, state, xs, ys, ax, ay, status);
+  F(T, wsb, xs, ys, ax, ay, status);
   sx = sx + ux;
   sy = sy + uy;
   svx = svx + ax;
@@ -343,6 +353,7 @@ template <int FORCE, class Sink>
 __device__ __forceinline__ bool lens_rk4(const ctraj_elem &E, const Tables &T, int state, double g, int cap,
                                          MState &m, uint32_t &status, WalkCounters &wc, Sink &sink) {
   const Force<FORCE> F = make_force<FORCE>(E, g);
+  const lds_double *wsb = ws_of(T, state);
   const double R2 = E.p[1];
   const double h = E.p[2];
   const double L = E.p[3];
@@ -354,12 +365,12 @@ __device__ __forceinline__ bool lens_rk4(const ctraj_elem &E, const Tables &T, i
   bool alive = true;
   for (int i = 0; i <= nfull; ++i) {
     if (i < nfull) {
-      rk4_step<FORCE>(F, T, state, h, x, y, vx, vy, r2, status);
+      rk4_step<FORCE>(F, T, wsb, h, x, y, vx, vy, r2, status);
       z = z + dzs;
       t = t + h;
     } else {
       const double hp = L / m.vz - (double)nfull * h;
-      rk4_step<FORCE>(F, T, state, hp, x, y, vx, vy, r2, status);
+      rk4_step<FORCE>(F, T, wsb, hp, x, y, vx, vy, r2, status);
       z = E.z1;
       t = t + hp;
     }
@@ -369,7 +380,7 @@ __device__ __forceinline__ bool lens_rk4(const ctraj_elem &E, const Tables &T, i
       MState r = m;
       r.x = x; r.y = y; r.z = z; r.vx = vx; r.vy = vy; r.t = t;
       if (i < nfull || r2 > R2) {
-        F(T, state, x, y, r.ax, r.ay, status);
+        F(T, wsb, x, y, r.ax, r.ay, status);
       } else {
         r.ax = 0.0; r.ay = -g;
       }
@@ -386,7 +397,7 @@ __device__ __forceinline__ bool lens_rk4(const ctraj_elem &E, const Tables &T, i
     m.ax = 0.0; m.ay = -g; m.az = 0.0;
   } else {
     uint32_t st = 0;
-    F(T, state, x, y, m.ax, m.ay, st);
+    F(T, wsb, x, y, m.ax, m.ay, st);
     m.az = 0.0;
   }
   return alive;
@@ -441,13 +452,14 @@ __device__ __forceinline__ void walk(const ctraj_elem *__restrict__ elems, int n
       ++wc.faces;
       if constexpr (Sink::kRows) sink.row(m);
       if (!inside) { label = E.label_hit; return; }
-    } else if (E.kind == CTRAJ_LENS) {
+    } else if (INTEG != CTRAJ_INTEG_NONE && E.kind == CTRAJ_LENS) {
       // ElectrostaticLens.propagate_through (electrostatic_lens.py:48-77)
       drift_dt(m, time_to(m, E.z0), g);
       ++wc.faces;
       if constexpr (Sink::kRows) sink.row(m);
       if (rho_of(m.x, m.y) > E.p[0]) { label = E.label_hit; return; }
-      if constexpr (INTEG == CTRAJ_INTEG_REF) {
+      if constexpr (INTEG == CTRAJ_INTEG_NONE) {
+      } else if constexpr (INTEG == CTRAJ_INTEG_REF) {
         if (!lens_ref<FORCE>(E, T, state, g, m, status, wc, sink)) { label = E.label_inside; return; }
         // exit hop with the stored a = l1 of the last step (electrostatic_lens.py:74-77)
         drift_dt(m, time_to(m, E.z1), g);

@@ -49,6 +49,7 @@ struct KBeam {
 struct KTab {
   const double *ws;
   int ws_nseg, ws_states;
+  double umax;  // largest double below ws_nseg
   const double *rt;
   int rt_total;
 };

@@ -130,8 +130,8 @@ class WSpline:
 
     # host evaluation (tests, plotting) -- the same segment rule as the device
     def _seg(self, E):
-        u = np.minimum(np.asarray(E, dtype=np.float64) / self.dE, float(self.n_seg))
-        k = np.minimum(u.astype(np.int64), self.n_seg - 1)
+        u = np.minimum(np.asarray(E, dtype=np.float64) / self.dE, np.nextafter(float(self.n_seg), 0.0))
+        k = u.astype(np.int64)
         return u, k, u - k
 
     def dW(self, E):

@@ -23,6 +23,14 @@ from .engine import Engine
 from .molecule import Molecule, Trajectory
 
 
+def reference_chunking(N_traj: int, n_jobs: int):
+    """(N_loops, N, total) of run_simulation: N_loops = 100*n_jobs chunks of N = int(N_traj/N_loops)
+    molecules, so total = N*N_loops are simulated (trajectory_simulator.py:48-49)."""
+    N_loops = 100 * n_jobs
+    N = int(N_traj / N_loops)
+    return N_loops, N, N * N_loops
+
+
 class Counter:
     """Label -> count, insertion-ordered (trajectory_simulator.py:106-158)."""
 
@@ -127,9 +135,7 @@ class TrajectorySimulator:
 
         ``n_jobs`` only sets the reference's chunking (and hence which molecules are drawn); the
         work runs on the GPU.  ``states`` (extension) gives each molecule a lens state id."""
-        N_loops = 100 * n_jobs
-        N = int(N_traj / N_loops)
-        total = N * N_loops
+        N_loops, N, total = reference_chunking(N_traj, n_jobs)
         vs, xs = [], []
         for _ in range(N_loops):
             vs.append(np.asarray(vdist.draw(N), dtype=np.float64).reshape(3, N))

@@ -93,12 +93,11 @@ static void force_rtable(const OTab *T, int id, double g, double x, double y, do
   *ay = fy - g;
 }
 
-/* W'(u) on the spline of one state: segment trunc(u) (u clamped to the table end, segment to the
- * last one), s = u - seg, W' = (A s + B) s + C. */
+/* W'(u) on the spline of one state: u clamped to the largest double below nseg, segment k = trunc(u),
+ * s = u - k (exact), W' = (A s + B) s + C. */
 static double wspline_eval(const OTab *T, int state, double u) {
-  u = fmin(u, (double)T->nseg);
-  int k = (int)u;
-  if (k > T->nseg - 1) k = T->nseg - 1;
+  u = fmin(u, nextafter((double)T->nseg, 0.0));
+  const int k = (int)u;
   const double s = u - (double)k;
   const double *c = T->ws + 4 * ((size_t)state * T->nseg + k);
   return fma(fma(c[0], s, c[1]), s, c[2]);

@@ -129,8 +129,8 @@ class LensForce:
             spl = self.splines[state]
             r2 = x[0] * x[0] + x[1] * x[1]
             if self.kind == "hexapole":
-                u = min(r2 * (self.cE / spl.dE), float(spl.n_seg))
-                k = min(int(u), spl.n_seg - 1)
+                u = min(r2 * (self.cE / spl.dE), np.nextafter(float(spl.n_seg), 0.0))
+                k = int(u)
                 s = u - k
                 A, B, C = spl.coef[k, 0], spl.coef[k, 1], spl.coef[k, 2]
                 f = ((A * s + B) * s + C) * (-2.0 * self.cE / self.el.mass)
@@ -139,8 +139,8 @@ class LensForce:
             else:
                 r = np.sqrt(r2)
                 if r != 0:
-                    u = min(r * (self.cE / spl.dE), float(spl.n_seg))
-                    k = min(int(u), spl.n_seg - 1)
+                    u = min(r * (self.cE / spl.dE), np.nextafter(float(spl.n_seg), 0.0))
+                    k = int(u)
                     s = u - k
                     A, B, C = spl.coef[k, 0], spl.coef[k, 1], spl.coef[k, 2]
                     f = ((A * s + B) * s + C) * (-self.el.mass ** -1 * self.cE) / r
