@@ -82,39 +82,56 @@ __device__ __forceinline__ void wave_hist(uint32_t *s_hist, unsigned long long *
 }  // namespace ctraj
 
 // ---------------------------------------------------------------------------------------------
-// k_propagate: one molecule per thread, grid-stride over the SoA store.  The loop runs in
-// workgroup-uniform chunks so that kept (aoi) molecules can be staged in LDS and flushed to the
-// dense output with one global atomic per ~AOI_BUF/2 molecules instead of one per wavefront pass.
+// k_propagate: one molecule per thread, grid-stride over the SoA store.  Each wavefront stages its
+// kept (aoi) molecules in its own LDS slice and flushes them to the dense output with one global
+// atomic per ~AOI_WAVE/2 molecules (no workgroup barrier in the loop).  The next molecule's inputs
+// are loaded before the current one is walked, so HBM latency overlaps the walk.
 // INTEG = CTRAJ_INTEG_NONE instantiates the walker without lens code (beamlines with no lens).
 // ---------------------------------------------------------------------------------------------
-constexpr int AOI_BUF = 512;
+constexpr int AOI_WAVE = 128;          // staged entries per wavefront
+constexpr int AOI_ENTRY_BYTES = 8 + 4 + 7 * 8;
 
 struct AoiStage {
-  long long *idx;  // [AOI_BUF]
-  int *label;      // [AOI_BUF]
-  double *val;     // [7][AOI_BUF]
+  long long *idx;  // [AOI_WAVE]
+  int *label;      // [AOI_WAVE]
+  double *val;     // [7][AOI_WAVE]
 };
 
-__device__ __forceinline__ void aoi_flush(const ctraj_run &R, AoiStage &A, int *s_n, unsigned long long *s_base) {
-  __syncthreads();
-  const int cnt = *s_n;
-  if (threadIdx.x == 0) *s_base = atomicAdd((unsigned long long *)R.aoi_count, (unsigned long long)cnt);
-  __syncthreads();
-  const int64_t base = (int64_t)*s_base;
+// Copy a wavefront's staged entries [0, cnt) to the output (called by the whole wavefront).
+__device__ __forceinline__ void aoi_flush_wave(const ctraj_run &R, const AoiStage &A, int cnt, int lane) {
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd((unsigned long long *)R.aoi_count, (unsigned long long)cnt);
+  base = __shfl(base, 0);
   double *outs[7] = {R.aoi_out.x, R.aoi_out.y, R.aoi_out.z, R.aoi_out.vx, R.aoi_out.vy, R.aoi_out.vz, R.aoi_out.t};
-  for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
-    const int64_t slot = base + j;
+  for (int j = lane; j < cnt; j += 64) {
+    const int64_t slot = (int64_t)base + j;
     if (slot < R.aoi_cap) {
       R.aoi_index[slot] = A.idx[j];
       if (R.aoi_label) R.aoi_label[slot] = A.label[j];
 #pragma unroll
       for (int f = 0; f < 7; ++f)
-        if (outs[f]) outs[f][slot] = A.val[f * AOI_BUF + j];
+        if (outs[f]) outs[f][slot] = A.val[f * AOI_WAVE + j];
     }
   }
-  __syncthreads();
-  if (threadIdx.x == 0) *s_n = 0;
-  __syncthreads();
+}
+
+struct InMol {
+  double x, y, z, vx, vy, vz, t, ax, ay, az;
+  int st;
+};
+
+__device__ __forceinline__ void load_mol(const ctraj_run &R, const KBeam &B, int64_t i, InMol &q) {
+  q.x = R.in.x[i];
+  q.y = R.in.y[i];
+  q.z = R.in.z ? R.in.z[i] : R.z_const;
+  q.vx = R.in.vx[i];
+  q.vy = R.in.vy[i];
+  q.vz = R.in.vz[i];
+  q.t = R.in.t ? R.in.t[i] : R.t_const;
+  q.ax = R.a0[0] ? R.a0[0][i] : 0.0;
+  q.ay = R.a0[1] ? R.a0[1][i] : -B.g;
+  q.az = R.a0[2] ? R.a0[2][i] : 0.0;
+  q.st = R.state ? min((int)R.state[i], B.n_states - 1) : 0;
 }
 
 template <int FORCE, int INTEG>
@@ -123,46 +140,43 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_work[4];
-  __shared__ int s_aoi_n;
-  __shared__ unsigned long long s_aoi_base;
   const int nbins = B.n_states * B.n_labels;
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0;
   for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x) s_first[b] = ~0ull >> 1;
   if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
-  if (threadIdx.x == 0) s_aoi_n = 0;
   Tables T{};
   if constexpr (INTEG != CTRAJ_INTEG_NONE) T = stage_tables(lds, KT);
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
   AoiStage A;
-  A.idx = reinterpret_cast<long long *>(lds + aoi_lds_off);
-  A.label = reinterpret_cast<int *>(lds + aoi_lds_off + AOI_BUF);
-  A.val = lds + aoi_lds_off + AOI_BUF + AOI_BUF / 2;
+  {
+    char *base = reinterpret_cast<char *>(lds + aoi_lds_off) + (size_t)wave * AOI_WAVE * AOI_ENTRY_BYTES;
+    A.idx = reinterpret_cast<long long *>(base);
+    A.val = reinterpret_cast<double *>(base + AOI_WAVE * 8);
+    A.label = reinterpret_cast<int *>(base + AOI_WAVE * 64);
+  }
+  int aoi_n = 0;  // entries staged by this wavefront (wave-uniform)
   __syncthreads();
 
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   WalkCounters wc{0u, 0u};
   uint32_t n_oot = 0, n_cap = 0;
-  const int lane = lane_id();
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < R.n; base += stride) {
-    const int64_t i = base + threadIdx.x;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  InMol q;
+  if (i < R.n) load_mol(R, B, i, q);
+  // wave-uniform trip count: every lane of a wavefront runs the same iterations
+  for (int64_t wbase = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < R.n; wbase += stride, i += stride) {
     const bool active = i < R.n;
-    int label = 0;
     MState m;
+    m.x = q.x; m.y = q.y; m.z = q.z; m.vx = q.vx; m.vy = q.vy; m.vz = q.vz; m.t = q.t;
+    m.ax = q.ax; m.ay = q.ay; m.az = q.az;
+    const int st = q.st;
+    if (i + stride < R.n) load_mol(R, B, i + stride, q);  // prefetch the next molecule
+    int label = 0;
     if (active) {
-      m.x = R.in.x[i];
-      m.y = R.in.y[i];
-      m.z = R.in.z ? R.in.z[i] : R.z_const;
-      m.vx = R.in.vx[i];
-      m.vy = R.in.vy[i];
-      m.vz = R.in.vz[i];
-      m.t = R.in.t ? R.in.t[i] : R.t_const;
-      m.ax = R.a0[0] ? R.a0[0][i] : 0.0;
-      m.ay = R.a0[1] ? R.a0[1][i] : -B.g;
-      m.az = R.a0[2] ? R.a0[2][i] : 0.0;
-      const int st = R.state ? min((int)R.state[i], B.n_states - 1) : 0;
       uint32_t status = 0;
       NullSink sk;
       walk<FORCE, INTEG>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status, wc, sk);
-
       if (R.out.x) R.out.x[i] = m.x;
       if (R.out.y) R.out.y[i] = m.y;
       if (R.out.z) R.out.z[i] = m.z;
@@ -184,31 +198,27 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
       const bool keep = active && ((R.aoi_mask >> label) & 1ull);
       const uint64_t mask = __ballot(keep);
       if (mask) {
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        int slot0 = 0;
-        if (lane == leader) slot0 = atomicAdd(&s_aoi_n, __popcll(mask));
-        slot0 = __shfl(slot0, leader);
         if (keep) {
-          const int j = slot0 + lanes_below(mask);
+          const int j = aoi_n + lanes_below(mask);
           A.idx[j] = R.index_base + i;
           A.label[j] = label;
-          A.val[0 * AOI_BUF + j] = m.x;
-          A.val[1 * AOI_BUF + j] = m.y;
-          A.val[2 * AOI_BUF + j] = m.z;
-          A.val[3 * AOI_BUF + j] = m.vx;
-          A.val[4 * AOI_BUF + j] = m.vy;
-          A.val[5 * AOI_BUF + j] = m.vz;
-          A.val[6 * AOI_BUF + j] = m.t;
+          A.val[0 * AOI_WAVE + j] = m.x;
+          A.val[1 * AOI_WAVE + j] = m.y;
+          A.val[2 * AOI_WAVE + j] = m.z;
+          A.val[3 * AOI_WAVE + j] = m.vx;
+          A.val[4 * AOI_WAVE + j] = m.vy;
+          A.val[5 * AOI_WAVE + j] = m.vz;
+          A.val[6 * AOI_WAVE + j] = m.t;
+        }
+        aoi_n += __popcll(mask);
+        if (aoi_n > AOI_WAVE - 64) {
+          aoi_flush_wave(R, A, aoi_n, lane);
+          aoi_n = 0;
         }
       }
-      __syncthreads();
-      if (s_aoi_n > AOI_BUF - (int)blockDim.x) aoi_flush(R, A, &s_aoi_n, &s_aoi_base);
     }
   }
-  if (R.aoi_mask) {
-    __syncthreads();
-    if (s_aoi_n > 0) aoi_flush(R, A, &s_aoi_n, &s_aoi_base);
-  }
+  if (R.aoi_mask && aoi_n > 0) aoi_flush_wave(R, A, aoi_n, lane);
   if (R.work) {
     if (wc.faces) atomicAdd(&s_work[0], (unsigned long long)wc.faces);
     if (wc.steps) atomicAdd(&s_work[1], (unsigned long long)wc.steps);
@@ -1016,9 +1026,9 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
     if (n_lens == 0) kt.ws_states = kt.rt_total = 0;
     size_t lds_tab = n_lens > 0 ? lds : 0;
-    // kept-molecule staging area after the tables: int64 idx, int32 label, 7 doubles per entry
+    // kept-molecule staging area after the tables: AOI_WAVE entries per wavefront
     const int aoi_off = (int)(lds_tab / sizeof(double));
-    const size_t lds_all = lds_tab + (R->aoi_mask ? (size_t)AOI_BUF * (8 + 4 + 7 * 8) : 0);
+    const size_t lds_all = lds_tab + (R->aoi_mask ? (size_t)(block / 64) * AOI_WAVE * AOI_ENTRY_BYTES : 0);
     int rc = lds_limit(c, (const void *)k, lds_all);
     if (rc) return rc;
     const int grid = grid_for(c, (const void *)k, block, lds_all, R->n, R->grid);
