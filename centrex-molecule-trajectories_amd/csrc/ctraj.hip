@@ -21,65 +21,9 @@
 #include "ctraj_internal.hpp"
 #include "ctraj_lanes.hpp"
 #include "ctraj_sinks.hpp"
+#include "ctraj_wave.hpp"
 
 using namespace ctraj;
-
-// ---------------------------------------------------------------------------------------------
-// LDS staging of the lens tables
-// ---------------------------------------------------------------------------------------------
-namespace ctraj {
-
-// The W' spline is kept in LDS as [state][seg][3] = {A, B, C}: a 24-byte segment stride spreads
-// the lanes' random segments over the LDS banks far better than the 32-byte stride of the global
-// table (which puts every segment on one of 8 bank groups).  r tables keep [knot][4].
-__device__ Tables stage_tables(double *lds, const KTab &kt) {
-  Tables T;
-  const int nseg_all = kt.ws_states * kt.ws_nseg;
-  const int nws = nseg_all * 3;
-  const int nrt = kt.rt_total * 4;
-  for (int i = threadIdx.x; i < nseg_all; i += blockDim.x) {
-    const double2 ab = reinterpret_cast<const double2 *>(kt.ws)[2 * i];
-    const double c = kt.ws[4 * i + 2];
-    lds[3 * i] = ab.x;
-    lds[3 * i + 1] = ab.y;
-    lds[3 * i + 2] = c;
-  }
-  const double2 *src_rt = reinterpret_cast<const double2 *>(kt.rt);
-  double2 *dst_rt = reinterpret_cast<double2 *>(lds + nws + (nws & 1));
-  for (int i = threadIdx.x; i < nrt / 2; i += blockDim.x) dst_rt[i] = src_rt[i];
-  const lds_double *base = (const lds_double *)lds;
-  T.ws = base;
-  T.nseg = kt.ws_nseg;
-  T.umax = kt.umax;
-  T.rt = base + nws + (nws & 1);
-  return T;
-}
-
-__device__ __forceinline__ int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
-
-__device__ __forceinline__ int lanes_below(uint64_t mask) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-// Wave-aggregated histogram: one LDS atomic per distinct key in the wave.  The leader of each
-// key group is its lowest lane, which also holds the group's lowest molecule index.
-__device__ __forceinline__ void wave_hist(uint32_t *s_hist, unsigned long long *s_first, int key, int label,
-                                          int64_t gidx) {
-  uint64_t todo = __ballot(1);
-  const int lane = lane_id();
-  while (todo) {
-    const int leader = __ffsll((unsigned long long)todo) - 1;
-    const int lk = __shfl(key, leader);
-    const uint64_t same = __ballot(key == lk) & todo;
-    if (lane == leader) {
-      atomicAdd(&s_hist[lk], (uint32_t)__popcll(same));
-      if (s_first) atomicMin(&s_first[label], (unsigned long long)gidx);
-    }
-    todo &= ~same;
-  }
-}
-
-}  // namespace ctraj
 
 // ---------------------------------------------------------------------------------------------
 // k_propagate: one molecule per thread, grid-stride over the SoA store.  Each wavefront stages its
@@ -711,6 +655,23 @@ int grid_for(ctraj_ctx *ctx, const void *kern, int block, size_t lds, int64_t n,
   return (int)std::max<int64_t>(1, std::min(need, full));
 }
 
+int ensure_buf(ctraj_ctx *ctx, ctraj_devbuf &b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return CTRAJ_OK;
+  if (b.p) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  if (hipMalloc(&b.p, bytes) != hipSuccess) {
+    b.p = nullptr;
+    (void)hipGetLastError();
+    return fail(ctx, CTRAJ_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+  }
+  b.bytes = bytes;
+  return CTRAJ_OK;
+}
+
 }  // namespace ctraj
 
 // kernel tables over (force, integrator)
@@ -803,6 +764,8 @@ int32_t ctraj_ctx_destroy(ctraj_ctx *c) {
   if (c->d_rt) (void)hipFree(c->d_rt);
   if (c->d_counter) (void)hipFree(c->d_counter);
   if (c->d_run) (void)hipFree(c->d_run);
+  if (c->sm_ws.p) (void)hipFree(c->sm_ws.p);
+  if (c->sm_part.p) (void)hipFree(c->sm_part.p);
   for (int i = 0; i < 64; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   (void)hipStreamDestroy(c->stream);

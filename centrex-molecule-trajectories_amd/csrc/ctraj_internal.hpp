@@ -9,6 +9,12 @@
 
 #include "../../include/ctraj.h"
 
+// Grow-only device buffer owned by a context.
+struct ctraj_devbuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
 struct ctraj_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -35,6 +41,8 @@ struct ctraj_ctx {
   ctraj_run last_run{};
   bool have_last_run = false;
   hipEvent_t ev[64];
+  // smooth objective: per-wave checkpoint/recompute store and per-item partial sums
+  ctraj_devbuf sm_ws, sm_part;
 };
 
 namespace ctraj {
@@ -66,6 +74,8 @@ int check_phys(ctraj_ctx *ctx, const ctraj_phys *p);
 KBeam make_kbeam(const ctraj_elem *d_elems, int n, const ctraj_phys &p);
 int lds_limit(ctraj_ctx *ctx, const void *kern, size_t bytes);
 int grid_for(ctraj_ctx *ctx, const void *kern, int block, size_t lds, int64_t n, int req);
+// make b hold at least `bytes` (grow-only; the old buffer is released first)
+int ensure_buf(ctraj_ctx *ctx, ctraj_devbuf &b, size_t bytes);
 
 #define HIPCHK(ctx, call)                                         \
   do {                                                            \

@@ -1,0 +1,71 @@
+// ctraj_wave.hpp -- LDS table staging and wavefront helpers shared by the kernel files.
+#pragma once
+
+#include "ctraj_device.hpp"
+#include "ctraj_internal.hpp"
+
+// ---------------------------------------------------------------------------------------------
+// LDS staging of the lens tables
+// ---------------------------------------------------------------------------------------------
+namespace ctraj {
+
+// The W' spline is kept in LDS as [state][seg][3] = {A, B, C}: a 24-byte segment stride spreads
+// the lanes' random segments over the LDS banks far better than the 32-byte stride of the global
+// table (which puts every segment on one of 8 bank groups).  r tables keep [knot][4].
+__device__ inline Tables stage_tables(double *lds, const KTab &kt) {
+  Tables T;
+  const int nseg_all = kt.ws_states * kt.ws_nseg;
+  const int nws = nseg_all * 3;
+  const int nrt = kt.rt_total * 4;
+  for (int i = threadIdx.x; i < nseg_all; i += blockDim.x) {
+    const double2 ab = reinterpret_cast<const double2 *>(kt.ws)[2 * i];
+    const double c = kt.ws[4 * i + 2];
+    lds[3 * i] = ab.x;
+    lds[3 * i + 1] = ab.y;
+    lds[3 * i + 2] = c;
+  }
+  const double2 *src_rt = reinterpret_cast<const double2 *>(kt.rt);
+  double2 *dst_rt = reinterpret_cast<double2 *>(lds + nws + (nws & 1));
+  for (int i = threadIdx.x; i < nrt / 2; i += blockDim.x) dst_rt[i] = src_rt[i];
+  const lds_double *base = (const lds_double *)lds;
+  T.ws = base;
+  T.nseg = kt.ws_nseg;
+  T.umax = kt.umax;
+  T.rt = base + nws + (nws & 1);
+  return T;
+}
+
+__device__ __forceinline__ int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Wave-aggregated histogram: one LDS atomic per distinct key in the wave.  The leader of each
+// key group is its lowest lane, which also holds the group's lowest molecule index.
+__device__ __forceinline__ void wave_hist(uint32_t *s_hist, unsigned long long *s_first, int key, int label,
+                                          int64_t gidx) {
+  uint64_t todo = __ballot(1);
+  const int lane = lane_id();
+  while (todo) {
+    const int leader = __ffsll((unsigned long long)todo) - 1;
+    const int lk = __shfl(key, leader);
+    const uint64_t same = __ballot(key == lk) & todo;
+    if (lane == leader) {
+      atomicAdd(&s_hist[lk], (uint32_t)__popcll(same));
+      if (s_first) atomicMin(&s_first[label], (unsigned long long)gidx);
+    }
+    todo &= ~same;
+  }
+}
+
+
+// Sum over the 64 lanes by an xor butterfly: every lane receives the same, bitwise identical
+// total (each level adds the same two partial sums, only in swapped order), so the result does not
+// depend on which lane is read.  All 64 lanes must be active.
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 1; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace ctraj

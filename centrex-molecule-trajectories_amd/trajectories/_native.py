@@ -142,14 +142,17 @@ class Smooth(C.Structure):
         ("vy", dptr),
         ("eps", dptr),
         ("z_src", C.c_double),
-        ("mu_vz", C.c_double),
         ("sigma_vz", C.c_double),
         ("softness", C.c_double),
-        ("ckpt_every", C.c_int32),
-        ("block", C.c_int32),
-        ("J_host", dptr),
-        ("grad_host", dptr),
+        ("n_cfg", C.c_int32),
         ("n_params", C.c_int32),
+        ("params", dptr),
+        ("J", dptr),
+        ("grad", dptr),
+        ("n_capped", dptr),
+        ("ckpt_every", C.c_int32),
+        ("max_steps", C.c_int32),
+        ("block", C.c_int32),
         ("reserved", C.c_int32),
     ]
 
@@ -184,6 +187,7 @@ SIGNATURES = [
     ("ctraj_batch_run", C.c_int32,
      [C.c_void_p, C.POINTER(BatchTune), C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     ("ctraj_batch_destroy", C.c_int32, [C.c_void_p]),
+    ("ctraj_smooth_params", C.c_int32, [C.c_void_p, C.POINTER(C.c_int32), C.c_void_p]),
     ("ctraj_smooth_objective", C.c_int32, [C.c_void_p, C.POINTER(Smooth)]),
 ]
 

@@ -111,3 +111,30 @@ def config2_params(n_cfg: int = 256, seed: int = 2):
     r = rs.uniform(2e-3, 5e-3, n_cfg)
     n = np.round(10.0 ** rs.uniform(3.0, 6.0, n_cfg)).astype(np.int64)
     return V0, z0, r, n
+
+
+# Differentiable objective: config-1 geometry, theta = (r1, V0, lens z0, r2, r3, mu_vz) in beamline
+# order, sigmoid width 50 um, lens checkpoints every 64 steps.
+CONFIG3_SOFTNESS = 50e-6
+CONFIG3_CKPT = 64
+
+
+def config3_source(n: int, seed: int = 3):
+    """(x, y, vx, vy, eps) of the differentiable workload: the benchmark source with vz
+    reparameterised as mu + 16 eps (eps standard normal, drawn where the source draws vz)."""
+    X, V = draw_source(n, seed)
+    rs = np.random.RandomState(seed)
+    loops = 100
+    N = n // loops
+    eps = np.empty(n)
+    for c in range(loops):
+        rs.standard_normal(N)
+        rs.standard_normal(N)
+        eps[c * N:(c + 1) * N] = rs.standard_normal(N)
+        rs.standard_normal(N)
+        rs.standard_normal(N)
+    return X[0], X[1], V[0], V[1], eps
+
+
+def config3_theta() -> np.ndarray:
+    return np.array([6e-3, 30e3, 0.8, 3e-3, 4e-3, VZ_MEAN])

@@ -213,22 +213,36 @@ int32_t ctraj_batch_run(ctraj_batch *b, const ctraj_batch_tune *tune, int64_t *t
                         float *elapsed_ms);
 int32_t ctraj_batch_destroy(ctraj_batch *b);
 
-/* ---- differentiable detected flux and its adjoint ----
- * J = (1/n) sum_i prod_{CIRC faces f} sigmoid((r_f - rho_i,f) / softness); no molecule is stopped.
- * Parameters, in beamline order: each CIRC contributes d/2; each LENS contributes (V, z0); then mean vz.
- * Only CIRC and LENS(WSPLINE_HEX, INTEG_RK4) elements are supported. */
+/* ---- differentiable detected flux and its adjoint (no counterpart in the reference: the objective
+ * and gradients of the sigmoid-smoothed detected flux over the beamline of ctraj_set_beamline) ----
+ * For parameter set c and molecule i: vz_i = mu_c + sigma_vz * eps_i (molecules with vz_i <= 0
+ * contribute 0); the molecule starts at (x_i, y_i, z_src) with (vx_i, vy_i, vz_i) and is walked through
+ * every element without being stopped.  Each CIRC face (z0, and z1 when z1 > z0) contributes the factor
+ * sigmoid((r - rho) / softness), rho = sqrt(x^2 + y^2) on the face;  J_c = (1/n) sum_i prod factors.
+ * Lens bores and entrances are not factors.  Parameters of set c, in beamline order: each CIRC its
+ * radius r (its d/2 is ignored); each LENS (V, z0): field scaled by V / V_table (p[7], nonzero) and
+ * z1 = z0 + L; last, mu.  Only CIRC and LENS (WSPLINE_HEX, INTEG_RK4, lens state 0) elements are
+ * accepted.  The gradient is the exact derivative of the computed J (discrete adjoint of the drifts
+ * and of every RK4 step, including the final partial step), obtained by reverse sweeps over lens
+ * checkpoints taken every ckpt_every steps.  Results are deterministic: per-64-molecule partial sums
+ * are combined in a fixed order. */
 typedef struct ctraj_smooth {
-  int64_t n;
-  const double *x, *y, *vx, *vy, *eps; /* source; vz_i = mu_vz + sigma_vz * eps_i; z = z_src, t = 0 */
-  double z_src, mu_vz, sigma_vz;
-  double softness;                     /* sigmoid width in m */
-  int32_t ckpt_every;                  /* lens checkpoint interval K in steps (>= 1) */
-  int32_t block;
-  double *J_host;                      /* host [1] */
-  double *grad_host;                   /* host [n_params] or NULL (objective only) */
-  int32_t n_params;                    /* must equal the parameter count of the beamline */
+  int64_t n;                            /* molecules */
+  const double *x, *y, *vx, *vy, *eps;  /* device [n] */
+  double z_src, sigma_vz, softness;     /* softness: sigmoid width in m (> 0) */
+  int32_t n_cfg, n_params;              /* parameter sets in this launch; entries per set */
+  const double *params;                 /* device [n_cfg][n_params] */
+  double *J;                            /* device [n_cfg] */
+  double *grad;                         /* device [n_cfg][n_params], or NULL: objective only */
+  int64_t *n_capped;                    /* device [1] or NULL: += molecules whose lens step count was cut to max_steps */
+  int32_t ckpt_every;                   /* K >= 1 */
+  int32_t max_steps;                    /* full RK4 steps per lens at most (>= 1); bounds the checkpoint store */
+  int32_t block;                        /* threads per block, multiple of 64 (0 = 256) */
   int32_t reserved;
 } ctraj_smooth;
+/* number of parameters of the current beamline (CIRC: 1, LENS: 2, plus mu) and, when theta is not NULL,
+ * its first n_params - 1 entries as set in the element table (host array of at least n_params) */
+int32_t ctraj_smooth_params(ctraj_ctx *ctx, int32_t *n_params, double *theta);
 int32_t ctraj_smooth_objective(ctraj_ctx *ctx, const ctraj_smooth *s);
 
 #ifdef __cplusplus

@@ -13,9 +13,10 @@
  *   electrostatic_lens.py:48-118  lens entrance / RK4 variant / exit hop         -> lens_ref
  *   electrostatic_lens.py:168-228 lens_acceleration with interp1d (np.interp)   -> rtable_eval
  *   post_processing.py:9-75 take_timestep + find_radial_pos_dist resampling     -> orc_zgrid_from_rows
- * plus the configuration-mode lens the reference does not have (hexapole |E| = cE r^2, cubic
- * spline of W(|E|), classical RK4 at fixed dt with a final partial step), whose operation order,
- * including every fused multiply-add, is the one libctraj.so documents.
+ * plus two things the reference does not have: the configuration-mode lens (hexapole |E| = cE r^2,
+ * cubic spline of W(|E|), classical RK4 at fixed dt with a final partial step), whose operation
+ * order, including every fused multiply-add, is the one libctraj.so documents; and orc_smooth, the
+ * sigmoid-smoothed detected flux with its reverse-mode gradient (the CPU FP64 adjoint).
  *
  * Arithmetic: compiled with -ffp-contract=off; fma() is used exactly where the device code uses
  * __builtin_fma, so results are comparable bit for bit.  One deliberate difference from numpy:
@@ -498,3 +499,304 @@ void orc_draw_gaussian(uint64_t seed, uint64_t offset, int64_t n, int nd, const 
 }
 
 int orc_abi_version(void) { return CTRAJ_ABI_VERSION; }
+
+/*
+ * Sigmoid-smoothed detected flux and its gradient (the differentiable objective of libctraj.so's
+ * ctraj_smooth_objective; the reference has no counterpart).  For parameter set c and molecule i:
+ * vz = mu + sigma_vz * eps_i (vz <= 0 contributes 0); walk the beamline without stopping; every
+ * CIRC face (z0, and z1 when z1 > z0) multiplies by sigmoid((r - rho)/s); lenses: classical RK4
+ * as rk4_step with ks, cu scaled by V / p[7] and z1 = z0 + L.  Parameters in beamline order: CIRC r;
+ * LENS (V, z0); then mu.  J[c] = mean of the products; grad[c][j] = dJ/dtheta_j by reverse-mode
+ * differentiation of exactly these operations, with every lens step input stored (the device
+ * recomputes them from checkpoints; recomputation reproduces them bit for bit).
+ * per_mol (optional, [n_cfg][n][n_params + 1]) receives each molecule's gradient terms and product.
+ * Sums run over molecules in index order.
+ */
+static void sig2(double w, double *phi, double *om) {
+  const double e = exp(-fabs(w));
+  const double q = 1.0 / (1.0 + e);
+  const double eq = e * q;
+  *phi = w >= 0.0 ? q : eq;
+  *om = w >= 0.0 ? eq : q;
+}
+
+static void drift_g(double *x, double *y, double vx, double *vy, double tau, double g) {
+  *x = *x + vx * tau;
+  *y = (*y + *vy * tau) + ((-g) * (tau * tau)) * 0.5;
+  *vy = *vy + (-g) * tau;
+}
+
+typedef struct {
+  const double *ws; /* state-0 spline [nseg][4] */
+  double umax, ks, cu, kc2, invV, g;
+} SLens;
+
+static void hex_d(const SLens *L, double r2, double *f, double *d, double *gv) {
+  const double u0 = r2 * L->cu;
+  const double u = fmin(u0, L->umax);
+  const int k = (int)u;
+  const double s = u - (double)k;
+  const double *c = L->ws + 4 * (size_t)k;
+  const double P = fma(fma(c[0], s, c[1]), s, c[2]);
+  double dP = fma(2.0 * c[0], s, c[1]);
+  if (u0 > L->umax) dP = 0.0;
+  *f = P * L->ks;
+  *d = L->kc2 * dP;
+  *gv = (*f + (0.5 * *d) * r2) * L->invV;
+}
+
+static double rk4_adj(const SLens *L, double h, double x, double y, double vx, double vy, double *pbx, double *pby,
+                      double *vbx, double *vby, double *Vbar, int want_h) {
+  const double hh = 0.5 * h, h6 = h * (1.0 / 6.0), h3 = 2.0 * h6, g = L->g;
+  double f1, d1, g1, f2, d2, g2, f3, d3, g3, f4, d4, g4;
+  hex_d(L, fma(x, x, y * y), &f1, &d1, &g1);
+  const double a1x = f1 * x, a1y = fma(f1, y, -g);
+  const double p2x = fma(hh, vx, x), p2y = fma(hh, vy, y);
+  const double v2x = fma(hh, a1x, vx), v2y = fma(hh, a1y, vy);
+  hex_d(L, fma(p2x, p2x, p2y * p2y), &f2, &d2, &g2);
+  const double a2x = f2 * p2x, a2y = fma(f2, p2y, -g);
+  const double p3x = fma(hh, v2x, x), p3y = fma(hh, v2y, y);
+  const double v3x = fma(hh, a2x, vx), v3y = fma(hh, a2y, vy);
+  hex_d(L, fma(p3x, p3x, p3y * p3y), &f3, &d3, &g3);
+  const double a3x = f3 * p3x, a3y = fma(f3, p3y, -g);
+  const double p4x = fma(h, v3x, x), p4y = fma(h, v3y, y);
+  hex_d(L, fma(p4x, p4x, p4y * p4y), &f4, &d4, &g4);
+  const double pbx0 = *pbx, pby0 = *pby, vbx0 = *vbx, vby0 = *vby;
+
+  const double ab4x = h6 * vbx0, ab4y = h6 * vby0;
+  double q = p4x * ab4x + p4y * ab4y;
+  double dq = d4 * q;
+  const double pb4x = f4 * ab4x + dq * p4x, pb4y = f4 * ab4y + dq * p4y;
+  *Vbar = *Vbar + q * g4;
+  const double vb4x = h6 * pbx0, vb4y = h6 * pby0;
+  double Px = pbx0 + pb4x, Py = pby0 + pb4y;
+  const double vb3x = h3 * pbx0 + h * pb4x, vb3y = h3 * pby0 + h * pb4y;
+  double Vx = vbx0 + vb4x, Vy = vby0 + vb4y;
+  const double ab3x = h3 * vbx0 + h * vb4x, ab3y = h3 * vby0 + h * vb4y;
+  q = p3x * ab3x + p3y * ab3y;
+  dq = d3 * q;
+  const double pb3x = f3 * ab3x + dq * p3x, pb3y = f3 * ab3y + dq * p3y;
+  *Vbar = *Vbar + q * g3;
+  Px = Px + pb3x;
+  Py = Py + pb3y;
+  const double vb2x = h3 * pbx0 + hh * pb3x, vb2y = h3 * pby0 + hh * pb3y;
+  Vx = Vx + vb3x;
+  Vy = Vy + vb3y;
+  const double ab2x = h3 * vbx0 + hh * vb3x, ab2y = h3 * vby0 + hh * vb3y;
+  q = p2x * ab2x + p2y * ab2y;
+  dq = d2 * q;
+  const double pb2x = f2 * ab2x + dq * p2x, pb2y = f2 * ab2y + dq * p2y;
+  *Vbar = *Vbar + q * g2;
+  Px = Px + pb2x;
+  Py = Py + pb2y;
+  Vx = ((Vx + hh * pb2x) + vb2x) + h6 * pbx0;
+  Vy = ((Vy + hh * pb2y) + vb2y) + h6 * pby0;
+  const double ab1x = h6 * vbx0 + hh * vb2x, ab1y = h6 * vby0 + hh * vb2y;
+  q = x * ab1x + y * ab1y;
+  dq = d1 * q;
+  Px = Px + (f1 * ab1x + dq * x);
+  Py = Py + (f1 * ab1y + dq * y);
+  *Vbar = *Vbar + q * g1;
+  double hbar = 0.0;
+  if (want_h) {
+    const double a4x = f4 * p4x, a4y = fma(f4, p4y, -g);
+    const double v4x = fma(h, a3x, vx), v4y = fma(h, a3y, vy);
+    const double Svx = ((vx + 2.0 * v2x) + 2.0 * v3x) + v4x, Svy = ((vy + 2.0 * v2y) + 2.0 * v3y) + v4y;
+    const double Sax = ((a1x + 2.0 * a2x) + 2.0 * a3x) + a4x, Say = ((a1y + 2.0 * a2y) + 2.0 * a3y) + a4y;
+    hbar = (((pbx0 * Svx + pby0 * Svy) + (vbx0 * Sax + vby0 * Say)) / 6.0 +
+            0.5 * (((pb2x * vx + pb2y * vy) + (vb2x * a1x + vb2y * a1y)) +
+                   ((pb3x * v2x + pb3y * v2y) + (vb3x * a2x + vb3y * a2y)))) +
+           ((pb4x * v3x + pb4y * v3y) + (vb4x * a3x + vb4y * a3y));
+  }
+  *pbx = Px;
+  *pby = Py;
+  *vbx = Vx;
+  *vby = Vy;
+  return hbar;
+}
+
+static double exit_z_of(const ctraj_elem *el, const double *P, int e, int pidx_e, double z_src) {
+  if (e < 0) return z_src;
+  if (el[e].kind == CTRAJ_LENS) return P[pidx_e + 1] + el[e].p[3];
+  return el[e].z1;
+}
+
+/* one molecule; out[0..np-1] = gradient terms (if grad), out[np] = product.  Returns lens steps
+ * capped (0/1 per lens). */
+static int smooth_one(const ctraj_elem *el, int n_el, double g, const double *ws, int nseg, const double *P, int np,
+                      double x, double y, double vx, double vy, double vz, double z_src, double inv_s, int max_steps,
+                      int grad, double *fs /* [n_el*2][4] */, double *steps /* [max_steps+1][4] per lens */,
+                      double *out) {
+  for (int j = 0; j <= np; ++j) out[j] = 0.0;
+  if (!(vz > 0.0)) return 0;
+  const double umax = nextafter((double)nseg, 0.0);
+  double Jw = 1.0, zc = z_src;
+  int fslot = 0, pidx = 0, capped = 0, lens_i = 0;
+  const int per_lens = 4 * (max_steps + 1);
+  for (int e = 0; e < n_el; ++e) {
+    const ctraj_elem *E = &el[e];
+    if (E->kind == CTRAJ_CIRC) {
+      const double r = P[pidx];
+      const int nf = E->z1 > E->z0 ? 2 : 1;
+      for (int f = 0; f < nf; ++f) {
+        const double zf = f ? E->z1 : E->z0;
+        drift_g(&x, &y, vx, &vy, (zf - zc) / vz, g);
+        double *S = fs + 4 * fslot;
+        S[0] = x; S[1] = y; S[2] = vx; S[3] = vy;
+        double phi, om;
+        sig2((r - sqrt(x * x + y * y)) * inv_s, &phi, &om);
+        Jw = Jw * phi;
+        zc = zf;
+        ++fslot;
+      }
+      pidx += 1;
+    } else {
+      const double V = P[pidx], z0 = P[pidx + 1];
+      const double scale = V / E->p[7];
+      if (!(scale >= 0.0)) return capped; /* no table for a negative or NaN field scale: contributes 0 */
+      OForce F = {CTRAJ_FORCE_WSPLINE_HEX, 0, E->p[4] * scale, E->p[5] * scale, g};
+      OTab T;
+      memset(&T, 0, sizeof T);
+      T.ws = ws;
+      T.nseg = nseg;
+      const double h = E->p[2], L = E->p[3];
+      drift_g(&x, &y, vx, &vy, (z0 - zc) / vz, g);
+      double *S = fs + 4 * fslot;
+      S[0] = x; S[1] = y; S[2] = vx; S[3] = vy;
+      double q = floor(L / (vz * h));
+      int nfull;
+      if (!(q >= 0.0)) nfull = 0;
+      else if (q > (double)max_steps) { nfull = max_steps; capped++; }
+      else nfull = (int)q;
+      const double hp = L / vz - (double)nfull * h;
+      double *ST = steps + (size_t)lens_i * per_lens;
+      uint32_t st = 0;
+      for (int s = 0; s <= nfull; ++s) {
+        ST[4 * s] = x; ST[4 * s + 1] = y; ST[4 * s + 2] = vx; ST[4 * s + 3] = vy;
+        rk4_step(&F, &T, 0, s < nfull ? h : hp, &x, &y, &vx, &vy, &st);
+      }
+      zc = z0 + L;
+      ++fslot;
+      ++lens_i;
+      pidx += 2;
+    }
+  }
+  out[np] = Jw;
+  if (!grad) return capped;
+
+  double pbx = 0, pby = 0, vbx = 0, vby = 0, vzbar = 0, zexit_bar = 0;
+  for (int e = n_el - 1; e >= 0; --e) {
+    const ctraj_elem *E = &el[e];
+    pidx -= E->kind == CTRAJ_LENS ? 2 : 1;
+    const int pprev = e > 0 ? pidx - (el[e - 1].kind == CTRAJ_LENS ? 2 : 1) : 0;
+    const double zprev = exit_z_of(el, P, e - 1, pprev, z_src);
+    if (E->kind == CTRAJ_CIRC) {
+      const double r = P[pidx];
+      const int nf = E->z1 > E->z0 ? 2 : 1;
+      double rbar = 0.0, zfrom_bar = 0.0;
+      for (int f = nf - 1; f >= 0; --f) {
+        --fslot;
+        const double zf = f ? E->z1 : E->z0;
+        const double zfrom = f ? E->z0 : zprev;
+        const double *S = fs + 4 * fslot;
+        const double rho = sqrt(S[0] * S[0] + S[1] * S[1]);
+        double phi, om;
+        sig2((r - rho) * inv_s, &phi, &om);
+        const double wbar = Jw * om;
+        rbar = rbar + wbar * inv_s;
+        if (rho > 0.0) {
+          const double rb = -(wbar * inv_s) / rho;
+          pbx = pbx + rb * S[0];
+          pby = pby + rb * S[1];
+        }
+        const double tau = (zf - zfrom) / vz;
+        const double taubar = (pbx * S[2] + pby * S[3]) - g * vby;
+        vbx = vbx + tau * pbx;
+        vby = vby + tau * pby;
+        vzbar = vzbar - (taubar * tau) / vz;
+        if (f == 0) zfrom_bar = -taubar / vz;
+      }
+      out[pidx] += rbar;
+      zexit_bar = zfrom_bar;
+    } else {
+      --lens_i;
+      const double V = P[pidx], z0 = P[pidx + 1];
+      const double scale = V / E->p[7];
+      SLens LK = {ws, umax, E->p[4] * scale, E->p[5] * scale, 0, 1.0 / V, g};
+      LK.kc2 = (2.0 * LK.ks) * LK.cu;
+      const double h = E->p[2], L = E->p[3];
+      double Vbar = 0.0, z0bar = zexit_bar;
+      --fslot;
+      double q = floor(L / (vz * h));
+      int nfull;
+      if (!(q >= 0.0)) nfull = 0;
+      else if (q > (double)max_steps) nfull = max_steps;
+      else nfull = (int)q;
+      const double hp = L / vz - (double)nfull * h;
+      const double *ST = steps + (size_t)lens_i * per_lens;
+      for (int s = nfull; s >= 0; --s) {
+        const double *I = ST + 4 * s;
+        if (s == nfull) {
+          const double hbar = rk4_adj(&LK, hp, I[0], I[1], I[2], I[3], &pbx, &pby, &vbx, &vby, &Vbar, 1);
+          vzbar = vzbar - hbar * ((L / vz) / vz);
+        } else {
+          (void)rk4_adj(&LK, h, I[0], I[1], I[2], I[3], &pbx, &pby, &vbx, &vby, &Vbar, 0);
+        }
+      }
+      const double *S = fs + 4 * fslot;
+      const double tau = (z0 - zprev) / vz;
+      const double taubar = (pbx * S[2] + pby * S[3]) - g * vby;
+      vbx = vbx + tau * pbx;
+      vby = vby + tau * pby;
+      vzbar = vzbar - (taubar * tau) / vz;
+      z0bar = z0bar + taubar / vz;
+      out[pidx] += Vbar;
+      out[pidx + 1] += z0bar;
+      zexit_bar = -taubar / vz;
+    }
+  }
+  out[np - 1] += vzbar;
+  return capped;
+}
+
+int orc_smooth(const ctraj_elem *el, int n_el, double g, const double *ws, int nseg, int64_t n, const double *x,
+               const double *y, const double *vx, const double *vy, const double *eps, double z_src, double sigma_vz,
+               double softness, int n_cfg, int np, const double *params, int max_steps, double *J, double *grad,
+               double *per_mol, int64_t *n_capped, int n_threads) {
+  int n_lens = 0;
+  for (int e = 0; e < n_el; ++e) {
+    if (el[e].kind == CTRAJ_LENS) n_lens++;
+    else if (el[e].kind != CTRAJ_CIRC) return -1;
+  }
+  const double inv_s = 1.0 / softness;
+  int64_t capped_total = 0;
+  for (int c = 0; c < n_cfg; ++c) {
+    const double *P = params + (size_t)c * np;
+    double *pm = per_mol ? per_mol + (size_t)c * n * (np + 1) : (double *)malloc((size_t)n * (np + 1) * sizeof(double));
+    int64_t cap_c = 0;
+#pragma omp parallel num_threads(n_threads > 0 ? n_threads : 1) reduction(+ : cap_c)
+    {
+      double *fs = (double *)malloc((size_t)(2 * n_el + 2) * 4 * sizeof(double));
+      double *steps = (double *)malloc((size_t)(n_lens > 0 ? n_lens : 1) * 4 * (max_steps + 1) * sizeof(double));
+#pragma omp for schedule(dynamic, 64)
+      for (int64_t i = 0; i < n; ++i) {
+        const double vz = P[np - 1] + sigma_vz * eps[i];
+        cap_c += smooth_one(el, n_el, g, ws, nseg, P, np, x[i], y[i], vx[i], vy[i], vz, z_src, inv_s, max_steps,
+                            grad != NULL, fs, steps, pm + (size_t)i * (np + 1));
+      }
+      free(fs);
+      free(steps);
+    }
+    capped_total += cap_c;
+    for (int j = 0; j <= np; ++j) {
+      if (j < np && !grad) continue;
+      double s = 0.0;
+      for (int64_t i = 0; i < n; ++i) s += pm[(size_t)i * (np + 1) + j];
+      if (j == np) J[c] = s / (double)n;
+      else grad[(size_t)c * np + j] = s / (double)n;
+    }
+    if (!per_mol) free(pm);
+  }
+  if (n_capped) *n_capped = capped_total;
+  return 0;
+}

@@ -43,6 +43,9 @@ def lib():
         L.orc_philox4x32_10.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.orc_draw_gaussian.restype = None
         L.orc_draw_gaussian.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_int, vp, vp, vp, vp]
+        L.orc_smooth.restype = C.c_int
+        L.orc_smooth.argtypes = [vp, C.c_int, C.c_double, vp, C.c_int, C.c_int64, vp, vp, vp, vp, vp, C.c_double,
+                                 C.c_double, C.c_double, C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, vp, C.c_int]
         _L = L
     return _L
 
@@ -116,3 +119,34 @@ def draw_gaussian(n, mean, sigma, seed, offset=0, raw=False):
     lib().orc_draw_gaussian(C.c_uint64(seed), C.c_uint64(offset), int(n), nd, mean.ctypes.data, sigma.ctypes.data,
                             _ptrs(outs), None if rawbuf is None else rawbuf.ctypes.data)
     return (outs, rawbuf) if raw else outs
+
+
+def smooth(cb, x, y, vx, vy, eps, params, *, sigma_vz: float, softness: float, z_src: float = 0.0,
+           max_steps: int = 4096, grad: bool = True, per_molecule: bool = False, threads: int | None = None):
+    """CPU FP64 smoothed detected flux J[c] and its gradient grad[c][j] (reverse mode) for the
+    parameter sets params [n_cfg][n_params] (see ctraj_smooth_objective).
+
+    Returns dict(J, grad (or None), n_capped[, per_mol [n_cfg][n][n_params + 1]])."""
+    L = lib()
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, vx, vy, eps)]
+    n = arrs[0].size
+    P = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+    n_cfg, n_params = P.shape
+    coef = cb.wspline_coef()
+    ws = np.zeros((1, 4)) if coef is None else np.ascontiguousarray(coef[0])
+    nseg = 0 if coef is None else coef.shape[1]
+    J = np.empty(n_cfg)
+    G = np.empty((n_cfg, n_params)) if grad else None
+    pm = np.empty((n_cfg, n, n_params + 1)) if per_molecule else None
+    capped = np.zeros(1, np.int64)
+    rc = L.orc_smooth(C.addressof(cb.elems), cb.n_elems, float(cb.phys.g), ws.ctypes.data, nseg, n,
+                      *[a.ctypes.data for a in arrs], float(z_src), float(sigma_vz), float(softness), n_cfg,
+                      n_params, P.ctypes.data, int(max_steps), J.ctypes.data,
+                      None if G is None else G.ctypes.data, None if pm is None else pm.ctypes.data,
+                      capped.ctypes.data, int(threads if threads is not None else min(16, os.cpu_count() or 1)))
+    if rc != 0:
+        raise ValueError("oracle smooth objective supports circular apertures and lenses only")
+    res = {"J": J, "grad": G, "n_capped": int(capped[0])}
+    if pm is not None:
+        res["per_mol"] = pm
+    return res

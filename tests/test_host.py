@@ -187,12 +187,34 @@ def test_c_abi_library_exports_every_declared_symbol():
     assert N.lib().ctraj_abi_version() == N.ABI_VERSION
 
 
-def test_ctypes_structs_match_the_header_layout():
-    assert ctypes.sizeof(N.Elem) == 8 * 4 + 8 * 10
-    assert ctypes.sizeof(N.Phys) == 24
-    assert ctypes.sizeof(N.SoA) == 56
-    assert ctypes.sizeof(N.BatchCfg) == 8 + 8 + 8 + 96
-    assert N.Run.flags.offset + 8 == ctypes.sizeof(N.Run)
+_STRUCTS = {"Elem": "ctraj_elem", "Phys": "ctraj_phys", "SoA": "ctraj_soa", "Run": "ctraj_run",
+            "Record": "ctraj_record", "BatchCfg": "ctraj_batch_cfg", "BatchTune": "ctraj_batch_tune",
+            "Smooth": "ctraj_smooth"}
+
+
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    """Every ctypes mirror has the C struct's size and field offsets (gcc compiles a probe of the header)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ctraj.h"', 'int main(void) {']
+    for py, c in _STRUCTS.items():
+        lines.append(f'printf("{py} size %zu\\n", sizeof({c}));')
+        for f, _ in getattr(N, py)._fields_:
+            cf = {"inp": "in"}.get(f, f)  # Python keyword renamed in the mirror
+            lines.append(f'printf("{py} {f} %zu\\n", offsetof({c}, {cf}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, out):
+        py, f, v = line.split()
+        S = getattr(N, py)
+        got = ctypes.sizeof(S) if f == "size" else getattr(S, f).offset
+        assert got == int(v), (py, f, got, v)
 
 
 def test_library_is_newer_than_its_sources():

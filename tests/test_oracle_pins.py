@@ -330,3 +330,74 @@ def test_record_rows_follow_the_reference_row_rule():
     assert cb.labels[r["label"][0]] == "Detected"
     assert n == 1 + 6 + (1 + N + 1) + 2 + 2
     assert np.all(np.diff(r["rows"][0, :n, 2]) >= -1e-12)
+
+
+# ---------------------------------------------------------------------------------------------
+# Smoothed detected flux and its reverse-mode gradient (orc_smooth)
+# ---------------------------------------------------------------------------------------------
+def _smooth_case(n, seed=3, sign=-1.0):
+    bl = configs.config1(stark=configs.stark_table(sign))
+    cb = compile_elements(bl.elements, bl.g)
+    X, V = configs.draw_source(n, seed)
+    eps = (V[2] - configs.VZ_MEAN) / configs.VZ_SIGMA
+    theta = np.array([6e-3, 30e3, 0.8, 3e-3, 4e-3, configs.VZ_MEAN])
+    return bl, cb, X, V, eps, theta
+
+
+@pytest.mark.parametrize("softness", [50e-6, 1e-3])
+def test_smooth_gradient_matches_richardson_central_differences(softness):
+    """Reverse-mode gradient of the oracle's J against Richardson-extrapolated central differences
+    of the oracle's own J, for every parameter (radii, lens V and z0, mu): a sign, a missing factor
+    or a wrong stage weight in any adjoint term fails this.  Compared as theta_j dJ/dtheta_j; the
+    wide sigmoid makes every aperture radius matter."""
+    bl, cb, X, V, eps, theta = _smooth_case(400)
+    kw = dict(sigma_vz=configs.VZ_SIGMA, softness=softness, max_steps=2000)
+    r = O.smooth(cb, X[0], X[1], V[0], V[1], eps, theta, **kw)
+    G = r["grad"][0]
+    fd = []
+    for j in range(theta.size):
+        def d(hrel):
+            h = hrel * abs(theta[j])
+            P = np.repeat(theta[None], 2, axis=0)
+            P[0, j] += h
+            P[1, j] -= h
+            J = O.smooth(cb, X[0], X[1], V[0], V[1], eps, P, grad=False, **kw)["J"]
+            return (J[0] - J[1]) / (P[0, j] - P[1, j])
+        fd.append((4 * d(1e-5) - d(2e-5)) / 3)
+    gs, fs = G * theta, np.array(fd) * theta
+    scale = np.max(np.abs(gs))
+    if softness > 1e-4:
+        assert np.all(np.abs(gs) > 1e-3 * scale)
+    np.testing.assert_allclose(gs, fs, rtol=2e-6, atol=1e-9 * scale)
+
+
+def test_smooth_sharp_limit_counts_detected_molecules():
+    """With a vanishing sigmoid width J is the fraction of molecules inside every aperture: it must
+    equal the detected fraction of the ordinary (hard-edged) walk with the lens bore opened up.
+    (Focusing sign of the Stark shift, so that many molecules reach the detector.)"""
+    bl, cb, X, V, eps, theta = _smooth_case(3000, seed=5, sign=+1.0)
+    r = O.smooth(cb, X[0], X[1], V[0], V[1], eps, theta, sigma_vz=configs.VZ_SIGMA, softness=1e-15,
+                 max_steps=2000, grad=False)
+    cb.elems[1].p[0] = 1e9  # lens entrance radius
+    cb.elems[1].p[1] = 1e18  # bore radius squared
+    vz = configs.VZ_MEAN + configs.VZ_SIGMA * eps
+    p = O.propagate(cb, X[0], X[1], X[2], V[0], V[1], vz)
+    det = int(np.count_nonzero(p["label"] == cb.label_detected))
+    assert det > 30
+    assert r["J"][0] * 3000 == pytest.approx(det, abs=1e-9)
+
+
+def test_smooth_parameter_sets_are_independent_and_mu_shifts_vz():
+    """A batch of parameter sets equals the sets evaluated alone; mu enters only through vz."""
+    bl, cb, X, V, eps, theta = _smooth_case(200, seed=7)
+    kw = dict(sigma_vz=configs.VZ_SIGMA, softness=50e-6, max_steps=2000)
+    P = np.stack([theta, theta * [1, 1.1, 1, 1, 1, 1], theta * [1, 1, 1, 1, 1, 0.9]])
+    rb = O.smooth(cb, X[0], X[1], V[0], V[1], eps, P, **kw)
+    for c in range(3):
+        rs = O.smooth(cb, X[0], X[1], V[0], V[1], eps, P[c], **kw)
+        assert rb["J"][c] == rs["J"][0]
+        np.testing.assert_array_equal(rb["grad"][c], rs["grad"][0])
+    # mu -> mu + d with eps -> eps - d/sigma describes the same molecules
+    d = 2.0
+    r2 = O.smooth(cb, X[0], X[1], V[0], V[1], eps - d / configs.VZ_SIGMA, theta + [0, 0, 0, 0, 0, d], **kw)
+    assert r2["J"][0] == pytest.approx(rb["J"][0], rel=1e-12)
