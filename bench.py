@@ -28,6 +28,9 @@ import numpy as np  # noqa: E402
 
 FLOP_PER_FACE = 29  # drift to a plane (25) + radius test (4); see DESIGN.md
 FLOP_PER_RK4_STEP = 106  # classical RK4 step with 4 spline-force evaluations + bore test
+FLOP_ADJ_STEP = 204  # reverse of one RK4 step: 4 force+derivative evaluations, stage points, adjoint sums
+FLOP_SMOOTH_FACE = 23  # smoothed objective, per aperture face: drift, radius, sigmoid, product
+FLOP_SMOOTH_FACE_REV = 34  # its reverse: radius, sigmoid, factor derivative, drift adjoint
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X FP64 vector
 PEAK_HBM = 8000.0  # GB/s
 
@@ -44,6 +47,11 @@ def parse():
     ap.add_argument("--simple", action="store_true", help="one molecule per thread (no lane refill)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline wall time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint measurement")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the forward+adjoint measurement")
+    ap.add_argument("--adjoint-n", type=int, default=1_000_000)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scan measurement")
+    ap.add_argument("--batch-only", action="store_true", help="only the batched-scan measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (HBM counter calibration)")
     return ap.parse_args()
@@ -89,6 +97,138 @@ def cpu_baseline(bl, X, V, seconds: float):
             "counter": counter}
 
 
+def timed_ms(ctx, fn, reps: int, warmup: int = 2) -> float:
+    """Mean device time of fn() over reps back-to-back launches (HIP events on the context stream)."""
+    for _ in range(warmup):
+        fn()
+    ctx.sync()
+    ctx.event_record(0)
+    for _ in range(reps):
+        fn()
+    ctx.event_record(1)
+    ctx.sync()
+    return ctx.event_elapsed_ms(0, 1) / reps
+
+
+def bench_adjoint(ctx, n: int, reps: int = 5) -> dict:
+    """Forward+adjoint of the smoothed detected flux (config-1 geometry, 10^6 molecules, theta =
+    (r1, V0, z0, r2, r3, mu_vz), sigmoid width 50 um, checkpoints every 64 steps) against the
+    2 n_params objective-only evaluations that central differences need."""
+    from trajectories import configs
+    from trajectories.smooth import SmoothFlux
+
+    bl = configs.config1()
+    src = configs.config3_source(n, 3)
+    prob = SmoothFlux(bl, *src, sigma_vz=configs.VZ_SIGMA, softness=configs.CONFIG3_SOFTNESS,
+                      ckpt_every=configs.CONFIG3_CKPT, ctx=ctx)
+    theta = configs.config3_theta()
+    npar = theta.size
+    ms = prob.steps_needed(theta)
+    P1 = ctx.to_device(theta)
+    J1 = ctx.alloc(1)
+    G1 = ctx.alloc(npar)
+    tp, tm = theta + 1e-6 * np.abs(theta), theta - 1e-6 * np.abs(theta)
+    Pfd = np.repeat(theta[None], 2 * npar, axis=0)
+    for j in range(npar):
+        Pfd[2 * j, j], Pfd[2 * j + 1, j] = tp[j], tm[j]
+    dPfd = ctx.to_device(Pfd.ravel())
+    Jfd = ctx.alloc(2 * npar)
+    Psep = [ctx.to_device(Pfd[k]) for k in range(2 * npar)]
+    Jsep = ctx.alloc(2 * npar)
+
+    t_adj = timed_ms(ctx, lambda: prob.launch(P1.ptr, 1, J1.ptr, G1.ptr, ms), reps)
+    t_fwd = timed_ms(ctx, lambda: prob.launch(P1.ptr, 1, J1.ptr, None, ms), reps)
+    t_fd_b = timed_ms(ctx, lambda: prob.launch(dPfd.ptr, 2 * npar, Jfd.ptr, None, ms), reps)
+
+    def fd_separate():
+        for k in range(2 * npar):
+            prob.launch(Psep[k].ptr, 1, Jsep.ptr + 8 * k, None, ms)
+
+    t_fd_s = timed_ms(ctx, fd_separate, max(1, reps // 2), warmup=1)
+    grad = G1.to_host()
+    Jf = Jfd.to_host()
+    fd = (Jf[0::2] - Jf[1::2]) / (tp - tm)
+    gs, fs = grad * theta, fd * theta
+    big = np.abs(fs) > 1e-8 * np.max(np.abs(fs))
+    fd_err = float(np.max(np.abs(gs[big] - fs[big]) / np.abs(fs[big])))
+
+    # FLOPs of one forward+adjoint launch, from each molecule's step count
+    L, dt = bl.elements[1].L, bl.elements[1].dt
+    vz = theta[-1] + configs.VZ_SIGMA * src[4]
+    nfull = np.clip(np.floor(L / (vz * dt)), 0, ms)
+    S = nfull + 1
+    segs = np.ceil(S / configs.CONFIG3_CKPT)
+    faces = 4 * n  # three aperture faces and the lens entrance
+    f_fwd = FLOP_PER_RK4_STEP * S.sum() + FLOP_SMOOTH_FACE * faces
+    f_adj = f_fwd + FLOP_PER_RK4_STEP * (S - segs).sum() + FLOP_ADJ_STEP * S.sum() + FLOP_SMOOTH_FACE_REV * faces
+    achieved = f_adj / (t_adj * 1e-3) / 1e12
+    return {
+        "workload": "smoothed detected flux over the ES-lens beamline, theta=(r1,V0,z0,r2,r3,mu_vz), sigmoid 50 um, "
+                    "RK4 dt=10 us, checkpoints every 64 steps, source RandomState(3)",
+        "molecules": n, "n_params": npar,
+        "fwd_adjoint_ms": t_adj, "fwd_adjoint_molecules_per_s": n / (t_adj * 1e-3),
+        "fwd_only_ms": t_fwd, "fwd_only_molecules_per_s": n / (t_fwd * 1e-3),
+        "fd_batched_ms": t_fd_b, "fd_separate_ms": t_fd_s, "fd_evaluations": 2 * npar,
+        "speedup_vs_fd_separate": t_fd_s / t_adj, "speedup_vs_fd_batched": t_fd_b / t_adj,
+        "grad_vs_fd_max_rel_err": fd_err, "J": float(J1.to_host()[0]),
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64, "kernel": "k_smooth<true>", "flop_per_launch": float(f_adj),
+                     "rk4_steps_forward": float(S.sum())},
+    }
+
+
+def bench_batch(ctx, reps: int = 5, tune_reps: int = 2) -> dict:
+    """The batched scan: 256 ES-lens configurations (V0, lens z0, detector radius, molecule count
+    drawn from RandomState(2)), molecules drawn on the device, one persistent graph-replayed launch.
+    The autotuner picks (block, chunk, refill); the same configurations run one by one (device draw +
+    ctraj_propagate each) give the unbatched time."""
+    from trajectories import configs
+    from trajectories.batch import BatchScan
+    from trajectories.engine import Engine
+
+    bls, n, seeds = configs.config2()
+    scan = BatchScan(bls, n, seeds, configs.SOURCE_MEAN, configs.SOURCE_SIGMA, ctx=ctx)
+    t0 = time.perf_counter()
+    best = scan.autotune(reps=tune_reps)
+    t_tune = time.perf_counter() - t0
+    scan.run(best)
+    runs = [scan.run(best) for _ in range(reps)]
+    ms = float(np.median([r[2] for r in runs]))
+    tally, work = runs[0][0], runs[0][1]
+    for r in runs[1:]:
+        assert np.array_equal(r[0], tally)
+    frac = scan.detected_fraction(tally)
+    flop = FLOP_PER_FACE * float(work[0]) + FLOP_PER_RK4_STEP * float(work[1])
+    achieved = flop / (ms * 1e-3) / 1e12
+    # unbatched: the same configurations one at a time on the same device
+    engines = [Engine(bl, ctx) for bl in bls]
+    ctx.sync()
+    wall0 = time.perf_counter()
+    ctx.event_record(0)
+    for c, eng in enumerate(engines):
+        xs = eng.draw_gaussian(int(n[c]), configs.SOURCE_MEAN, configs.SOURCE_SIGMA, int(seeds[c]))
+        inp = dict(zip(("x", "y", "z", "vx", "vy", "vz"), xs))
+        tal = ctx.alloc(eng.cb.n_labels, np.int64).fill(0)
+        eng.launch(inp, tally=tal, n=int(n[c]))
+    ctx.event_record(1)
+    ctx.sync()
+    seq_ms = ctx.event_elapsed_ms(0, 1)
+    seq_wall = (time.perf_counter() - wall0) * 1e3
+    top = [(b, c, r, t) for t, b, c, r in scan.tuning[:5]]
+    return {
+        "workload": "256 ES-lens configurations: V0~U[10,40] kV, lens z0~U[0.7,0.9] m, detector r~U[2,5] mm, "
+                    "n~round(10^U[3,6]) from RandomState(2); device Philox source per configuration",
+        "configurations": len(bls), "molecules": int(n.sum()),
+        "ms": ms, "molecules_per_s": float(n.sum()) / (ms * 1e-3),
+        "tune": {"block": best.block, "chunk": best.chunk, "refill": best.refill, "graph": bool(best.use_graph),
+                 "candidates": len(scan.tuning), "tuning_s": t_tune, "top5_block_chunk_refill_ms": top},
+        "unbatched_device_ms": seq_ms, "unbatched_wall_ms": seq_wall, "speedup_vs_unbatched": seq_ms / ms,
+        "detected_fraction": {"min": float(frac.min()), "median": float(np.median(frac)), "max": float(frac.max())},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64, "kernel": "k_batch_lanes", "flop_per_launch": flop},
+    }
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -105,6 +245,12 @@ def main():
     from trajectories.engine import Engine
 
     ctx = get_context(0)
+    if args.adjoint_only:
+        print(json.dumps(bench_adjoint(ctx, args.adjoint_n, max(1, args.steps // 4))), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps(bench_batch(ctx)), flush=True)
+        return
     if args.config == "lens":
         bl = configs.config1()
         seed = 1
@@ -205,6 +351,10 @@ def main():
         "detected": n_det,
         "host_draw_s": t_draw,
     }
+    if args.config == "lens" and not args.no_adjoint:
+        line["fwd_adjoint"] = bench_adjoint(ctx, args.adjoint_n)
+    if args.config == "lens" and not args.no_batch:
+        line["batch_scan"] = bench_batch(ctx)
     if not args.no_cpu:
         cb = cpu_baseline(bl, X, V, args.cpu_seconds)
         line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}

@@ -1,14 +1,453 @@
-// batch.hip -- persistent batched scan over many beamline configurations (being written).
+// batch.hip -- many beamline configurations in one persistent, graph-replayed launch (gfx950).
+//
+// The molecules of all configurations form one index space [0, total): configuration c owns
+// [begin_c, begin_c + n_c).  Wavefronts claim `chunk` indices at a time from one global queue
+// (an atomic counter reset by a fill kernel inside the captured graph), so long and short
+// configurations are load-balanced by the same queue.  The lane-refill walker of the ordinary
+// propagation is used with per-lane configuration state: a lane finds its configuration by a
+// binary search of the begin offsets (LDS), draws its molecule from that configuration's Philox
+// stream (the ctraj_draw_gaussian stream), walks the configuration's elements before the lens,
+// integrates the lens with the configuration's field constants held in its own registers, walks
+// the elements after the lens and tallies (configuration, label) in a workgroup LDS histogram.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "ctraj_device.hpp"
 #include "ctraj_internal.hpp"
+#include "ctraj_lanes.hpp"
+#include "ctraj_wave.hpp"
+
+using namespace ctraj;
+
+namespace {
+
+constexpr int kMaxBins = 4096;  // configurations x labels tallied in LDS
+
+struct BCfg {
+  int64_t begin, n;
+  uint64_t seed;
+  double mean[6], sigma[6];
+  int32_t e0, ne, lens, pad;  // element slice; lens index within the slice (-1: none)
+};
+
+struct BArgs {
+  const BCfg *cfgs;
+  const ctraj_elem *elems;
+  unsigned long long *counter;
+  int64_t *tally;               // [n_cfg][n_labels]
+  unsigned long long *work;     // [CTRAJ_WORK_N]
+  int64_t total;
+  double g;
+  int n_cfg, n_labels, label_detected, cap, chunk, refill_min;
+  int off_begin, off_hist;      // in doubles from the LDS base
+};
+
+struct BLane {
+  int c, e0, ne, lens, label_inside;
+  double ks, cu, R2, z1;
+};
+
+__device__ __forceinline__ int find_cfg(const int64_t *s_begin, int n_cfg, int64_t gi) {
+  int lo = 0, hi = n_cfg;  // largest c with begin[c] <= gi
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (s_begin[mid] <= gi) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
+  extern __shared__ double lds[];
+  __shared__ unsigned long long s_work[4];
+  const Tables T = stage_tables(lds, KT);
+  int64_t *s_begin = reinterpret_cast<int64_t *>(lds + A.off_begin);
+  uint32_t *s_hist = reinterpret_cast<uint32_t *>(lds + A.off_hist);
+  const int nbins = A.n_cfg * A.n_labels;
+  for (int i = threadIdx.x; i < A.n_cfg; i += blockDim.x) s_begin[i] = A.cfgs[i].begin;
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0;
+  if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
+  __syncthreads();
+
+  const lds_double *wsb = ws_of(T, 0);
+  const int lane = lane_id();
+  const double g = A.g;
+  WalkCounters wc{0u, 0u};
+  uint32_t n_oot = 0, n_cap = 0;
+  unsigned long long steps_w = 0;
+  int64_t next = 0, end = 0;
+  bool exhausted = false;
+  LaneMol s;
+  s.phase = PH_NEED;
+  s.fin = 0;
+  s.label = 0;
+  s.status = 0;
+  s.wsb = wsb;
+  BLane b{0, 0, 0, -1, 0, 0.0, 0.0, 0.0, 0.0};
+
+  for (;;) {
+    const uint64_t lensm = __ballot(s.phase == PH_LENS);
+    const uint64_t idlem = __ballot(s.phase == PH_NEED || s.phase == PH_POST);
+    if (lensm == 0 || __popcll(idlem) >= A.refill_min) {
+      for (;;) {
+        if (s.phase == PH_POST) {
+          s.m.ax = 0.0;  // the lens exit row stores (0, -g, 0)
+          s.m.ay = -g;
+          s.m.az = 0.0;
+          int lab = A.label_detected;
+          walk_plain(opaque(A.elems) + b.e0, b.lens + 1, b.ne, g, s.m, lab, wc);
+          s.label = lab;
+          s.fin = 1;
+          s.phase = PH_NEED;
+        }
+        if (__ballot(s.fin)) {
+          if (s.fin) {
+            n_oot += (s.status & CTRAJ_ST_OOT) ? 1u : 0u;
+            n_cap += (s.status & CTRAJ_ST_CAPPED) ? 1u : 0u;
+          }
+          // every lane takes part; lanes without a finished molecule add to a key no one else uses
+          const int key = s.fin ? b.c * A.n_labels + s.label : nbins + lane;
+          uint64_t todo = __ballot(s.fin != 0);
+          while (todo) {
+            const int leader = __ffsll((unsigned long long)todo) - 1;
+            const int lk = __shfl(key, leader);
+            const uint64_t same = __ballot(key == lk) & todo;
+            if (lane == leader) atomicAdd(&s_hist[lk], (uint32_t)__popcll(same));
+            todo &= ~same;
+          }
+          s.fin = 0;
+        }
+        const uint64_t need = __ballot(s.phase == PH_NEED);
+        if (!need || exhausted) {
+          if (s.phase == PH_NEED && exhausted) s.phase = PH_DONE;
+          break;
+        }
+        if (next >= end) {
+          unsigned long long c = 0;
+          if (lane == 0) c = atomicAdd(A.counter, (unsigned long long)A.chunk);
+          c = __shfl(c, 0);
+          next = (int64_t)c;
+          end = next + A.chunk < A.total ? next + A.chunk : A.total;
+          if (next >= end) {
+            exhausted = true;
+            continue;
+          }
+        }
+        const int64_t avail = end - next;
+        if (s.phase == PH_NEED) {
+          const int rank = lanes_below(need);
+          if (rank < avail) {
+            const int64_t gi = next + rank;
+            const int c = find_cfg(s_begin, A.n_cfg, gi);
+            const BCfg &C = opaque(A.cfgs)[c];
+            const uint64_t j = (uint64_t)(gi - C.begin);
+            double z[6];
+            gaussian_pair(C.seed, j, 0u, z[0], z[1]);
+            gaussian_pair(C.seed, j, 1u, z[2], z[3]);
+            gaussian_pair(C.seed, j, 2u, z[4], z[5]);
+            MState &m = s.m;
+            m.x = C.mean[0] + C.sigma[0] * z[0];
+            m.y = C.mean[1] + C.sigma[1] * z[1];
+            m.z = C.mean[2] + C.sigma[2] * z[2];
+            m.vx = C.mean[3] + C.sigma[3] * z[3];
+            m.vy = C.mean[4] + C.sigma[4] * z[4];
+            m.vz = C.mean[5] + C.sigma[5] * z[5];
+            m.t = 0.0;
+            m.ax = 0.0;
+            m.ay = -g;
+            m.az = 0.0;
+            b.c = c;
+            b.e0 = C.e0;
+            b.ne = C.ne;
+            b.lens = C.lens;
+            s.status = 0;
+            const ctraj_elem *el = opaque(A.elems) + b.e0;
+            int lab = A.label_detected;
+            bool ok = walk_plain(el, 0, b.lens >= 0 ? b.lens : b.ne, g, m, lab, wc);
+            if (ok && b.lens >= 0) {
+              const ctraj_elem &L = el[b.lens];
+              drift_dt(m, time_to(m, L.z0), g);
+              ++wc.faces;
+              if (rho_of(m.x, m.y) > L.p[0]) {
+                ok = false;
+                lab = L.label_hit;
+              } else {
+                const double Hs = L.p[2], Ll = L.p[3];
+                b.ks = L.p[4];
+                b.cu = L.p[5];
+                b.R2 = L.p[1];
+                b.z1 = L.z1;
+                b.label_inside = L.label_inside;
+                s.i = 0;
+                s.nfull = rk4_full_steps(Ll, m.vz, Hs, A.cap, s.status);
+                s.dzs = m.vz * Hs;
+                s.hp = Ll / m.vz - (double)s.nfull * Hs;
+                s.h = s.nfull == 0 ? s.hp : Hs;
+                s.r2 = __builtin_fma(m.x, m.x, m.y * m.y);
+                s.phase = PH_LENS;
+              }
+            }
+            if (s.phase != PH_LENS) {
+              s.label = lab;
+              s.fin = 1;
+            }
+          }
+        }
+        next += avail < (int64_t)__popcll(need) ? avail : (int64_t)__popcll(need);
+      }
+      if (!__ballot(s.phase == PH_LENS)) break;
+    }
+
+    steps_w += (unsigned long long)__popcll(__ballot(s.phase == PH_LENS));
+    if (s.phase == PH_LENS) {
+      MState &m = s.m;
+      const Force<CTRAJ_FORCE_WSPLINE_HEX> F{b.ks, b.cu, g};
+      const bool last = (s.i == s.nfull);
+      rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, s.h, m.x, m.y, m.vx, m.vy, s.r2, s.status);
+      m.z = last ? b.z1 : m.z + s.dzs;
+      m.t = m.t + s.h;
+      s.r2 = __builtin_fma(m.x, m.x, m.y * m.y);
+      ++s.i;
+      s.h = (s.i == s.nfull) ? s.hp : s.h;
+      const bool dead = s.r2 > b.R2;
+      s.label = dead ? b.label_inside : s.label;
+      s.fin = dead ? 2 : 0;
+      s.phase = dead ? (int)PH_NEED : (last ? (int)PH_POST : (int)PH_LENS);
+    }
+  }
+
+  if (wc.faces) atomicAdd(&s_work[0], (unsigned long long)wc.faces);
+  if (lane == 0 && steps_w) atomicAdd(&s_work[1], steps_w);
+  if (n_oot) atomicAdd(&s_work[2], (unsigned long long)n_oot);
+  if (n_cap) atomicAdd(&s_work[3], (unsigned long long)n_cap);
+  __syncthreads();
+  for (int k = threadIdx.x; k < nbins; k += blockDim.x)
+    if (s_hist[k]) atomicAdd((unsigned long long *)&A.tally[k], (unsigned long long)s_hist[k]);
+  if (threadIdx.x < 4 && s_work[threadIdx.x]) atomicAdd(&A.work[threadIdx.x], s_work[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(256) k_zero_u64(unsigned long long *p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+}  // namespace
+
+struct ctraj_batch {
+  ctraj_ctx *ctx = nullptr;
+  int n_cfg = 0, n_labels = 0, label_detected = 0, cap = 0;
+  int64_t total = 0;
+  double g = 0.0;
+  BCfg *d_cfgs = nullptr;
+  ctraj_elem *d_elems = nullptr;
+  unsigned long long *d_counter = nullptr;  // [1] queue, then [CTRAJ_WORK_N] work counters
+  int64_t *d_tally = nullptr;
+  double *d_ws = nullptr;  // the batch's own copy of the W' spline (state 0), [nseg][4]
+  int ws_nseg = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // captured launches by (block, chunk, grid, refill)
+  std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
+};
+
+static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refill) {
+  ctraj_ctx *c = b->ctx;
+  KTab kt{};
+  kt.ws = b->d_ws;
+  kt.ws_nseg = b->ws_nseg;
+  kt.ws_states = b->d_ws ? 1 : 0;
+  kt.umax = b->ws_nseg > 0 ? std::nextafter((double)b->ws_nseg, 0.0) : 0.0;
+  kt.rt = nullptr;
+  kt.rt_total = 0;
+  const size_t tab = ktab_lds_bytes(kt) / sizeof(double);
+  BArgs A{};
+  A.cfgs = b->d_cfgs;
+  A.elems = b->d_elems;
+  A.counter = b->d_counter;
+  A.work = b->d_counter + 1;
+  A.tally = b->d_tally;
+  A.total = b->total;
+  A.g = b->g;
+  A.n_cfg = b->n_cfg;
+  A.n_labels = b->n_labels;
+  A.label_detected = b->label_detected;
+  A.cap = b->cap;
+  A.chunk = chunk;
+  A.refill_min = refill;
+  A.off_begin = (int)tab;
+  A.off_hist = (int)(tab + b->n_cfg);
+  const size_t lds = (tab + b->n_cfg) * sizeof(double) + (size_t)b->n_cfg * b->n_labels * sizeof(uint32_t);
+  int rc = lds_limit(c, (const void *)k_batch_lanes, lds);
+  if (rc != CTRAJ_OK) return rc;
+  if (grid <= 0) grid = grid_for(c, (const void *)k_batch_lanes, block, lds, (int64_t)1 << 40, 0);
+  const int64_t nz = 1 + CTRAJ_WORK_N;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, b->d_counter, nz);
+  const int64_t nt = (int64_t)b->n_cfg * b->n_labels;
+  hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<int64_t>((nt + 255) / 256, 1024)), dim3(256), 0, c->stream,
+                     (unsigned long long *)b->d_tally, nt);
+  hipLaunchKernelGGL(k_batch_lanes, dim3(grid), dim3(block), lds, c->stream, A, kt);
+  HIPCHK(c, hipGetLastError());
+  return CTRAJ_OK;
+}
 
 extern "C" {
-int32_t ctraj_batch_create(ctraj_ctx *c, const ctraj_elem *, int32_t, const ctraj_batch_cfg *, int32_t,
-                           const ctraj_phys *, ctraj_batch **out) {
+
+int32_t ctraj_batch_create(ctraj_ctx *c, const ctraj_elem *elems, int32_t n_elems_total, const ctraj_batch_cfg *cfgs,
+                           int32_t n_cfgs, const ctraj_phys *phys, ctraj_batch **out) {
   if (out) *out = nullptr;
-  return ctraj::fail(c, CTRAJ_EINVAL, "batched scan not available in this build");
+  if (!c || !elems || !cfgs || !phys || !out) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (n_cfgs < 1 || n_elems_total < 1) return fail(c, CTRAJ_EINVAL, "need at least one configuration and element");
+  int rc = check_phys(c, phys);
+  if (rc != CTRAJ_OK) return rc;
+  if ((int64_t)n_cfgs * phys->n_labels > kMaxBins)
+    return fail(c, CTRAJ_EINVAL, "configurations x labels exceeds " + std::to_string(kMaxBins) + "; split the batch");
+  std::vector<BCfg> hc(n_cfgs);
+  int64_t total = 0;
+  bool any_lens = false;
+  for (int k = 0; k < n_cfgs; ++k) {
+    const ctraj_batch_cfg &C = cfgs[k];
+    if (C.elem_begin < 0 || C.n_elems < 0 || C.elem_begin + C.n_elems > n_elems_total)
+      return fail(c, CTRAJ_EINVAL, "configuration " + std::to_string(k) + ": element slice out of range");
+    if (C.n < 0) return fail(c, CTRAJ_EINVAL, "configuration " + std::to_string(k) + ": n < 0");
+    int force, integ;
+    if ((rc = check_elems(c, elems + C.elem_begin, C.n_elems, phys, &force, &integ)) != CTRAJ_OK) return rc;
+    int lens = -1;
+    for (int e = 0; e < C.n_elems; ++e) {
+      const ctraj_elem &E = elems[C.elem_begin + e];
+      if (E.kind != CTRAJ_LENS) continue;
+      if (lens >= 0) return fail(c, CTRAJ_EINVAL, "configuration " + std::to_string(k) + ": more than one lens");
+      if (E.force != CTRAJ_FORCE_WSPLINE_HEX || E.integrator != CTRAJ_INTEG_RK4)
+        return fail(c, CTRAJ_EINVAL, "batched lenses use the hexapole W' spline and classical RK4");
+      lens = e;
+    }
+    any_lens |= lens >= 0;
+    BCfg &B = hc[k];
+    B.begin = total;
+    B.n = C.n;
+    B.seed = C.seed;
+    for (int d = 0; d < 6; ++d) {
+      B.mean[d] = C.mean[d];
+      B.sigma[d] = C.sigma[d];
+    }
+    B.e0 = C.elem_begin;
+    B.ne = C.n_elems;
+    B.lens = lens;
+    B.pad = 0;
+    total += C.n;
+  }
+  if (any_lens && !c->d_ws) return fail(c, CTRAJ_ESTATE, "lens W spline not set");
+  HIPCHK(c, hipSetDevice(c->device));
+  ctraj_batch *b = new ctraj_batch();
+  b->ctx = c;
+  b->n_cfg = n_cfgs;
+  b->n_labels = phys->n_labels;
+  b->label_detected = phys->label_detected;
+  b->cap = phys->max_lens_steps > 0 ? phys->max_lens_steps : (1 << 20);
+  b->g = phys->g;
+  b->total = total;
+  const size_t nt = (size_t)n_cfgs * phys->n_labels;
+  if (hipMalloc((void **)&b->d_cfgs, sizeof(BCfg) * n_cfgs) != hipSuccess ||
+      hipMalloc((void **)&b->d_elems, sizeof(ctraj_elem) * n_elems_total) != hipSuccess ||
+      hipMalloc((void **)&b->d_counter, sizeof(unsigned long long) * (1 + CTRAJ_WORK_N)) != hipSuccess ||
+      hipMalloc((void **)&b->d_tally, sizeof(int64_t) * nt) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+    (void)hipGetLastError();
+    ctraj_batch_destroy(b);
+    return fail(c, CTRAJ_ENOMEM, "batch allocation failed");
+  }
+  if (any_lens) {
+    b->ws_nseg = c->ws_nseg;
+    if (hipMalloc((void **)&b->d_ws, sizeof(double) * 4 * b->ws_nseg) != hipSuccess) {
+      (void)hipGetLastError();
+      ctraj_batch_destroy(b);
+      return fail(c, CTRAJ_ENOMEM, "batch allocation failed");
+    }
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess ||
+      (b->d_ws && hipMemcpy(b->d_ws, c->d_ws, sizeof(double) * 4 * b->ws_nseg, hipMemcpyDeviceToDevice) != hipSuccess) ||
+      hipMemcpy(b->d_cfgs, hc.data(), sizeof(BCfg) * n_cfgs, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_elems, elems, sizeof(ctraj_elem) * n_elems_total, hipMemcpyHostToDevice) != hipSuccess) {
+    const hipError_t e = hipGetLastError();
+    ctraj_batch_destroy(b);
+    return hipfail(c, e, "batch upload");
+  }
+  *out = b;
+  return CTRAJ_OK;
 }
-int32_t ctraj_batch_run(ctraj_batch *, const ctraj_batch_tune *, int64_t *, uint64_t *, float *) {
-  return ctraj::fail(nullptr, CTRAJ_EINVAL, "batched scan not available in this build");
+
+int32_t ctraj_batch_run(ctraj_batch *b, const ctraj_batch_tune *tune, int64_t *tally_host, uint64_t *work_host,
+                        float *elapsed_ms) {
+  if (!b) return fail(nullptr, CTRAJ_EINVAL, "null batch");
+  ctraj_ctx *c = b->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int block = tune && tune->block ? tune->block : 256;
+  const int chunk = tune && tune->chunk ? tune->chunk : 128;
+  const int grid = tune ? tune->grid : 0;
+  const int refill = tune && tune->refill ? tune->refill : 16;
+  const bool use_graph = tune && tune->use_graph;
+  if (block % 64 || block < 64 || block > 256) return fail(c, CTRAJ_EINVAL, "block must be 64, 128, 192 or 256");
+  if (chunk < 1 || chunk > (1 << 20)) return fail(c, CTRAJ_EINVAL, "chunk must be 1..2^20");
+  if (refill < 1 || refill > 64) return fail(c, CTRAJ_EINVAL, "refill must be 1..64");
+  if (grid < 0) return fail(c, CTRAJ_EINVAL, "grid < 0");
+  HIPCHK(c, hipEventRecord(b->ev0, c->stream));
+  int rc = CTRAJ_OK;
+  if (use_graph) {
+    const auto key = std::make_tuple(block, chunk, grid, refill);
+    auto it = b->graphs.find(key);
+    if (it == b->graphs.end()) {
+      hipGraph_t graph = nullptr;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      rc = batch_launch(b, block, chunk, grid, refill);
+      const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+      if (rc != CTRAJ_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (e != hipSuccess) return hipfail(c, e, "hipStreamEndCapture");
+      hipGraphExec_t exec = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) return hipfail(c, ei, "hipGraphInstantiate");
+      it = b->graphs.emplace(key, exec).first;
+    }
+    HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+  } else {
+    rc = batch_launch(b, block, chunk, grid, refill);
+    if (rc != CTRAJ_OK) return rc;
+  }
+  HIPCHK(c, hipEventRecord(b->ev1, c->stream));
+  if (tally_host)
+    HIPCHK(c, hipMemcpyAsync(tally_host, b->d_tally, sizeof(int64_t) * b->n_cfg * b->n_labels, hipMemcpyDeviceToHost,
+                             c->stream));
+  if (work_host) {
+    uint64_t w[1 + CTRAJ_WORK_N];
+    HIPCHK(c, hipMemcpyAsync(w, b->d_counter, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < CTRAJ_WORK_N; ++k) work_host[k] = w[1 + k];
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (elapsed_ms) HIPCHK(c, hipEventElapsedTime(elapsed_ms, b->ev0, b->ev1));
+  return CTRAJ_OK;
 }
-int32_t ctraj_batch_destroy(ctraj_batch *) { return CTRAJ_OK; }
+
+int32_t ctraj_batch_destroy(ctraj_batch *b) {
+  if (!b) return CTRAJ_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (b->d_cfgs) (void)hipFree(b->d_cfgs);
+  if (b->d_elems) (void)hipFree(b->d_elems);
+  if (b->d_counter) (void)hipFree(b->d_counter);
+  if (b->d_tally) (void)hipFree(b->d_tally);
+  if (b->d_ws) (void)hipFree(b->d_ws);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  delete b;
+  return CTRAJ_OK;
 }
+
+}  // extern "C"

@@ -249,15 +249,6 @@ __device__ __forceinline__ void record_finished(const KBeam &B, const ctraj_run 
   }
 }
 
-// A copy of `p` the compiler cannot see through: loads made through it stay where they are
-// written instead of being hoisted out of the walker loop (which would keep dozens of output
-// pointers live in SGPRs and spill them).
-template <class T>
-__device__ __forceinline__ const T *opaque(const T *p) {
-  uint64_t v = (uint64_t)p;
-  asm volatile("" : "+s"(v));
-  return (const T *)v;
-}
 
 template <int FORCE, int INTEG>
 __global__ void __launch_bounds__(256) k_propagate_lanes(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg,

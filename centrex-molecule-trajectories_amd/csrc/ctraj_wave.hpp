@@ -68,4 +68,14 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// A copy of `p` the compiler cannot see through: loads made through it stay where they are
+// written instead of being hoisted out of a walker loop (which would keep dozens of pointers live
+// in SGPRs and spill them).
+template <class T>
+__device__ __forceinline__ const T *opaque(const T *p) {
+  uint64_t v = (uint64_t)p;
+  asm volatile("" : "+s"(v));
+  return (const T *)v;
+}
+
 }  // namespace ctraj

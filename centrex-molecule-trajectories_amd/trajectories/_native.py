@@ -129,7 +129,9 @@ class BatchTune(C.Structure):
         ("block", C.c_int32),
         ("chunk", C.c_int32),
         ("grid", C.c_int32),
+        ("refill", C.c_int32),
         ("use_graph", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

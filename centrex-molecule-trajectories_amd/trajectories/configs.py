@@ -101,6 +101,20 @@ def config4() -> Beamline:
     return config1(stark=config4_tables())
 
 
+SOURCE_MEAN = (0.0, 0.0, 0.0, 0.0, 0.0, VZ_MEAN)
+SOURCE_SIGMA = (SIGMA_X, SIGMA_X, 0.0, SIGMA_V, SIGMA_V, VZ_SIGMA)
+
+
+def config2(n_cfg: int = 256, seed: int = 2):
+    """The batched scan: (beamlines, counts, device source seeds).  Configuration c is the ES-lens
+    beamline with its own V0, lens z0 and final (detector) radius; its molecules come from the device
+    source with seed (seed << 32) + c."""
+    V0, z0, r, n = config2_params(n_cfg, seed)
+    bls = [config1(V0=float(V0[c]), lens_z0=float(z0[c]), r3=float(r[c])) for c in range(n_cfg)]
+    seeds = (np.uint64(seed) << np.uint64(32)) + np.arange(n_cfg, dtype=np.uint64)
+    return bls, n, seeds
+
+
 def config2_params(n_cfg: int = 256, seed: int = 2):
     """Per-configuration (V0, lens z0, final aperture radius, molecule count), drawn from
     RandomState(seed) in that order: V0 ~ U[10, 40] kV, z0 ~ U[0.7, 0.9] m, r ~ U[2, 5] mm,

@@ -191,7 +191,14 @@ int32_t ctraj_record_trajectories(ctraj_ctx *ctx, const ctraj_record *rec);
 int32_t ctraj_draw_gaussian(ctraj_ctx *ctx, uint64_t seed, uint64_t offset, int64_t n, int32_t n_dims,
                             const double *mean, const double *sigma, double *const *out);
 
-/* ---- batched scan: many beamline configurations in one persistent, graph-replayed launch ---- */
+/* ---- batched scan: many beamline configurations in one persistent, graph-replayed launch ----
+ * Replaces running TrajectorySimulator.run_simulation (trajectory_simulator.py:34-103) once per
+ * configuration.  Every configuration is a z0-sorted slice of one element array (at most one lens,
+ * hexapole W' spline + classical RK4, lens state 0) and draws its own molecules on the device:
+ * molecule j of configuration c is ctraj_draw_gaussian(seed_c, offset 0)'s molecule j over the six
+ * dimensions (x, y, z, vx, vy, vz) with t = 0 and a = (0, -g, 0), so a batch equals per-configuration
+ * ctraj_propagate runs over those draws, molecule for molecule.  Lanes of one wavefront may hold
+ * molecules of different configurations; each molecule runs exactly the arithmetic of ctraj_propagate. */
 typedef struct ctraj_batch_cfg {
   int32_t elem_begin, n_elems; /* slice of the batch element array (z0-sorted) */
   int64_t n;                   /* molecules in this configuration */
@@ -199,16 +206,19 @@ typedef struct ctraj_batch_cfg {
   double mean[6], sigma[6];    /* Gaussian source for x, y, z, vx, vy, vz */
 } ctraj_batch_cfg;
 typedef struct ctraj_batch_tune {
-  int32_t block;     /* threads per block */
-  int32_t chunk;     /* molecules per work item (multiple of block) */
-  int32_t grid;      /* persistent blocks (0 = n_cu * blocks_per_cu) */
-  int32_t use_graph; /* replay a captured hipGraph */
+  int32_t block;     /* threads per block: 64, 128, 192 or 256 (0 = 256) */
+  int32_t chunk;     /* molecules a wavefront claims at a time from the global queue (0 = 128) */
+  int32_t grid;      /* persistent blocks (0 = as many as are resident) */
+  int32_t refill;    /* refill a wavefront when at least this many lanes are idle, 1..64 (0 = 16) */
+  int32_t use_graph; /* 1: capture the launch (queue reset, tally reset, kernel) once as a hipGraph and replay it */
+  int32_t reserved;
 } ctraj_batch_tune;
-/* host arrays; the W spline (ctraj_set_wspline) and r-tables of the context are shared by all configurations */
+/* host arrays; the context's W' spline (ctraj_set_wspline, state 0) at creation is copied into the batch and
+ * shared by all configurations */
 int32_t ctraj_batch_create(ctraj_ctx *ctx, const ctraj_elem *elems, int32_t n_elems_total, const ctraj_batch_cfg *cfgs,
                            int32_t n_cfgs, const ctraj_phys *phys, ctraj_batch **out);
-/* runs every configuration; tally_host [n_cfgs][n_labels] and work_host [CTRAJ_WORK_N] are host outputs (may be NULL);
- * elapsed_ms (host, may be NULL) is the device time of the launch measured with HIP events */
+/* runs every configuration once; tally_host [n_cfgs][n_labels] and work_host [CTRAJ_WORK_N] are host outputs
+ * (may be NULL); elapsed_ms (host, may be NULL) is the device time of the launch measured with HIP events */
 int32_t ctraj_batch_run(ctraj_batch *b, const ctraj_batch_tune *tune, int64_t *tally_host, uint64_t *work_host,
                         float *elapsed_ms);
 int32_t ctraj_batch_destroy(ctraj_batch *b);

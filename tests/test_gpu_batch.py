@@ -1,0 +1,106 @@
+"""GPU: the batched scan (ctraj_batch_*) against per-configuration propagation.
+
+Each configuration's molecules are the device source draws (ctraj_draw_gaussian with the
+configuration's seed); the same draws are propagated configuration by configuration with
+ctraj_propagate and, copied to the host, with the CPU oracle.  Tallies must agree exactly.
+Graph replays are checked against eager launches, with eager work between the replays.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from trajectories import configs
+from trajectories.batch import BatchScan
+from trajectories.beamline import Beamline
+from trajectories.beamline_elements import CircularAperture
+from trajectories.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+
+def small_scan(ctx):
+    V0 = [30e3, 12e3, 38e3, 25e3, 30e3, 20e3]
+    z0 = [0.8, 0.72, 0.88, 0.8, 0.75, 0.8]
+    r = [4e-3, 2.2e-3, 4.9e-3, 3e-3, 4e-3, 3.5e-3]
+    n = [20_000, 1_000, 7_777, 0, 12_345, 64]
+    bls = [configs.config1(V0=V0[c], lens_z0=z0[c], r3=r[c]) for c in range(6)]
+    # a configuration without a lens rides along
+    bls.append(Beamline([CircularAperture("Aperture 1", 0.3, 0.0, d=12e-3),
+                         CircularAperture("Detector", 2.0, 0.0, d=10e-3)], configs.G))
+    n.append(5_000)
+    seeds = np.arange(len(bls), dtype=np.uint64) + np.uint64(12345)
+    return bls, np.array(n), seeds, BatchScan(bls, n, seeds, configs.SOURCE_MEAN, configs.SOURCE_SIGMA, ctx=ctx)
+
+
+def per_config_reference(ctx, scan, bls, n, seeds):
+    """Tallies of ctraj_propagate and of the oracle on each configuration's device draws."""
+    dev, orc = [], []
+    for c, bl in enumerate(bls):
+        eng = Engine(bl, ctx)
+        if n[c] == 0:
+            dev.append({})
+            orc.append({})
+            continue
+        cols = eng.draw_gaussian(int(n[c]), configs.SOURCE_MEAN, configs.SOURCE_SIGMA, int(seeds[c]))
+        x, y, z, vx, vy, vz = [a.to_host() for a in cols]
+        r = eng.propagate(x, y, z, vx, vy, vz, outputs=False)
+        dev.append(dict(zip(eng.labels, r["tally"].sum(axis=0).tolist())))
+        o = O.propagate(eng.cb, x, y, z, vx, vy, vz)
+        lab = np.bincount(o["label"], minlength=eng.cb.n_labels)
+        orc.append(dict(zip(eng.labels, lab.tolist())))
+    return dev, orc
+
+
+def as_dict(scan, tally, c):
+    return {scan.labels[k]: int(tally[c, k]) for k in range(len(scan.labels)) if tally[c, k]}
+
+
+def test_batch_equals_per_configuration_propagation(ctx):
+    bls, n, seeds, scan = small_scan(ctx)
+    tally, work, ms = scan.run(graph=False)
+    assert tally.sum() == n.sum()
+    dev, orc = per_config_reference(ctx, scan, bls, n, seeds)
+    for c in range(len(bls)):
+        got = as_dict(scan, tally, c)
+        want = {k: v for k, v in dev[c].items() if v}
+        assert got == want, (c, got, want)
+        assert got == {k: v for k, v in orc[c].items() if v}, c
+    print("batch ms", ms, "work", work[:4])
+
+
+def test_graph_replays_match_eager_runs(ctx):
+    bls, n, seeds, scan = small_scan(ctx)
+    eager, _, _ = scan.run(graph=False)
+    for _ in range(3):
+        g, _, _ = scan.run(graph=True)
+        np.testing.assert_array_equal(g, eager)
+        e2, _, _ = scan.run(graph=False)  # eager work between the replays
+        np.testing.assert_array_equal(e2, eager)
+
+
+def test_every_tuning_gives_the_same_tallies(ctx):
+    bls, n, seeds, scan = small_scan(ctx)
+    ref, _, _ = scan.run(graph=False)
+    best = scan.autotune(blocks=(64, 256), chunks=(32, 256), refills=(1, 16, 64), reps=1)
+    for _, b, c, r in scan.tuning:
+        t, _, _ = scan.run(scan.tune.__class__(block=b, chunk=c, grid=0, refill=r, use_graph=0))
+        np.testing.assert_array_equal(t, ref)
+    t, _, _ = scan.run(best)
+    np.testing.assert_array_equal(t, ref)
+
+
+def test_config2_scan_runs_and_matches_sampled_configurations(ctx):
+    """The 256-configuration scan; three configurations are re-run one by one."""
+    bls, n, seeds = configs.config2()
+    scan = BatchScan(bls, n, seeds, configs.SOURCE_MEAN, configs.SOURCE_SIGMA, ctx=ctx)
+    tally, work, ms = scan.run(graph=True)
+    assert tally.sum() == n.sum()
+    frac = scan.detected_fraction(tally)
+    print("configs", len(bls), "molecules", int(n.sum()), "ms", ms, "detected fraction range", frac.min(), frac.max())
+    for c in (0, int(np.argmin(n)), int(np.argmax(frac))):
+        eng = Engine(bls[c], ctx)
+        x, y, z, vx, vy, vz = [a.to_host() for a in eng.draw_gaussian(int(n[c]), configs.SOURCE_MEAN,
+                                                                          configs.SOURCE_SIGMA, int(seeds[c]))]
+        r = eng.propagate(x, y, z, vx, vy, vz, outputs=False)
+        want = dict(zip(eng.labels, r["tally"].sum(axis=0).tolist()))
+        assert as_dict(scan, tally, c) == {k: v for k, v in want.items() if v}, c
