@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--adjoint-n", type=int, default=1_000_000)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scan measurement")
     ap.add_argument("--batch-only", action="store_true", help="only the batched-scan measurement")
+    ap.add_argument("--no-stream", action="store_true", help="skip the streamed four-state run")
+    ap.add_argument("--stream-only", action="store_true", help="only the streamed four-state run")
+    ap.add_argument("--stream-n", type=int, default=100_000_000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (HBM counter calibration)")
     return ap.parse_args()
@@ -229,6 +232,34 @@ def bench_batch(ctx, reps: int = 5, tune_reps: int = 2) -> dict:
     }
 
 
+def bench_stream(ctx, n: int) -> dict:
+    """The streamed run: n molecules of the ES-lens beamline in 2^22-molecule chunks drawn on the
+    device, four lens states assigned uniformly, per-state tallies, and the 1-mm z-grid samples of
+    every detected molecule written to one compacted device buffer."""
+    from trajectories import configs
+    from trajectories.stream import StreamedRun
+
+    run = StreamedRun(configs.config4(), n, chunk=1 << 22, seed=4, ctx=ctx)
+    t0 = time.perf_counter()
+    res = run.run(trajectories=True)
+    wall = time.perf_counter() - t0
+    det = res.tally[:, res.labels.index("Detected")]
+    tb = 6 * 8 * res.n_samples * res.n_detected
+    p2 = res.timings["pass2_device_ms"]
+    steps = float(res.work[1])
+    return {
+        "workload": "ES-lens beamline, 4 Stark states (signs -/+, dipole x0.5, x0.8) assigned uniformly, "
+                    "2^22-molecule chunks drawn on the device (seed 4), detected trajectories every 1 mm in z",
+        "molecules": n, "chunks": (n + (1 << 22) - 1) >> 22,
+        "pass1_device_ms": res.timings["pass1_device_ms"], "pass1_wall_s": res.timings["pass1_wall_s"],
+        "pass1_molecules_per_s": n / (res.timings["pass1_device_ms"] * 1e-3),
+        "pass2_device_ms": p2, "detected": res.n_detected, "detected_per_state": det.tolist(),
+        "samples_per_molecule": res.n_samples, "trajectory_bytes": tb,
+        "trajectory_write_GBps": tb / (p2 * 1e-3) / 1e9 if p2 > 0 else None,
+        "total_wall_s": wall, "rk4_steps_pass1": steps,
+    }
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -250,6 +281,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps(bench_batch(ctx)), flush=True)
+        return
+    if args.stream_only:
+        print(json.dumps(bench_stream(ctx, args.stream_n)), flush=True)
         return
     if args.config == "lens":
         bl = configs.config1()
@@ -355,6 +389,8 @@ def main():
         line["fwd_adjoint"] = bench_adjoint(ctx, args.adjoint_n)
     if args.config == "lens" and not args.no_batch:
         line["batch_scan"] = bench_batch(ctx)
+    if args.config == "lens" and not args.no_stream:
+        line["streamed"] = bench_stream(ctx, args.stream_n)
     if not args.no_cpu:
         cb = cpu_baseline(bl, X, V, args.cpu_seconds)
         line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}

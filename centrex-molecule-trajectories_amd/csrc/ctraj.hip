@@ -495,6 +495,39 @@ __global__ void __launch_bounds__(256) k_draw(uint64_t seed, uint64_t offset, in
   }
 }
 
+__global__ void __launch_bounds__(256) k_draw_u8(uint64_t seed, uint64_t offset, int64_t n, uint32_t n_values,
+                                                  uint8_t *out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t k = (uint64_t)i + offset;
+    uint32_t c[4] = {(uint32_t)k, (uint32_t)(k >> 32), 0x53544154u, 0x43545241u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    out[i] = (uint8_t)(((uint64_t)c[0] * n_values) >> 32);
+  }
+}
+
+struct GatherArgs {
+  const double *src[8];
+  double *dst[8];
+};
+
+// Reads are scattered (kept molecules are sparse), writes are dense.
+__global__ void __launch_bounds__(256) k_gather_f64(int64_t n, const int64_t *index, int64_t base, int nf,
+                                                    GatherArgs G) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    const int64_t i = index[j] - base;
+    for (int f = 0; f < nf; ++f)
+      if (G.dst[f]) G.dst[f][j] = G.src[f][i];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_gather_u8(int64_t n, const int64_t *index, int64_t base, const uint8_t *src,
+                                                   uint8_t *dst) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) dst[j] = src[index[j] - base];
+}
+
 __global__ void __launch_bounds__(256) k_fill_i64(int64_t *dst, int64_t v, int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = v;
@@ -1041,6 +1074,66 @@ int32_t ctraj_draw_gaussian(ctraj_ctx *c, uint64_t seed, uint64_t offset, int64_
   HIPCHK(c, hipSetDevice(c->device));
   const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->n_cu * 16);
   hipLaunchKernelGGL(k_draw, dim3(grid), dim3(256), 0, c->stream, seed, offset, n, nd, A);
+  HIPCHK(c, hipGetLastError());
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_draw_uniform_u8(ctraj_ctx *c, uint64_t seed, uint64_t offset, int64_t n, int32_t n_values,
+                              uint8_t *out) {
+  if (!c || !out) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (n_values < 1 || n_values > 256) return fail(c, CTRAJ_EINVAL, "n_values must be 1..256");
+  if (n < 0) return fail(c, CTRAJ_EINVAL, "n < 0");
+  if (n == 0) return CTRAJ_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->n_cu * 16);
+  hipLaunchKernelGGL(k_draw_u8, dim3(grid), dim3(256), 0, c->stream, seed, offset, n, (uint32_t)n_values, out);
+  HIPCHK(c, hipGetLastError());
+  return CTRAJ_OK;
+}
+
+// The index range is checked on the host before launch (one copy of the index array), so a bad
+// index is an EINVAL, never an out-of-bounds read.
+static int check_gather_index(ctraj_ctx *c, int64_t n, const int64_t *index, int64_t base, int64_t src_len) {
+  std::vector<int64_t> h((size_t)n);
+  HIPCHK(c, hipMemcpyAsync(h.data(), index, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int64_t j = 0; j < n; ++j)
+    if (h[j] - base < 0 || h[j] - base >= src_len)
+      return fail(c, CTRAJ_EINVAL, "gather index " + std::to_string(h[j]) + " outside [index_base, index_base + src_len)");
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_gather_f64(ctraj_ctx *c, int64_t n, const int64_t *index, int64_t index_base, int64_t src_len,
+                         int32_t n_fields, const double *const *src, double *const *dst) {
+  if (!c || !src || !dst || (n > 0 && !index)) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (n_fields < 1 || n_fields > 8) return fail(c, CTRAJ_EINVAL, "n_fields must be 1..8");
+  if (n < 0 || src_len < 0) return fail(c, CTRAJ_EINVAL, "negative size");
+  if (n == 0) return CTRAJ_OK;
+  GatherArgs G{};
+  for (int f = 0; f < n_fields; ++f) {
+    if (dst[f] && !src[f]) return fail(c, CTRAJ_EINVAL, "gather field with a destination but no source");
+    G.src[f] = src[f];
+    G.dst[f] = dst[f];
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = check_gather_index(c, n, index, index_base, src_len);
+  if (rc != CTRAJ_OK) return rc;
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->n_cu * 16);
+  hipLaunchKernelGGL(k_gather_f64, dim3(grid), dim3(256), 0, c->stream, n, index, index_base, n_fields, G);
+  HIPCHK(c, hipGetLastError());
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_gather_u8(ctraj_ctx *c, int64_t n, const int64_t *index, int64_t index_base, int64_t src_len,
+                        const uint8_t *src, uint8_t *dst) {
+  if (!c || !src || !dst || (n > 0 && !index)) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (n < 0 || src_len < 0) return fail(c, CTRAJ_EINVAL, "negative size");
+  if (n == 0) return CTRAJ_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = check_gather_index(c, n, index, index_base, src_len);
+  if (rc != CTRAJ_OK) return rc;
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->n_cu * 16);
+  hipLaunchKernelGGL(k_gather_u8, dim3(grid), dim3(256), 0, c->stream, n, index, index_base, src, dst);
   HIPCHK(c, hipGetLastError());
   return CTRAJ_OK;
 }

@@ -211,3 +211,29 @@ def draw_gaussian(ctx: Context, n: int, mean, sigma, seed: int, offset: int = 0,
                                         mean.ctypes.data, sigma.ctypes.data, C.cast(ptrs, C.c_void_p)),
             ctx.handle)
     return out
+
+
+def draw_uniform_u8(ctx: Context, n: int, n_values: int, seed: int, offset: int = 0, out=None) -> DeviceArray:
+    """Device uniform ids in [0, n_values) (lens-state assignment), Philox4x32-10."""
+    if out is None:
+        out = ctx.alloc(max(int(n), 1), np.uint8)
+    N.check("ctraj_draw_uniform_u8",
+            N.lib().ctraj_draw_uniform_u8(ctx.handle, C.c_uint64(seed), C.c_uint64(offset), int(n), int(n_values),
+                                          out.ptr), ctx.handle)
+    return out
+
+
+def gather(ctx: Context, n: int, index, index_base: int, src_len: int, src: list, dst: list) -> None:
+    """dst[f][j] = src[f][index[j] - index_base] on the device (float64 fields, or one uint8 field)."""
+    if n == 0:
+        return
+    if src and src[0] is not None and src[0].dtype == np.uint8:
+        N.check("ctraj_gather_u8", N.lib().ctraj_gather_u8(ctx.handle, int(n), N.addr(index), int(index_base),
+                                                           int(src_len), N.addr(src[0]), N.addr(dst[0])), ctx.handle)
+        return
+    nf = len(src)
+    sp = (C.c_void_p * nf)(*[N.addr(a) for a in src])
+    dp = (C.c_void_p * nf)(*[N.addr(a) for a in dst])
+    N.check("ctraj_gather_f64", N.lib().ctraj_gather_f64(ctx.handle, int(n), N.addr(index), int(index_base), int(src_len),
+                                                         nf, C.cast(sp, C.c_void_p), C.cast(dp, C.c_void_p)),
+            ctx.handle)

@@ -1,0 +1,167 @@
+"""Streamed large runs: molecules drawn on the device chunk by chunk, per-state tallies, and the
+z-grid trajectories of every detected molecule in one dense device buffer.
+
+This is ``TrajectorySimulator.run_simulation`` (trajectory_simulator.py:34-103) for runs too large
+to draw on the host, with the reference's saved trajectories (trajectory_simulator.py:75-76) resampled
+every ``z_step`` by the post-processing rule (post_processing.py:20-78).
+
+Pass 1, per chunk of ``chunk`` molecules kept in HBM: draw (x, y, z, vx, vy, vz) with the device
+Gaussian source at offset = the chunk's first molecule and a uniform lens-state id per molecule;
+propagate (per-state, per-element tallies accumulate on the device); compact the detected
+molecules' initial conditions, in molecule order, into a device list.
+Pass 2: one record launch re-walks the detected molecules and writes their samples
+[6][n_samples][n_detected] (x, y, vx, vy, vz, t), molecule-minor so that writes are coalesced.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from .device import Context, DeviceArray, get_context
+from .engine import Engine, _soa, bind, draw_gaussian, draw_uniform_u8, gather
+
+FIELDS6 = ("x", "y", "z", "vx", "vy", "vz")
+
+
+@dataclass
+class StreamResult:
+    n: int
+    labels: list
+    tally: np.ndarray               # [n_states][n_labels]
+    detected_index: np.ndarray      # global molecule index of each detected molecule, ascending
+    detected_state: np.ndarray
+    n_samples: int
+    z_start: float
+    z_step: float
+    trajectories: DeviceArray | None  # [6][n_samples][n_detected] on the device, or None
+    samples_written: np.ndarray | None
+    timings: dict = field(default_factory=dict)
+    work: np.ndarray | None = None
+
+    @property
+    def n_detected(self) -> int:
+        return int(self.detected_index.size)
+
+    def trajectories_of(self, js) -> np.ndarray:
+        """Samples of the detected molecules ``js`` (positions in ``detected_index``) as a host array
+        [len(js)][n_samples][6] (x, y, vx, vy, vz, t), gathered on the device."""
+        if self.trajectories is None:
+            raise ValueError("trajectories were not recorded")
+        ctx = self.trajectories.ctx
+        js = np.atleast_1d(np.asarray(js, dtype=np.int64))
+        nd, ns = self.n_detected, self.n_samples
+        if js.size and (js.min() < 0 or js.max() >= nd):
+            raise IndexError("detected molecule out of range")
+        f = np.arange(6)[None, :, None]
+        k = np.arange(ns)[None, None, :]
+        idx = np.ascontiguousarray(((f * ns + k) * nd + js[:, None, None]).reshape(-1))
+        d_idx = ctx.to_device(idx)
+        out = ctx.alloc(max(idx.size, 1))
+        gather(ctx, idx.size, d_idx, 0, 6 * ns * nd, [self.trajectories], [out])
+        return out.to_host()[: idx.size].reshape(js.size, 6, ns).transpose(0, 2, 1)
+
+
+class StreamedRun:
+    def __init__(self, beamline, n: int, *, chunk: int = 1 << 22, seed: int = 4, n_states: int = 4,
+                 mean=None, sigma=None, z_start: float = 0.0, z_step: float = 1e-3, z_end: float | None = None,
+                 ctx: Context | None = None):
+        from . import configs
+
+        self.ctx = ctx or get_context()
+        self.engine = Engine(beamline, self.ctx, n_states=n_states)
+        self.n = int(n)
+        self.chunk = int(chunk)
+        self.seed = int(seed)
+        self.n_states = int(n_states)
+        self.mean = np.asarray(configs.SOURCE_MEAN if mean is None else mean, dtype=np.float64)
+        self.sigma = np.asarray(configs.SOURCE_SIGMA if sigma is None else sigma, dtype=np.float64)
+        self.z_start = float(z_start)
+        self.z_step = float(z_step)
+        if z_end is None:
+            z_end = max(el.z1 for el in beamline.elements)
+        self.n_samples = int(np.floor((z_end - self.z_start) / self.z_step + 1e-9)) + 1
+
+    def run(self, trajectories: bool = True) -> StreamResult:
+        ctx, eng = self.ctx, self.engine
+        cb = eng.cb
+        nl, ns = cb.n_labels, cb.n_states
+        det = cb.label_detected
+        ch = min(self.chunk, max(self.n, 1))
+        inp = {k: ctx.alloc(ch) for k in FIELDS6}
+        state = ctx.alloc(ch, np.uint8)
+        tally = ctx.alloc(ns * nl, np.int64).fill(0)
+        work = ctx.alloc(N.WORK_N, np.uint64).fill(0)
+        aoi_count = ctx.alloc(1, np.int64)
+        aoi_index = ctx.alloc(ch, np.int64)
+        idx_sorted = ctx.alloc(ch, np.int64)
+        cap = max(1024, int(0.02 * self.n))
+        init = {k: ctx.alloc(cap) for k in FIELDS6}
+        init_state = ctx.alloc(cap, np.uint8)
+        count = 0
+        gidx = []
+        t0 = time.perf_counter()
+        ctx.event_record(0)
+        for start in range(0, self.n, ch):
+            m = min(ch, self.n - start)
+            draw_gaussian(ctx, m, self.mean, self.sigma, self.seed, offset=start, out=[inp[k] for k in FIELDS6])
+            draw_uniform_u8(ctx, m, self.n_states, self.seed, offset=start, out=state)
+            aoi_count.fill(0)
+            eng.launch(inp, state=state, tally=tally, work=work, aoi_mask=1 << det, aoi_cap=m, aoi_count=aoi_count,
+                       aoi_index=aoi_index, n=m)
+            cnt = int(aoi_count.to_host()[0])
+            if cnt == 0:
+                continue
+            loc = np.sort(aoi_index.to_host()[:cnt])
+            gidx.append(loc + start)
+            ctx.memcpy(idx_sorted.ptr, loc.ctypes.data, loc.nbytes, N.H2D)
+            if count + cnt > cap:
+                new_cap = max(2 * cap, count + cnt)
+                init = {k: self._grow(v, count, new_cap) for k, v in init.items()}
+                init_state = self._grow(init_state, count, new_cap)
+                cap = new_cap
+            gather(ctx, cnt, idx_sorted, 0, m, [inp[k] for k in FIELDS6],
+                   [init[k].ptr + 8 * count for k in FIELDS6])
+            gather(ctx, cnt, idx_sorted, 0, m, [state], [init_state.ptr + count])
+            count += cnt
+        ctx.event_record(1)
+        ctx.sync()
+        t_pass1 = ctx.event_elapsed_ms(0, 1)
+        wall1 = time.perf_counter() - t0
+        traj = written = None
+        t_pass2 = 0.0
+        if trajectories and count:
+            traj = ctx.alloc(6 * self.n_samples * count)
+            written = ctx.alloc(count, np.int32)
+            R = N.Record()
+            R.n = count
+            R.index = None
+            R.inp = _soa(init)
+            R.z_const, R.t_const = 0.0, 0.0
+            R.state = init_state.ptr
+            R.mode, R.cap = N.REC_ZGRID, self.n_samples
+            R.z_start, R.z_step = self.z_start, self.z_step
+            R.buf, R.n_rows, R.label = traj.ptr, written.ptr, None
+            bind(ctx, cb)
+            ctx.event_record(2)
+            N.check("ctraj_record_trajectories", N.lib().ctraj_record_trajectories(ctx.handle, C.byref(R)),
+                    ctx.handle)
+            ctx.event_record(3)
+            ctx.sync()
+            t_pass2 = ctx.event_elapsed_ms(2, 3)
+        st = init_state.to_host()[:count] if count else np.zeros(0, np.uint8)
+        res = StreamResult(self.n, list(cb.labels), tally.to_host().reshape(ns, nl),
+                           np.concatenate(gidx) if gidx else np.zeros(0, np.int64), st, self.n_samples,
+                           self.z_start, self.z_step, traj, None if written is None else written.to_host(),
+                           {"pass1_device_ms": t_pass1, "pass1_wall_s": wall1, "pass2_device_ms": t_pass2},
+                           work.to_host())
+        return res
+
+    def _grow(self, arr: DeviceArray, used: int, new_n: int) -> DeviceArray:
+        out = self.ctx.alloc(new_n, arr.dtype)
+        if used:
+            self.ctx.memcpy(out.ptr, arr.ptr, used * arr.dtype.itemsize, N.D2D)
+        return out

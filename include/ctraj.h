@@ -191,6 +191,19 @@ int32_t ctraj_record_trajectories(ctraj_ctx *ctx, const ctraj_record *rec);
 int32_t ctraj_draw_gaussian(ctraj_ctx *ctx, uint64_t seed, uint64_t offset, int64_t n, int32_t n_dims,
                             const double *mean, const double *sigma, double *const *out);
 
+/* out[i] = floor(n_values * u32 / 2^32) for molecule i (+ offset), u32 the first word of Philox4x32-10 with
+ * counter (i_lo, i_hi, 'STAT' = 0x53544154, 'CTRA') and key (seed_lo, seed_hi): a uniform lens-state id per
+ * molecule (exactly uniform when n_values divides 2^32).  1 <= n_values <= 256; out is device [n]. */
+int32_t ctraj_draw_uniform_u8(ctraj_ctx *ctx, uint64_t seed, uint64_t offset, int64_t n, int32_t n_values,
+                              uint8_t *out);
+
+/* ---- gather (device arrays): dst[f][j] = src[f][index[j] - index_base], j < n, f < n_fields (<= 8);
+ * NULL dst[f] skips field f.  Every index[j] - index_base must lie in [0, src_len). ---- */
+int32_t ctraj_gather_f64(ctraj_ctx *ctx, int64_t n, const int64_t *index, int64_t index_base, int64_t src_len,
+                         int32_t n_fields, const double *const *src, double *const *dst);
+int32_t ctraj_gather_u8(ctraj_ctx *ctx, int64_t n, const int64_t *index, int64_t index_base, int64_t src_len,
+                        const uint8_t *src, uint8_t *dst);
+
 /* ---- batched scan: many beamline configurations in one persistent, graph-replayed launch ----
  * Replaces running TrajectorySimulator.run_simulation (trajectory_simulator.py:34-103) once per
  * configuration.  Every configuration is a z0-sorted slice of one element array (at most one lens,

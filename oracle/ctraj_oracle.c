@@ -498,6 +498,16 @@ void orc_draw_gaussian(uint64_t seed, uint64_t offset, int64_t n, int nd, const 
   }
 }
 
+/* ctraj_draw_uniform_u8: state id floor(n_values * c0 / 2^32), counter (i_lo, i_hi, 'STAT', 'CTRA') */
+void orc_draw_uniform_u8(uint64_t seed, uint64_t offset, int64_t n, int n_values, uint8_t *out) {
+  for (int64_t i = 0; i < n; ++i) {
+    const uint64_t k = (uint64_t)i + offset;
+    uint32_t c[4] = {(uint32_t)k, (uint32_t)(k >> 32), 0x53544154u, 0x43545241u};
+    orc_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    out[i] = (uint8_t)(((uint64_t)c[0] * (uint64_t)n_values) >> 32);
+  }
+}
+
 int orc_abi_version(void) { return CTRAJ_ABI_VERSION; }
 
 /*

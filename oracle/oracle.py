@@ -43,6 +43,8 @@ def lib():
         L.orc_philox4x32_10.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.orc_draw_gaussian.restype = None
         L.orc_draw_gaussian.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_int, vp, vp, vp, vp]
+        L.orc_draw_uniform_u8.restype = None
+        L.orc_draw_uniform_u8.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_int, vp]
         L.orc_smooth.restype = C.c_int
         L.orc_smooth.argtypes = [vp, C.c_int, C.c_double, vp, C.c_int, C.c_int64, vp, vp, vp, vp, vp, C.c_double,
                                  C.c_double, C.c_double, C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, vp, C.c_int]
@@ -119,6 +121,12 @@ def draw_gaussian(n, mean, sigma, seed, offset=0, raw=False):
     lib().orc_draw_gaussian(C.c_uint64(seed), C.c_uint64(offset), int(n), nd, mean.ctypes.data, sigma.ctypes.data,
                             _ptrs(outs), None if rawbuf is None else rawbuf.ctypes.data)
     return (outs, rawbuf) if raw else outs
+
+
+def draw_uniform_u8(n, n_values, seed, offset=0):
+    out = np.empty(n, np.uint8)
+    lib().orc_draw_uniform_u8(C.c_uint64(seed), C.c_uint64(offset), int(n), int(n_values), out.ctypes.data)
+    return out
 
 
 def smooth(cb, x, y, vx, vy, eps, params, *, sigma_vz: float, softness: float, z_src: float = 0.0,

@@ -1,0 +1,71 @@
+"""GPU: streamed runs (device source, four lens states, chunked) and their saved z-grid trajectories.
+
+A streamed run over several chunks must give the per-state tallies and the detected molecules of
+one propagation over the same molecules (device draws copied to the host and run through the CPU
+oracle), and the recorded samples must equal the oracle's post-processing resampling of its own
+trajectory rows, bit for bit.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from trajectories import _native as N
+from trajectories import configs
+from trajectories.engine import Engine, draw_gaussian, draw_uniform_u8, gather
+from trajectories.stream import StreamedRun
+
+pytestmark = pytest.mark.gpu
+
+
+def test_state_draw_matches_oracle_and_is_uniform(ctx):
+    n = 200_003
+    d = draw_uniform_u8(ctx, n, 4, seed=99, offset=12345).to_host()
+    o = O.draw_uniform_u8(n, 4, 99, 12345)
+    np.testing.assert_array_equal(d, o)
+    counts = np.bincount(d, minlength=4)
+    assert counts.sum() == n and np.all(np.abs(counts - n / 4) < 5 * np.sqrt(n / 4))
+    d3 = draw_uniform_u8(ctx, 1000, 3, seed=5).to_host()
+    assert d3.max() == 2
+
+
+def test_gather_and_its_index_check(ctx):
+    src = ctx.to_device(np.arange(100, dtype=np.float64) * 0.5)
+    idx = ctx.to_device(np.array([107, 100, 199], np.int64))
+    out = ctx.alloc(3)
+    gather(ctx, 3, idx, 100, 100, [src], [out])
+    np.testing.assert_array_equal(out.to_host(), [3.5, 0.0, 49.5])
+    bad = ctx.to_device(np.array([100, 200], np.int64))
+    with pytest.raises(N.NativeError):
+        gather(ctx, 2, bad, 100, 100, [src], [out])
+
+
+def test_streamed_run_equals_one_propagation_and_the_oracle(ctx):
+    n, chunk, seed = 50_000, 16_384, 41
+    bl = configs.config4()
+    run = StreamedRun(bl, n, chunk=chunk, seed=seed, ctx=ctx)
+    res = run.run(trajectories=True)
+    # the same molecules, drawn whole
+    cols = [a.to_host() for a in draw_gaussian(ctx, n, configs.SOURCE_MEAN, configs.SOURCE_SIGMA, seed)]
+    st = draw_uniform_u8(ctx, n, 4, seed).to_host()
+    eng = Engine(bl, ctx, n_states=4)
+    o = O.propagate(eng.cb, *cols, state=st)
+    ref = np.zeros((4, eng.cb.n_labels), np.int64)
+    np.add.at(ref, (st.astype(np.int64), o["label"]), 1)
+    np.testing.assert_array_equal(res.tally, ref)
+    det = np.nonzero(o["label"] == eng.cb.label_detected)[0]
+    np.testing.assert_array_equal(res.detected_index, det)
+    np.testing.assert_array_equal(res.detected_state, st[det])
+    assert res.n_samples == 2501
+    print("detected", res.n_detected, "per state", res.tally[:, eng.cb.label_detected], res.timings)
+    # samples of some detected molecules against the oracle's resampling of its rows
+    rng = np.random.RandomState(0)
+    js = np.unique(rng.randint(0, res.n_detected, 40))
+    got = res.trajectories_of(js)
+    sel = det[js]
+    ro = O.propagate(eng.cb, *[c[sel] for c in cols], state=st[sel], rows_cap=1400)
+    assert np.all(ro["n_rows"] <= 1400)
+    for q, j in enumerate(js):
+        want = O.zgrid_from_rows(ro["rows"][q], int(ro["n_rows"][q]), 0.0, 1e-3, 2501).T
+        same = (got[q] == want) | (np.isnan(got[q]) & np.isnan(want))
+        assert same.all(), (j, np.argwhere(~same)[:5])
+    assert np.all(res.samples_written[js] > 2400)
