@@ -251,7 +251,7 @@ __device__ __forceinline__ void record_finished(const KBeam &B, const ctraj_run 
 
 
 template <int FORCE, int INTEG>
-__global__ void __launch_bounds__(256) k_propagate_lanes(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg,
+__global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg,
                                                          LaneArgs LA) {
   extern __shared__ double lds[];
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
@@ -972,8 +972,6 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     return fail(c, CTRAJ_EINVAL, "aoi compaction needs aoi_count, aoi_index and aoi_cap");
   if (c->phys.n_labels < 64 && (R->aoi_mask >> c->phys.n_labels))
     return fail(c, CTRAJ_EINVAL, "aoi_mask names labels beyond n_labels");
-  const int block = R->block ? R->block : 256;
-  if (block < 64 || block > 256 || block % 64) return fail(c, CTRAJ_EINVAL, "block must be 64, 128, 192 or 256");
   int rc0 = check_tables(c, c->h_elems.data(), c->n_elems, c->phys.n_states);
   if (rc0) return rc0;
   KTab kt = ctx_ktab(c);
@@ -986,8 +984,16 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
       ++n_lens;
       lens_e = e;
     }
+  const bool lanes = n_lens == 1 && !(R->flags & CTRAJ_RUN_SIMPLE);
+  // The lane walker keeps its tables once per workgroup: with several lens-state tables (more
+  // than ~40 KB) one 768-thread workgroup per CU holds as many wavefronts as its registers allow,
+  // where 256-thread workgroups would be limited by LDS to one per CU.
+  const int block = R->block ? R->block : (lanes && lds > 40960 ? 768 : 256);
+  const int max_block = lanes ? 768 : 256;
+  if (block < 64 || block > max_block || block % 64)
+    return fail(c, CTRAJ_EINVAL, "block must be a multiple of 64 up to " + std::to_string(max_block));
   HIPCHK(c, hipSetDevice(c->device));
-  if (n_lens == 1 && !(R->flags & CTRAJ_RUN_SIMPLE)) {
+  if (lanes) {
     LaneFn k = lane_kernel(c->force, c->integ);
     if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
     int rc = lds_limit(c, (const void *)k, lds);
@@ -1004,7 +1010,7 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     LaneArgs LA;
     LA.counter = (unsigned long long *)c->d_counter;
     LA.lens_e = lens_e;
-    LA.refill_min = R->refill > 0 ? std::min(R->refill, 64) : 16;
+    LA.refill_min = R->refill > 0 ? std::min(R->refill, 64) : 24;
     LA.claim = 128;
     const int grid = grid_for(c, (const void *)k, block, lds, R->n, R->grid);
     hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, (const ctraj_run *)c->d_run, LA);
