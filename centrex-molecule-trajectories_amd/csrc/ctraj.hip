@@ -495,14 +495,38 @@ __global__ void __launch_bounds__(256) k_draw(uint64_t seed, uint64_t offset, in
   }
 }
 
+// index-list forms: slot j draws molecule index[j]
+__global__ void __launch_bounds__(256) k_draw_at(uint64_t seed, const int64_t *index, int64_t n, int nd, DrawArgs A) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    const uint64_t i = (uint64_t)index[j];
+    for (int p = 0; 2 * p < nd; ++p) {
+      double z0, z1;
+      gaussian_pair(seed, i, (uint32_t)p, z0, z1);
+      if (A.out[2 * p]) A.out[2 * p][j] = A.mean[2 * p] + A.sigma[2 * p] * z0;
+      if (2 * p + 1 < nd && A.out[2 * p + 1]) A.out[2 * p + 1][j] = A.mean[2 * p + 1] + A.sigma[2 * p + 1] * z1;
+    }
+  }
+}
+
+__device__ __forceinline__ uint8_t uniform_u8(uint64_t seed, uint64_t k, uint32_t n_values) {
+  uint32_t c[4] = {(uint32_t)k, (uint32_t)(k >> 32), 0x53544154u, 0x43545241u};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  return (uint8_t)(((uint64_t)c[0] * n_values) >> 32);
+}
+
+__global__ void __launch_bounds__(256) k_draw_u8_at(uint64_t seed, const int64_t *index, int64_t n, uint32_t n_values,
+                                                    uint8_t *out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride)
+    out[j] = uniform_u8(seed, (uint64_t)index[j], n_values);
+}
+
 __global__ void __launch_bounds__(256) k_draw_u8(uint64_t seed, uint64_t offset, int64_t n, uint32_t n_values,
                                                   uint8_t *out) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint64_t k = (uint64_t)i + offset;
-    uint32_t c[4] = {(uint32_t)k, (uint32_t)(k >> 32), 0x53544154u, 0x43545241u};
-    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-    out[i] = (uint8_t)(((uint64_t)c[0] * n_values) >> 32);
+    out[i] = uniform_u8(seed, (uint64_t)i + offset, n_values);
   }
 }
 
@@ -790,6 +814,8 @@ int32_t ctraj_ctx_destroy(ctraj_ctx *c) {
   if (c->d_run) (void)hipFree(c->d_run);
   if (c->sm_ws.p) (void)hipFree(c->sm_ws.p);
   if (c->sm_part.p) (void)hipFree(c->sm_part.p);
+  if (c->sort_tmp.p) (void)hipFree(c->sort_tmp.p);
+  if (c->sort_alt.p) (void)hipFree(c->sort_alt.p);
   for (int i = 0; i < 64; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   (void)hipStreamDestroy(c->stream);
@@ -1051,13 +1077,6 @@ int32_t ctraj_record_trajectories(ctraj_ctx *c, const ctraj_record *RC) {
   int rc = lds_limit(c, (const void *)k, lds);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
-  // NaN prefill of the whole buffer (rows past a molecule's end stay NaN, like Trajectory's np.full(nan))
-  const int nf = RC->mode == CTRAJ_REC_ROWS ? 10 : 6;
-  int64_t nan_bits;
-  const double qnan = std::nan("");
-  std::memcpy(&nan_bits, &qnan, 8);
-  rc = ctraj_fill_i64(c, (int64_t *)RC->buf, nan_bits, (int64_t)nf * RC->cap * RC->n);
-  if (rc) return rc;
   const int grid = grid_for(c, (const void *)k, block, lds, RC->n, 0);
   KBeam B = make_kbeam(c->d_elems, c->n_elems, c->phys);
   hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, *RC);
@@ -1080,6 +1099,38 @@ int32_t ctraj_draw_gaussian(ctraj_ctx *c, uint64_t seed, uint64_t offset, int64_
   HIPCHK(c, hipSetDevice(c->device));
   const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->n_cu * 16);
   hipLaunchKernelGGL(k_draw, dim3(grid), dim3(256), 0, c->stream, seed, offset, n, nd, A);
+  HIPCHK(c, hipGetLastError());
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_draw_gaussian_at(ctraj_ctx *c, uint64_t seed, const int64_t *index, int64_t n, int32_t nd,
+                               const double *mean, const double *sigma, double *const *out) {
+  if (!c || !mean || !sigma || !out || (n > 0 && !index)) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (nd < 1 || nd > 8) return fail(c, CTRAJ_EINVAL, "n_dims must be 1..8");
+  if (n < 0) return fail(c, CTRAJ_EINVAL, "n < 0");
+  if (n == 0) return CTRAJ_OK;
+  DrawArgs A{};
+  for (int d = 0; d < nd; ++d) {
+    A.mean[d] = mean[d];
+    A.sigma[d] = sigma[d];
+    A.out[d] = out[d];
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->n_cu * 16);
+  hipLaunchKernelGGL(k_draw_at, dim3(grid), dim3(256), 0, c->stream, seed, index, n, nd, A);
+  HIPCHK(c, hipGetLastError());
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_draw_uniform_u8_at(ctraj_ctx *c, uint64_t seed, const int64_t *index, int64_t n, int32_t n_values,
+                                 uint8_t *out) {
+  if (!c || !out || (n > 0 && !index)) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (n_values < 1 || n_values > 256) return fail(c, CTRAJ_EINVAL, "n_values must be 1..256");
+  if (n < 0) return fail(c, CTRAJ_EINVAL, "n < 0");
+  if (n == 0) return CTRAJ_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->n_cu * 16);
+  hipLaunchKernelGGL(k_draw_u8_at, dim3(grid), dim3(256), 0, c->stream, seed, index, n, (uint32_t)n_values, out);
   HIPCHK(c, hipGetLastError());
   return CTRAJ_OK;
 }

@@ -43,6 +43,8 @@ struct ctraj_ctx {
   hipEvent_t ev[64];
   // smooth objective: per-wave checkpoint/recompute store and per-item partial sums
   ctraj_devbuf sm_ws, sm_part;
+  // radix sort scratch (ctraj_sort_i64)
+  ctraj_devbuf sort_tmp, sort_alt;
 };
 
 namespace ctraj {

@@ -8,7 +8,8 @@ namespace ctraj {
 __device__ __forceinline__ double nan64() { return __builtin_nan(""); }
 
 // Every row of the molecule's Trajectory (molecule.py:111-144): buf[f][row][slot] for the ten
-// fields x, y, z, vx, vy, vz, ax, ay, az, t.  Rows past `cap` are counted but not stored.
+// fields x, y, z, vx, vy, vz, ax, ay, az, t.  Rows past `cap` are counted but not stored; rows after
+// the molecule's last row are NaN.
 struct RowSink {
   static constexpr bool kRows = true;
   double *buf;
@@ -25,7 +26,11 @@ struct RowSink {
     }
     ++count;
   }
-  __device__ __forceinline__ void finish(const MState &) {}  // unwritten rows keep the NaN prefill
+  // rows past the end are NaN, like the reference's np.full(nan) trajectory arrays
+  __device__ __forceinline__ void finish(const MState &) {
+    for (int r = count; r < cap; ++r)
+      for (int f = 0; f < 10; ++f) put(f, r, nan64());
+  }
 };
 
 // Samples at z_k = z_start + k*z_step by the rule of find_radial_pos_dist / find_vel_dist
@@ -35,7 +40,7 @@ struct RowSink {
 // takes index -1, i.e. the last row (:61).  Streaming form: a sample is decided at the first row
 // with z >= z_k; if that row is past z_k the decision waits one more row, because a zero-length
 // face step (an L = 0 aperture) can land the next row back on z_k exactly.
-// buf[f][k][slot]: x, y, vx, vy, vz, t.
+// buf[f][k][slot]: x, y, vx, vy, vz, t; every sample is written (NaN where the rule gives none).
 struct ZGridSink {
   static constexpr bool kRows = true;
   double *buf;
@@ -100,9 +105,15 @@ struct ZGridSink {
       --written;
     }
     for (int i = 0; i < kdefer; ++i) {
-      if (last.z < zk(i)) continue;
+      if (last.z < zk(i)) {
+        for (int f = 0; f < 6; ++f) put(f, i, nan64());
+        continue;
+      }
       extrap(i, last);
     }
+    // samples past the last row's z were never reached
+    for (int i = k; i < cap; ++i)
+      for (int f = 0; f < 6; ++f) put(f, i, nan64());
   }
 };
 

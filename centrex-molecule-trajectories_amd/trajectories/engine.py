@@ -237,3 +237,31 @@ def gather(ctx: Context, n: int, index, index_base: int, src_len: int, src: list
     N.check("ctraj_gather_f64", N.lib().ctraj_gather_f64(ctx.handle, int(n), N.addr(index), int(index_base), int(src_len),
                                                          nf, C.cast(sp, C.c_void_p), C.cast(dp, C.c_void_p)),
             ctx.handle)
+
+
+def draw_gaussian_at(ctx: Context, index, n: int, mean, sigma, seed: int, out=None) -> list:
+    """Device Gaussian source for molecules ``index[0:n]`` (device int64) of stream ``seed``."""
+    mean = np.ascontiguousarray(mean, dtype=np.float64)
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    nd = mean.size
+    if out is None:
+        out = [ctx.alloc(max(int(n), 1)) for _ in range(nd)]
+    ptrs = (C.c_void_p * nd)(*[N.addr(o) for o in out])
+    N.check("ctraj_draw_gaussian_at",
+            N.lib().ctraj_draw_gaussian_at(ctx.handle, C.c_uint64(seed), N.addr(index), int(n), nd, mean.ctypes.data,
+                                           sigma.ctypes.data, C.cast(ptrs, C.c_void_p)), ctx.handle)
+    return out
+
+
+def draw_uniform_u8_at(ctx: Context, index, n: int, n_values: int, seed: int, out=None) -> DeviceArray:
+    if out is None:
+        out = ctx.alloc(max(int(n), 1), np.uint8)
+    N.check("ctraj_draw_uniform_u8_at",
+            N.lib().ctraj_draw_uniform_u8_at(ctx.handle, C.c_uint64(seed), N.addr(index), int(n), int(n_values),
+                                             N.addr(out)), ctx.handle)
+    return out
+
+
+def sort_i64(ctx: Context, keys, n: int) -> None:
+    """Ascending in-place sort of the first n device int64 keys."""
+    N.check("ctraj_sort_i64", N.lib().ctraj_sort_i64(ctx.handle, N.addr(keys), int(n)), ctx.handle)

@@ -7,9 +7,10 @@ every ``z_step`` by the post-processing rule (post_processing.py:20-78).
 
 Pass 1, per chunk of ``chunk`` molecules kept in HBM: draw (x, y, z, vx, vy, vz) with the device
 Gaussian source at offset = the chunk's first molecule and a uniform lens-state id per molecule;
-propagate (per-state, per-element tallies accumulate on the device); compact the detected
-molecules' initial conditions, in molecule order, into a device list.
-Pass 2: one record launch re-walks the detected molecules and writes their samples
+propagate (per-state, per-element tallies and the global indices of detected molecules
+accumulate on the device; the host never waits between chunks).
+Pass 2: sort the detected indices (radix sort), regenerate those molecules' inputs from the same
+counter-based streams, and re-walk them in one record launch that writes their samples
 [6][n_samples][n_detected] (x, y, vx, vy, vz, t), molecule-minor so that writes are coalesced.
 """
 from __future__ import annotations
@@ -22,7 +23,8 @@ import numpy as np
 
 from . import _native as N
 from .device import Context, DeviceArray, get_context
-from .engine import Engine, _soa, bind, draw_gaussian, draw_uniform_u8, gather
+from .engine import (Engine, _soa, bind, draw_gaussian, draw_gaussian_at, draw_uniform_u8, draw_uniform_u8_at,
+                     gather, sort_i64)
 
 FIELDS6 = ("x", "y", "z", "vx", "vy", "vz")
 
@@ -93,72 +95,66 @@ class StreamedRun:
         ch = min(self.chunk, max(self.n, 1))
         inp = {k: ctx.alloc(ch) for k in FIELDS6}
         state = ctx.alloc(ch, np.uint8)
-        tally = ctx.alloc(ns * nl, np.int64).fill(0)
-        work = ctx.alloc(N.WORK_N, np.uint64).fill(0)
+        tally = ctx.alloc(ns * nl, np.int64)
+        work = ctx.alloc(N.WORK_N, np.uint64)
         aoi_count = ctx.alloc(1, np.int64)
-        aoi_index = ctx.alloc(ch, np.int64)
-        idx_sorted = ctx.alloc(ch, np.int64)
-        cap = max(1024, int(0.02 * self.n))
-        init = {k: ctx.alloc(cap) for k in FIELDS6}
-        init_state = ctx.alloc(cap, np.uint8)
-        count = 0
-        gidx = []
-        t0 = time.perf_counter()
-        ctx.event_record(0)
-        for start in range(0, self.n, ch):
-            m = min(ch, self.n - start)
-            draw_gaussian(ctx, m, self.mean, self.sigma, self.seed, offset=start, out=[inp[k] for k in FIELDS6])
-            draw_uniform_u8(ctx, m, self.n_states, self.seed, offset=start, out=state)
-            aoi_count.fill(0)
-            eng.launch(inp, state=state, tally=tally, work=work, aoi_mask=1 << det, aoi_cap=m, aoi_count=aoi_count,
-                       aoi_index=aoi_index, n=m)
-            cnt = int(aoi_count.to_host()[0])
-            if cnt == 0:
-                continue
-            loc = np.sort(aoi_index.to_host()[:cnt])
-            gidx.append(loc + start)
-            ctx.memcpy(idx_sorted.ptr, loc.ctypes.data, loc.nbytes, N.H2D)
-            if count + cnt > cap:
-                new_cap = max(2 * cap, count + cnt)
-                init = {k: self._grow(v, count, new_cap) for k, v in init.items()}
-                init_state = self._grow(init_state, count, new_cap)
-                cap = new_cap
-            gather(ctx, cnt, idx_sorted, 0, m, [inp[k] for k in FIELDS6],
-                   [init[k].ptr + 8 * count for k in FIELDS6])
-            gather(ctx, cnt, idx_sorted, 0, m, [state], [init_state.ptr + count])
-            count += cnt
-        ctx.event_record(1)
-        ctx.sync()
-        t_pass1 = ctx.event_elapsed_ms(0, 1)
-        wall1 = time.perf_counter() - t0
+        cap = max(4096, int(0.05 * self.n))
+        while True:
+            aoi_index = ctx.alloc(cap, np.int64)
+            for a in (tally, work, aoi_count):
+                a.fill(0)
+            t0 = time.perf_counter()
+            ctx.event_record(0)
+            # pass 1: no host synchronisation; kept indices accumulate on the device
+            for start in range(0, self.n, ch):
+                m = min(ch, self.n - start)
+                draw_gaussian(ctx, m, self.mean, self.sigma, self.seed, offset=start, out=[inp[k] for k in FIELDS6])
+                draw_uniform_u8(ctx, m, self.n_states, self.seed, offset=start, out=state)
+                eng.launch(inp, state=state, tally=tally, work=work, aoi_mask=1 << det, aoi_cap=cap,
+                           aoi_count=aoi_count, aoi_index=aoi_index, n=m, index_base=start)
+            ctx.event_record(1)
+            count = int(aoi_count.to_host()[0])
+            wall1 = time.perf_counter() - t0
+            t_pass1 = ctx.event_elapsed_ms(0, 1)
+            if count <= cap:
+                break
+            cap = count  # the kept molecules outgrew the index store: run again with room for all
         traj = written = None
         t_pass2 = 0.0
-        if trajectories and count:
-            traj = ctx.alloc(6 * self.n_samples * count)
-            written = ctx.alloc(count, np.int32)
-            R = N.Record()
-            R.n = count
-            R.index = None
-            R.inp = _soa(init)
-            R.z_const, R.t_const = 0.0, 0.0
-            R.state = init_state.ptr
-            R.mode, R.cap = N.REC_ZGRID, self.n_samples
-            R.z_start, R.z_step = self.z_start, self.z_step
-            R.buf, R.n_rows, R.label = traj.ptr, written.ptr, None
-            bind(ctx, cb)
+        st_h = np.zeros(0, np.uint8)
+        idx_h = np.zeros(0, np.int64)
+        if count:
             ctx.event_record(2)
-            N.check("ctraj_record_trajectories", N.lib().ctraj_record_trajectories(ctx.handle, C.byref(R)),
-                    ctx.handle)
+            sort_i64(ctx, aoi_index, count)
+            init = draw_gaussian_at(ctx, aoi_index, count, self.mean, self.sigma, self.seed)
+            init_state = draw_uniform_u8_at(ctx, aoi_index, count, self.n_states, self.seed)
+            if trajectories:
+                traj = ctx.alloc(6 * self.n_samples * count)
+                written = ctx.alloc(count, np.int32)
+                R = N.Record()
+                R.n = count
+                R.index = None
+                R.inp = _soa(dict(zip(FIELDS6, init)))
+                R.z_const, R.t_const = 0.0, 0.0
+                R.state = init_state.ptr
+                R.mode, R.cap = N.REC_ZGRID, self.n_samples
+                R.z_start, R.z_step = self.z_start, self.z_step
+                R.buf, R.n_rows, R.label = traj.ptr, written.ptr, None
+                bind(ctx, cb)
+                N.check("ctraj_record_trajectories", N.lib().ctraj_record_trajectories(ctx.handle, C.byref(R)),
+                        ctx.handle)
             ctx.event_record(3)
             ctx.sync()
             t_pass2 = ctx.event_elapsed_ms(2, 3)
-        st = init_state.to_host()[:count] if count else np.zeros(0, np.uint8)
-        res = StreamResult(self.n, list(cb.labels), tally.to_host().reshape(ns, nl),
-                           np.concatenate(gidx) if gidx else np.zeros(0, np.int64), st, self.n_samples,
-                           self.z_start, self.z_step, traj, None if written is None else written.to_host(),
-                           {"pass1_device_ms": t_pass1, "pass1_wall_s": wall1, "pass2_device_ms": t_pass2},
-                           work.to_host())
-        return res
+            idx_h = np.empty(count, np.int64)
+            ctx.memcpy(idx_h.ctypes.data, aoi_index.ptr, idx_h.nbytes, N.D2H)
+            st_h = np.empty(count, np.uint8)
+            ctx.memcpy(st_h.ctypes.data, init_state.ptr, st_h.nbytes, N.D2H)
+            ctx.sync()
+        return StreamResult(self.n, list(cb.labels), tally.to_host().reshape(ns, nl), idx_h, st_h, self.n_samples,
+                            self.z_start, self.z_step, traj, None if written is None else written.to_host(),
+                            {"pass1_device_ms": t_pass1, "pass1_wall_s": wall1, "pass2_device_ms": t_pass2},
+                            work.to_host())
 
     def _grow(self, arr: DeviceArray, used: int, new_n: int) -> DeviceArray:
         out = self.ctx.alloc(new_n, arr.dtype)

@@ -198,6 +198,16 @@ int32_t ctraj_draw_gaussian(ctraj_ctx *ctx, uint64_t seed, uint64_t offset, int6
 int32_t ctraj_draw_uniform_u8(ctraj_ctx *ctx, uint64_t seed, uint64_t offset, int64_t n, int32_t n_values,
                               uint8_t *out);
 
+/* index-list forms of the two draws: output slot j holds molecule index[j] (instead of j + offset) of the same
+ * streams, so any subset of a large run can be regenerated on the device (device index[n], out as above) */
+int32_t ctraj_draw_gaussian_at(ctraj_ctx *ctx, uint64_t seed, const int64_t *index, int64_t n, int32_t n_dims,
+                               const double *mean, const double *sigma, double *const *out);
+int32_t ctraj_draw_uniform_u8_at(ctraj_ctx *ctx, uint64_t seed, const int64_t *index, int64_t n, int32_t n_values,
+                                 uint8_t *out);
+
+/* ascending in-place radix sort of device keys [n] */
+int32_t ctraj_sort_i64(ctraj_ctx *ctx, int64_t *keys, int64_t n);
+
 /* ---- gather (device arrays): dst[f][j] = src[f][index[j] - index_base], j < n, f < n_fields (<= 8);
  * NULL dst[f] skips field f.  Every index[j] - index_base must lie in [0, src_len). ---- */
 int32_t ctraj_gather_f64(ctraj_ctx *ctx, int64_t n, const int64_t *index, int64_t index_base, int64_t src_len,

@@ -11,7 +11,8 @@ import pytest
 from oracle import oracle as O
 from trajectories import _native as N
 from trajectories import configs
-from trajectories.engine import Engine, draw_gaussian, draw_uniform_u8, gather
+from trajectories.engine import (Engine, draw_gaussian, draw_gaussian_at, draw_uniform_u8, draw_uniform_u8_at, gather,
+                                 sort_i64)
 from trajectories.stream import StreamedRun
 
 pytestmark = pytest.mark.gpu
@@ -26,6 +27,28 @@ def test_state_draw_matches_oracle_and_is_uniform(ctx):
     assert counts.sum() == n and np.all(np.abs(counts - n / 4) < 5 * np.sqrt(n / 4))
     d3 = draw_uniform_u8(ctx, 1000, 3, seed=5).to_host()
     assert d3.max() == 2
+
+
+def test_index_list_draws_equal_the_contiguous_streams(ctx):
+    n = 5000
+    full = [a.to_host() for a in draw_gaussian(ctx, n, configs.SOURCE_MEAN, configs.SOURCE_SIGMA, 77)]
+    fst = draw_uniform_u8(ctx, n, 4, 77).to_host()
+    idx = np.array([4999, 0, 17, 17, 2500, 1234], np.int64)
+    d_idx = ctx.to_device(idx)
+    part = [a.to_host()[: idx.size] for a in draw_gaussian_at(ctx, d_idx, idx.size, configs.SOURCE_MEAN,
+                                                               configs.SOURCE_SIGMA, 77)]
+    for f in range(6):
+        np.testing.assert_array_equal(part[f], full[f][idx])
+    np.testing.assert_array_equal(draw_uniform_u8_at(ctx, d_idx, idx.size, 4, 77).to_host()[: idx.size], fst[idx])
+
+
+@pytest.mark.parametrize("n", [1, 2, 1000, 300_001])
+def test_radix_sort(ctx, n):
+    rng = np.random.RandomState(n)
+    keys = rng.randint(0, 1 << 40, n).astype(np.int64)
+    d = ctx.to_device(keys)
+    sort_i64(ctx, d, n)
+    np.testing.assert_array_equal(d.to_host()[:n], np.sort(keys))
 
 
 def test_gather_and_its_index_check(ctx):
