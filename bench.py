@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0, help="lane-refill threshold (0 = library default)")
     ap.add_argument("--simple", action="store_true", help="one molecule per thread (no lane refill)")
+    ap.add_argument("--lanes", action="store_true", help="single-kernel lane-refill walker instead of the two phases")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline wall time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint measurement")
@@ -315,7 +316,7 @@ def main():
         eng.launch(inp, tally=tally, work=work, aoi_mask=1 << det, aoi_cap=cap, aoi_count=aoi_count,
                    aoi_index=aoi_index, aoi_out=aoi_out, n=n, block=args.block, flags=flags, refill=args.refill)
 
-    flags = N.RUN_SIMPLE if args.simple else 0
+    flags = N.RUN_SIMPLE if args.simple else (N.RUN_LANES if args.lanes else 0)
     for _ in range(args.warmup):
         step()
     ctx.sync()
@@ -377,8 +378,10 @@ def main():
         "config": {"workload": workload, "molecules": n, "outputs": "per-element tallies + compacted detected phase space"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": traffic,
-                     "kernel": "k_propagate<WSPLINE_HEX,RK4>" if args.simple or args.config != "lens"
-                     else "k_propagate_lanes<WSPLINE_HEX,RK4>", "kernel_ms_avg": kavg,
+                     "kernel": ("k_propagate<WSPLINE_HEX,RK4>" if args.simple or args.config != "lens"
+                                else "k_propagate_lanes<WSPLINE_HEX,RK4>" if args.lanes
+                                else "k_propagate<PRE> + k_scan_u32 + k_lens<WSPLINE_HEX,RK4> (one ctraj_propagate)"),
+                     "kernel_ms_avg": kavg,
                      "flop_per_launch": flop, "faces_per_launch": faces, "rk4_steps_per_launch": steps_},
         "region_ms_per_step": region_ms / K,
         "counts": counts,

@@ -498,6 +498,7 @@ int32_t ctraj_smooth_objective(ctraj_ctx *c, const ctraj_smooth *s) {
   const size_t lds = ktab_lds_bytes(kt);
   const void *kern = grad ? (const void *)k_smooth<true> : (const void *)k_smooth<false>;
   if ((rc = lds_limit(c, kern, lds)) != CTRAJ_OK) return rc;
+  if ((rc = block_ok(c, kern, block)) != CTRAJ_OK) return rc;
 
   SmoothArgs A{};
   A.x = s->x; A.y = s->y; A.vx = s->vx; A.vy = s->vy; A.eps = s->eps;

@@ -283,6 +283,7 @@ static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refi
   const size_t lds = (tab + b->n_cfg) * sizeof(double) + (size_t)b->n_cfg * b->n_labels * sizeof(uint32_t);
   int rc = lds_limit(c, (const void *)k_batch_lanes, lds);
   if (rc != CTRAJ_OK) return rc;
+  if ((rc = block_ok(c, (const void *)k_batch_lanes, block)) != CTRAJ_OK) return rc;
   if (grid <= 0) grid = grid_for(c, (const void *)k_batch_lanes, block, lds, (int64_t)1 << 40, 0);
   const int64_t nz = 1 + CTRAJ_WORK_N;
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, b->d_counter, nz);

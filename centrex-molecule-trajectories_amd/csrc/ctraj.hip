@@ -62,6 +62,31 @@ __device__ __forceinline__ void aoi_flush_wave(const ctraj_run &R, const AoiStag
 struct InMol {
   double x, y, z, vx, vy, vz, t, ax, ay, az;
   int st;
+  int status;
+  int64_t idx;
+};
+
+// ---- two-phase propagation of beamlines with one lens ----
+// PRE walks every molecule up to and through the lens entrance and queues the molecules that enter
+// the lens, each wavefront into its own region of the queue; k_lens integrates the queue with lane
+// refill and walks the molecules that leave the lens through the remaining elements.  A molecule
+// is recorded (outputs, tallies, kept-molecule compaction) by the phase in which it finishes.
+enum : int { PHASE_FULL = 0, PHASE_PRE = 1 };
+
+struct PipeQ {
+  double *v[7];                // x, y, z, vx, vy, vz, t
+  int64_t *idx;                // molecule index within the run
+  uint8_t *st;                 // lens state
+};
+
+struct PipeArgs {
+  PipeQ q;                     // lens-entry queue
+  uint32_t *wcount;            // PRE: entries queued by each wavefront
+  uint32_t *woff;              // exclusive prefix of wcount: [n_pre_waves + 1]
+  int64_t region;              // queue entries owned by each PRE wavefront
+  int n_pre_waves;
+  unsigned long long *claim;   // lens kernel: next unclaimed queue position
+  int lens_e;
 };
 
 __device__ __forceinline__ void load_mol(const ctraj_run &R, const KBeam &B, int64_t i, InMol &q) {
@@ -76,10 +101,12 @@ __device__ __forceinline__ void load_mol(const ctraj_run &R, const KBeam &B, int
   q.ay = R.a0[1] ? R.a0[1][i] : -B.g;
   q.az = R.a0[2] ? R.a0[2][i] : 0.0;
   q.st = R.state ? min((int)R.state[i], B.n_states - 1) : 0;
+  q.status = 0;
+  q.idx = i;
 }
 
-template <int FORCE, int INTEG>
-__global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R, int aoi_lds_off) {
+template <int FORCE, int INTEG, int PHASE>
+__global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R, int aoi_lds_off, PipeArgs P) {
   extern __shared__ double lds[];
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
@@ -89,7 +116,7 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
   for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x) s_first[b] = ~0ull >> 1;
   if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
   Tables T{};
-  if constexpr (INTEG != CTRAJ_INTEG_NONE) T = stage_tables(lds, KT);
+  if constexpr (INTEG != CTRAJ_INTEG_NONE && PHASE == PHASE_FULL) T = stage_tables(lds, KT);
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
   AoiStage A;
@@ -105,22 +132,58 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   WalkCounters wc{0u, 0u};
   uint32_t n_oot = 0, n_cap = 0;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n_items = R.n;
+  const int64_t gwave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  int64_t qn = 0;  // PRE: entries this wavefront has queued
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   InMol q;
-  if (i < R.n) load_mol(R, B, i, q);
+  if (j < n_items) load_mol(R, B, j, q);
   // wave-uniform trip count: every lane of a wavefront runs the same iterations
-  for (int64_t wbase = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < R.n; wbase += stride, i += stride) {
-    const bool active = i < R.n;
+  for (int64_t wbase = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < n_items;
+       wbase += stride, j += stride) {
+    const bool active = j < n_items;
     MState m;
     m.x = q.x; m.y = q.y; m.z = q.z; m.vx = q.vx; m.vy = q.vy; m.vz = q.vz; m.t = q.t;
     m.ax = q.ax; m.ay = q.ay; m.az = q.az;
     const int st = q.st;
-    if (i + stride < R.n) load_mol(R, B, i + stride, q);  // prefetch the next molecule
+    const int64_t i = q.idx;
+    uint32_t status = (uint32_t)q.status;
+    if (j + stride < n_items) load_mol(R, B, j + stride, q);  // prefetch the next molecule
     int label = 0;
+    bool enq = false;
     if (active) {
-      uint32_t status = 0;
-      NullSink sk;
-      walk<FORCE, INTEG>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status, wc, sk);
+      if constexpr (PHASE == PHASE_FULL) {
+        NullSink sk;
+        walk<FORCE, INTEG>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status, wc, sk);
+      } else if constexpr (PHASE == PHASE_PRE) {
+        label = B.label_detected;
+        if (walk_plain(B.elems, 0, P.lens_e, B.g, m, label, wc)) {
+          const ctraj_elem &L = B.elems[P.lens_e];
+          drift_dt(m, time_to(m, L.z0), B.g);  // lens entrance (electrostatic_lens.py:59-64)
+          ++wc.faces;
+          if (rho_of(m.x, m.y) > L.p[0]) label = L.label_hit;
+          else enq = true;
+        }
+      }
+    }
+    if constexpr (PHASE == PHASE_PRE) {
+      // wavefront-private region of the queue: no atomics
+      const uint64_t qm = __ballot(enq);
+      if (enq) {
+        const int64_t jq = gwave * P.region + qn + lanes_below(qm);
+        P.q.v[0][jq] = m.x;
+        P.q.v[1][jq] = m.y;
+        P.q.v[2][jq] = m.z;
+        P.q.v[3][jq] = m.vx;
+        P.q.v[4][jq] = m.vy;
+        P.q.v[5][jq] = m.vz;
+        P.q.v[6][jq] = m.t;
+        P.q.idx[jq] = i;
+        P.q.st[jq] = (uint8_t)st;
+      }
+      qn += __popcll(qm);
+    }
+    if (active && !enq) {
       if (R.out.x) R.out.x[i] = m.x;
       if (R.out.y) R.out.y[i] = m.y;
       if (R.out.z) R.out.z[i] = m.z;
@@ -139,20 +202,20 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
       wave_hist(s_hist, R.first_index ? s_first : nullptr, st * B.n_labels + label, label, R.index_base + i);
     }
     if (R.aoi_mask) {
-      const bool keep = active && ((R.aoi_mask >> label) & 1ull);
+      const bool keep = active && !enq && ((R.aoi_mask >> label) & 1ull);
       const uint64_t mask = __ballot(keep);
       if (mask) {
         if (keep) {
-          const int j = aoi_n + lanes_below(mask);
-          A.idx[j] = R.index_base + i;
-          A.label[j] = label;
-          A.val[0 * AOI_WAVE + j] = m.x;
-          A.val[1 * AOI_WAVE + j] = m.y;
-          A.val[2 * AOI_WAVE + j] = m.z;
-          A.val[3 * AOI_WAVE + j] = m.vx;
-          A.val[4 * AOI_WAVE + j] = m.vy;
-          A.val[5 * AOI_WAVE + j] = m.vz;
-          A.val[6 * AOI_WAVE + j] = m.t;
+          const int slot = aoi_n + lanes_below(mask);
+          A.idx[slot] = R.index_base + i;
+          A.label[slot] = label;
+          A.val[0 * AOI_WAVE + slot] = m.x;
+          A.val[1 * AOI_WAVE + slot] = m.y;
+          A.val[2 * AOI_WAVE + slot] = m.z;
+          A.val[3 * AOI_WAVE + slot] = m.vx;
+          A.val[4 * AOI_WAVE + slot] = m.vy;
+          A.val[5 * AOI_WAVE + slot] = m.vz;
+          A.val[6 * AOI_WAVE + slot] = m.t;
         }
         aoi_n += __popcll(mask);
         if (aoi_n > AOI_WAVE - 64) {
@@ -163,6 +226,9 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
     }
   }
   if (R.aoi_mask && aoi_n > 0) aoi_flush_wave(R, A, aoi_n, lane);
+  if constexpr (PHASE == PHASE_PRE) {
+    if (lane == 0) P.wcount[gwave] = (uint32_t)qn;
+  }
   if (R.work) {
     if (wc.faces) atomicAdd(&s_work[0], (unsigned long long)wc.faces);
     if (wc.steps) atomicAdd(&s_work[1], (unsigned long long)wc.steps);
@@ -408,6 +474,261 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
           s.phase = PH_NEED;
         } else if (s.i == s.nfull) {
           s.phase = PH_POST;
+        }
+      }
+    }
+  }
+
+  const ctraj_run &R = *Rg;
+  if (R.work) {
+    if (wc.faces) atomicAdd(&s_work[0], (unsigned long long)wc.faces);
+    if (lane == 0 && steps_w) atomicAdd(&s_work[1], steps_w);
+    if (n_oot) atomicAdd(&s_work[2], (unsigned long long)n_oot);
+    if (n_cap) atomicAdd(&s_work[3], (unsigned long long)n_cap);
+  }
+  __syncthreads();
+  if (R.tally)
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+      if (s_hist[b]) atomicAdd((unsigned long long *)&R.tally[b], (unsigned long long)s_hist[b]);
+  if (R.first_index)
+    for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x)
+      if (s_first[b] != (~0ull >> 1)) atomicMin((unsigned long long *)&R.first_index[b], s_first[b]);
+  if (R.work && threadIdx.x < 4 && s_work[threadIdx.x])
+    atomicAdd((unsigned long long *)&R.work[threadIdx.x], s_work[threadIdx.x]);
+}
+
+// Exclusive prefix of the PRE wavefronts' queue counts: woff[w] = sum of wcount[0..w), woff[n] = total.
+// One workgroup of 1024 threads; fixed order.
+__global__ void __launch_bounds__(1024) k_scan_u32(const uint32_t *wcount, int n, uint32_t *woff) {
+  __shared__ uint32_t part[1024];
+  const int per = (n + 1023) / 1024;
+  const int b = threadIdx.x * per, e = min(b + per, n);
+  uint32_t sum = 0;
+  for (int k = b; k < e; ++k) sum += wcount[k];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int k = 0; k < 1024; ++k) {
+      const uint32_t v = part[k];
+      part[k] = run;
+      run += v;
+    }
+    woff[n] = run;
+  }
+  __syncthreads();
+  uint32_t run = part[threadIdx.x];
+  for (int k = b; k < e; ++k) {
+    woff[k] = run;
+    run += wcount[k];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_lens: the lens and what follows it.  Persistent wavefronts integrate the lens-entry queue
+// with lane refill: every iteration advances all lanes inside the lens by one step; when at least
+// `refill_min` lanes are idle (or none is in the lens), a convergent section walks the molecules
+// that left the lens through the remaining elements, records every finished molecule (outputs,
+// tallies, kept-molecule compaction) and gives idle lanes the next queued molecules.  Queue
+// position g lives in the region of the PRE wavefront w with woff[w] <= g < woff[w + 1] (binary
+// search of woff, staged in LDS).  The per-molecule arithmetic is the one of walk().
+// ---------------------------------------------------------------------------------------------
+enum : int { LN_IDLE = 0, LN_LENS = 1, LN_EXIT = 2, LN_FIN = 3, LN_DONE = 4 };
+
+template <int FORCE, int INTEG>
+__global__ void __launch_bounds__(768)
+k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refill_min, int chunk, int woff_lds) {
+  extern __shared__ double lds[];
+  __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
+  __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
+  __shared__ unsigned long long s_work[4];
+  const int nbins = B.n_states * B.n_labels;
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0;
+  for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x) s_first[b] = ~0ull >> 1;
+  if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
+  const Tables T = stage_tables(lds, KT);
+  uint32_t *s_woff = reinterpret_cast<uint32_t *>(lds + woff_lds);
+  const int nw = P.n_pre_waves;
+  for (int k = threadIdx.x; k <= nw; k += blockDim.x) s_woff[k] = P.woff[k];
+  __syncthreads();
+
+  const int lane = lane_id();
+  const ctraj_elem &L0 = B.elems[P.lens_e];
+  const Force<FORCE> F = make_force<FORCE>(L0, B.g);
+  const double R2 = L0.p[1], z1 = L0.z1;
+  const int label_inside = L0.label_inside;
+  const int64_t nq = (int64_t)s_woff[nw];
+  WalkCounters wc{0u, 0u};
+  uint32_t n_oot = 0, n_cap = 0;
+  unsigned long long steps_w = 0;
+  int64_t next = 0, end = 0;
+  bool exhausted = false;
+
+  MState m;
+  m.ax = 0.0; m.ay = -B.g; m.az = 0.0;
+  double r2 = 0.0, h = 0.0, hp = 0.0, dzs = 0.0;
+  int i = 0, nfull = 0, st = 0, ph = LN_IDLE, label = 0;
+  uint32_t status = 0;
+  int64_t idx = 0;
+  const lds_double *wsb = ws_of(T, 0);
+
+  for (;;) {
+    const uint64_t lensm = __ballot(ph == LN_LENS);
+    if (lensm == 0 || 64 - __popcll(lensm) >= refill_min) {
+      for (;;) {
+        // molecules that left the lens walk the remaining elements
+        if (ph == LN_EXIT) {
+          if constexpr (INTEG == CTRAJ_INTEG_REF) {
+            drift_dt(m, time_to(m, z1), B.g);  // exit hop with the stored a = l1 (electrostatic_lens.py:70-75)
+            ++wc.faces;
+          } else {
+            m.ax = 0.0;  // the exit row at z1 stores (0, -g, 0)
+            m.ay = -B.g;
+            m.az = 0.0;
+          }
+          label = B.label_detected;
+          walk_plain(opaque(B.elems), P.lens_e + 1, B.n_elems, B.g, m, label, wc);
+          ph = LN_FIN;
+        }
+        // record finished molecules
+        if (__ballot(ph == LN_FIN)) {
+          const ctraj_run &R = *opaque(Rg);
+          if (ph == LN_FIN) {
+            if (R.out.x) R.out.x[idx] = m.x;
+            if (R.out.y) R.out.y[idx] = m.y;
+            if (R.out.z) R.out.z[idx] = m.z;
+            if (R.out.vx) R.out.vx[idx] = m.vx;
+            if (R.out.vy) R.out.vy[idx] = m.vy;
+            if (R.out.vz) R.out.vz[idx] = m.vz;
+            if (R.out.t) R.out.t[idx] = m.t;
+            if (R.out_a[0]) R.out_a[0][idx] = m.ax;
+            if (R.out_a[1]) R.out_a[1][idx] = m.ay;
+            if (R.out_a[2]) R.out_a[2][idx] = m.az;
+            if (R.label) R.label[idx] = label;
+            if (R.status)
+              R.status[idx] = (R.flags & CTRAJ_RUN_STATUS_OR) ? (uint8_t)(R.status[idx] | status) : (uint8_t)status;
+            n_oot += (status & CTRAJ_ST_OOT) ? 1u : 0u;
+            n_cap += (status & CTRAJ_ST_CAPPED) ? 1u : 0u;
+            wave_hist(s_hist, R.first_index ? s_first : nullptr, st * B.n_labels + label, label, R.index_base + idx);
+          }
+          if (R.aoi_mask) {
+            const bool keep = ph == LN_FIN && ((R.aoi_mask >> label) & 1ull);
+            const uint64_t mask = __ballot(keep);
+            if (mask) {
+              unsigned long long base = 0;
+              if (lane == 0) base = atomicAdd((unsigned long long *)R.aoi_count, (unsigned long long)__popcll(mask));
+              base = __shfl(base, 0);
+              if (keep) {
+                const int64_t slot = (int64_t)base + lanes_below(mask);
+                if (slot < R.aoi_cap) {
+                  R.aoi_index[slot] = R.index_base + idx;
+                  if (R.aoi_label) R.aoi_label[slot] = label;
+                  if (R.aoi_out.x) R.aoi_out.x[slot] = m.x;
+                  if (R.aoi_out.y) R.aoi_out.y[slot] = m.y;
+                  if (R.aoi_out.z) R.aoi_out.z[slot] = m.z;
+                  if (R.aoi_out.vx) R.aoi_out.vx[slot] = m.vx;
+                  if (R.aoi_out.vy) R.aoi_out.vy[slot] = m.vy;
+                  if (R.aoi_out.vz) R.aoi_out.vz[slot] = m.vz;
+                  if (R.aoi_out.t) R.aoi_out.t[slot] = m.t;
+                }
+              }
+            }
+          }
+          if (ph == LN_FIN) ph = LN_IDLE;
+        }
+        const uint64_t need = __ballot(ph == LN_IDLE);
+        if (!need || exhausted) {
+          if (ph == LN_IDLE && exhausted) ph = LN_DONE;
+          break;
+        }
+        if (next >= end) {
+          unsigned long long c = 0;
+          if (lane == 0) c = atomicAdd(P.claim, (unsigned long long)chunk);
+          c = __shfl(c, 0);
+          next = (int64_t)c;
+          end = next + chunk < nq ? next + chunk : nq;
+          if (next >= end) {
+            exhausted = true;
+            continue;
+          }
+        }
+        const int64_t avail = end - next;
+        if (ph == LN_IDLE) {
+          const int rank = lanes_below(need);
+          if (rank < avail) {
+            const uint32_t g = (uint32_t)(next + rank);
+            int lo = 0, hi = nw;  // the PRE wavefront whose region holds queue position g
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (s_woff[mid] <= g) lo = mid;
+              else hi = mid;
+            }
+            const int64_t j = (int64_t)lo * P.region + (int64_t)(g - s_woff[lo]);
+            m.x = P.q.v[0][j];
+            m.y = P.q.v[1][j];
+            m.z = P.q.v[2][j];
+            m.vx = P.q.v[3][j];
+            m.vy = P.q.v[4][j];
+            m.vz = P.q.v[5][j];
+            m.t = P.q.v[6][j];
+            m.ax = 0.0;
+            m.ay = -B.g;
+            m.az = 0.0;
+            idx = P.q.idx[j];
+            st = P.q.st[j];
+            wsb = ws_of(T, st);
+            status = 0;
+            i = 0;
+            const ctraj_elem &L = *opaque(B.elems + P.lens_e);
+            if constexpr (INTEG == CTRAJ_INTEG_RK4) {
+              const double Hs = L.p[2], Ll = L.p[3];
+              nfull = rk4_full_steps(Ll, m.vz, Hs, B.cap, status);
+              dzs = m.vz * Hs;
+              hp = Ll / m.vz - (double)nfull * Hs;
+              h = nfull == 0 ? hp : Hs;
+              r2 = __builtin_fma(m.x, m.x, m.y * m.y);
+              ph = LN_LENS;
+            } else {
+              nfull = L.n_steps;
+              h = L.p[2] / m.vz;
+              ph = nfull > 0 ? LN_LENS : LN_EXIT;
+            }
+          }
+        }
+        next += avail < (int64_t)__popcll(need) ? avail : (int64_t)__popcll(need);
+      }
+      if (!__ballot(ph == LN_LENS)) {
+        if (!__ballot(ph == LN_EXIT || ph == LN_FIN)) break;
+        continue;
+      }
+    }
+
+    steps_w += (unsigned long long)__popcll(__ballot(ph == LN_LENS));
+    if (ph == LN_LENS) {
+      if constexpr (INTEG == CTRAJ_INTEG_RK4) {
+        const bool last = (i == nfull);
+        rk4_step<FORCE>(F, T, wsb, h, m.x, m.y, m.vx, m.vy, r2, status);
+        m.z = last ? z1 : m.z + dzs;
+        m.t = m.t + h;
+        r2 = __builtin_fma(m.x, m.x, m.y * m.y);
+        ++i;
+        h = (i == nfull) ? hp : h;
+        const bool dead = r2 > R2;
+        label = dead ? label_inside : label;
+        ph = dead ? (int)LN_FIN : (last ? (int)LN_EXIT : (int)LN_LENS);
+        if (dead) {
+          uint32_t st2 = 0;
+          F(T, wsb, m.x, m.y, m.ax, m.ay, st2);  // the stopping row stores the force there
+          m.az = 0.0;
+        }
+      } else {
+        ref_step<FORCE>(F, T, wsb, h, m, status);  // the row stores a = l1
+        ++i;
+        if (rho_of(m.x, m.y) > L0.p[0]) {
+          label = label_inside;
+          ph = LN_FIN;
+        } else if (i == nfull) {
+          ph = LN_EXIT;
         }
       }
     }
@@ -694,6 +1015,15 @@ int lds_limit(ctraj_ctx *ctx, const void *kern, size_t bytes) {
   return CTRAJ_OK;
 }
 
+int block_ok(ctraj_ctx *ctx, const void *kern, int block) {
+  hipFuncAttributes fa{};
+  HIPCHK(ctx, hipFuncGetAttributes(&fa, kern));
+  if (block < 1 || block > fa.maxThreadsPerBlock)
+    return fail(ctx, CTRAJ_EINVAL, "block of " + std::to_string(block) + " threads exceeds the kernel's limit of " +
+                                       std::to_string(fa.maxThreadsPerBlock));
+  return CTRAJ_OK;
+}
+
 int grid_for(ctraj_ctx *ctx, const void *kern, int block, size_t lds, int64_t n, int req) {
   if (req > 0) return req;
   int per_cu = 0;
@@ -731,14 +1061,23 @@ int ensure_buf(ctraj_ctx *ctx, ctraj_devbuf &b, size_t bytes) {
   X(CTRAJ_FORCE_WSPLINE_LIN, CTRAJ_INTEG_REF)        \
   X(CTRAJ_FORCE_WSPLINE_LIN, CTRAJ_INTEG_RK4)
 
-using PropFn = void (*)(KBeam, KTab, ctraj_run, int);
+using PropFn = void (*)(KBeam, KTab, ctraj_run, int, PipeArgs);
+using LensFn = void (*)(KBeam, KTab, const ctraj_run *, PipeArgs, int, int, int);
 using LaneFn = void (*)(KBeam, KTab, const ctraj_run *, LaneArgs);
 using RecFn = void (*)(KBeam, KTab, ctraj_record);
 
 static PropFn prop_kernel(int f, int i, bool has_lens) {
-  if (!has_lens) return k_propagate<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_NONE>;
+  if (!has_lens) return k_propagate<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_NONE, PHASE_FULL>;
 #define X(F, I) \
-  if (f == F && i == I) return k_propagate<F, I>;
+  if (f == F && i == I) return k_propagate<F, I, PHASE_FULL>;
+  CT_FI_LIST(X)
+#undef X
+  return nullptr;
+}
+
+static LensFn lens_kernel(int f, int i) {
+#define X(F, I) \
+  if (f == F && i == I) return k_lens<F, I>;
   CT_FI_LIST(X)
 #undef X
   return nullptr;
@@ -815,6 +1154,8 @@ int32_t ctraj_ctx_destroy(ctraj_ctx *c) {
   if (c->sm_ws.p) (void)hipFree(c->sm_ws.p);
   if (c->sm_part.p) (void)hipFree(c->sm_part.p);
   if (c->sort_tmp.p) (void)hipFree(c->sort_tmp.p);
+  if (c->pipe.p) (void)hipFree(c->pipe.p);
+  if (c->pipe_cnt.p) (void)hipFree(c->pipe_cnt.p);
   if (c->sort_alt.p) (void)hipFree(c->sort_alt.p);
   for (int i = 0; i < 64; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -1010,19 +1351,73 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
       ++n_lens;
       lens_e = e;
     }
-  const bool lanes = n_lens == 1 && !(R->flags & CTRAJ_RUN_SIMPLE);
+  // queue positions are 32-bit: larger single runs use the single-kernel walker
+  const bool pipeline = n_lens == 1 && !(R->flags & (CTRAJ_RUN_SIMPLE | CTRAJ_RUN_LANES)) && R->n < ((int64_t)1 << 32);
+  const bool lanes = n_lens == 1 && !(R->flags & CTRAJ_RUN_SIMPLE) && (R->flags & CTRAJ_RUN_LANES);
   // The lane walker keeps its tables once per workgroup: with several lens-state tables (more
   // than ~40 KB) one 768-thread workgroup per CU holds as many wavefronts as its registers allow,
   // where 256-thread workgroups would be limited by LDS to one per CU.
-  const int block = R->block ? R->block : (lanes && lds > 40960 ? 768 : 256);
-  const int max_block = lanes ? 768 : 256;
+  const int block = R->block ? R->block : ((lanes || pipeline) && lds > 40960 ? 768 : 256);
+  const int max_block = (lanes || pipeline) ? 768 : 256;
   if (block < 64 || block > max_block || block % 64)
     return fail(c, CTRAJ_EINVAL, "block must be a multiple of 64 up to " + std::to_string(max_block));
   HIPCHK(c, hipSetDevice(c->device));
-  if (lanes) {
+  if (pipeline) {
+    LensFn kl = lens_kernel(c->force, c->integ);
+    if (!kl) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
+    // phase 1: walk to the lens and through its entrance; each wavefront queues the molecules that
+    // enter the lens into its own region (no atomics)
+    const int blk = 256;
+    const size_t lds_walk = R->aoi_mask ? (size_t)(blk / 64) * AOI_WAVE * AOI_ENTRY_BYTES : 0;
+    PropFn kpre = k_propagate<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_NONE, PHASE_PRE>;
+    int rc = lds_limit(c, (const void *)kpre, lds_walk);
+    if (rc) return rc;
+    int gpre = grid_for(c, (const void *)kpre, blk, lds_walk, R->n, R->grid);
+    gpre = std::min(gpre, 1024 / (blk / 64));  // at most 1024 queue regions (their offsets sit in LDS)
+    const int nw = gpre * (blk / 64);
+    const int64_t threads = (int64_t)gpre * blk;
+    const int64_t region = (R->n + threads - 1) / threads * 64;
+    const size_t cap = (size_t)nw * (size_t)region;
+    const size_t bytes = cap * (7 * sizeof(double) + sizeof(int64_t) + 1) + 256;
+    if ((rc = ensure_buf(c, c->pipe, bytes)) != CTRAJ_OK) return rc;
+    if ((rc = ensure_buf(c, c->pipe_cnt, 64 + (size_t)(2 * nw + 2) * sizeof(uint32_t))) != CTRAJ_OK) return rc;
+    PipeArgs P{};
+    char *b = (char *)c->pipe.p;
+    for (int k = 0; k < 7; ++k) P.q.v[k] = (double *)(b + (size_t)k * cap * 8);
+    P.q.idx = (int64_t *)(b + 7 * cap * 8);
+    P.q.st = (uint8_t *)(b + 8 * cap * 8);
+    char *cb = (char *)c->pipe_cnt.p;
+    P.claim = (unsigned long long *)cb;
+    P.wcount = (uint32_t *)(cb + 64);
+    P.woff = P.wcount + nw;
+    P.region = region;
+    P.n_pre_waves = nw;
+    P.lens_e = lens_e;
+    if ((rc = ctraj_fill_i64(c, (int64_t *)P.claim, 0, 1)) != CTRAJ_OK) return rc;
+    hipLaunchKernelGGL(kpre, dim3(gpre), dim3(blk), lds_walk, c->stream, B, kt, *R, 0, P);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, c->stream, (const uint32_t *)P.wcount, nw, P.woff);
+    HIPCHK(c, hipGetLastError());
+    // phase 2: the lens and the elements after it
+    const size_t lds_lens = ((lds + 7) & ~(size_t)7) + (size_t)(nw + 1) * sizeof(uint32_t);
+    if ((rc = lds_limit(c, (const void *)kl, lds_lens)) != CTRAJ_OK) return rc;
+    if ((rc = block_ok(c, (const void *)kl, block)) != CTRAJ_OK) return rc;
+    if (!c->d_run) HIPCHK(c, hipMalloc(&c->d_run, sizeof(ctraj_run)));
+    if (!c->have_last_run || std::memcmp(&c->last_run, R, sizeof(ctraj_run)) != 0) {
+      HIPCHK(c, hipMemcpyAsync(c->d_run, R, sizeof(ctraj_run), hipMemcpyHostToDevice, c->stream));
+      c->last_run = *R;
+      c->have_last_run = true;
+    }
+    const int refill = R->refill > 0 ? std::min(R->refill, 64) : 16;
+    const int grid = grid_for(c, (const void *)kl, block, lds_lens, (int64_t)1 << 40, 0);
+    hipLaunchKernelGGL(kl, dim3(grid), dim3(block), lds_lens, c->stream, B, kt, (const ctraj_run *)c->d_run, P, refill,
+                       64, (int)((lds + 7) / 8));
+  } else if (lanes) {
     LaneFn k = lane_kernel(c->force, c->integ);
     if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
     int rc = lds_limit(c, (const void *)k, lds);
+    if (rc) return rc;
+    rc = block_ok(c, (const void *)k, block);
     if (rc) return rc;
     if (!c->d_counter) HIPCHK(c, hipMalloc((void **)&c->d_counter, 256));
     rc = ctraj_fill_i64(c, (int64_t *)c->d_counter, 0, 1);
@@ -1050,8 +1445,10 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     const size_t lds_all = lds_tab + (R->aoi_mask ? (size_t)(block / 64) * AOI_WAVE * AOI_ENTRY_BYTES : 0);
     int rc = lds_limit(c, (const void *)k, lds_all);
     if (rc) return rc;
+    rc = block_ok(c, (const void *)k, block);
+    if (rc) return rc;
     const int grid = grid_for(c, (const void *)k, block, lds_all, R->n, R->grid);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds_all, c->stream, B, kt, *R, aoi_off);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds_all, c->stream, B, kt, *R, aoi_off, PipeArgs{});
   }
   HIPCHK(c, hipGetLastError());
   return CTRAJ_OK;
@@ -1075,6 +1472,8 @@ int32_t ctraj_record_trajectories(ctraj_ctx *c, const ctraj_record *RC) {
   if (c->force == CTRAJ_FORCE_RTABLE) kt.ws_states = 0;
   const size_t lds = ktab_lds_bytes(kt);
   int rc = lds_limit(c, (const void *)k, lds);
+  if (rc) return rc;
+  rc = block_ok(c, (const void *)k, block);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   const int grid = grid_for(c, (const void *)k, block, lds, RC->n, 0);

@@ -45,6 +45,8 @@ struct ctraj_ctx {
   ctraj_devbuf sm_ws, sm_part;
   // radix sort scratch (ctraj_sort_i64)
   ctraj_devbuf sort_tmp, sort_alt;
+  // three-phase propagation: lens-entry / lens-exit queues and their counters
+  ctraj_devbuf pipe, pipe_cnt;
 };
 
 namespace ctraj {
@@ -76,6 +78,8 @@ int check_phys(ctraj_ctx *ctx, const ctraj_phys *p);
 KBeam make_kbeam(const ctraj_elem *d_elems, int n, const ctraj_phys &p);
 int lds_limit(ctraj_ctx *ctx, const void *kern, size_t bytes);
 int grid_for(ctraj_ctx *ctx, const void *kern, int block, size_t lds, int64_t n, int req);
+// EINVAL unless `block` threads fit the kernel's launch bounds
+int block_ok(ctraj_ctx *ctx, const void *kern, int block);
 // make b hold at least `bytes` (grow-only; the old buffer is released first)
 int ensure_buf(ctraj_ctx *ctx, ctraj_devbuf &b, size_t bytes);
 

@@ -41,24 +41,22 @@ __device__ __forceinline__ int lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Wave-aggregated histogram: one LDS atomic per distinct key in the wave.  The leader of each
-// key group is its lowest lane, which also holds the group's lowest molecule index.
+// Wave-aggregated histogram: one LDS atomic per distinct key in the wavefront (its lowest lane
+// adds the group's count).  The first molecule index of each label is an LDS atomic minimum per
+// lane, so lanes need not hold their molecules in index order (the queue-fed phases do not).
 __device__ __forceinline__ void wave_hist(uint32_t *s_hist, unsigned long long *s_first, int key, int label,
                                           int64_t gidx) {
+  if (s_first) atomicMin(&s_first[label], (unsigned long long)gidx);
   uint64_t todo = __ballot(1);
   const int lane = lane_id();
   while (todo) {
     const int leader = __ffsll((unsigned long long)todo) - 1;
     const int lk = __shfl(key, leader);
     const uint64_t same = __ballot(key == lk) & todo;
-    if (lane == leader) {
-      atomicAdd(&s_hist[lk], (uint32_t)__popcll(same));
-      if (s_first) atomicMin(&s_first[label], (unsigned long long)gidx);
-    }
+    if (lane == leader) atomicAdd(&s_hist[lk], (uint32_t)__popcll(same));
     todo &= ~same;
   }
 }
-
 
 // Sum over the 64 lanes by an xor butterfly: every lane receives the same, bitwise identical
 // total (each level adds the same two partial sums, only in swapped order), so the result does not

@@ -131,12 +131,16 @@ int32_t ctraj_set_rtable(ctraj_ctx *ctx, int32_t id, const double *r, const doub
 int32_t ctraj_set_wspline(ctraj_ctx *ctx, int32_t n_states, int32_t n_seg, const double *coef);
 
 /* ---- forward propagation: replaces run_simulation's per-molecule loop (trajectory_simulator.py:52-78)
- *      and Beamline.propagate_through (beamline.py:20-38).  A beamline with exactly one lens runs on
- *      the persistent lane-refill kernel (each wavefront keeps its lanes busy by handing idle lanes
- *      new molecules); other beamlines, or CTRAJ_RUN_SIMPLE, run one molecule per thread.  Both give
- *      bit-identical results. ---- */
+ *      and Beamline.propagate_through (beamline.py:20-38).  A beamline with exactly one lens runs in
+ *      three phases queued on the stream: a streaming walk to and through the lens entrance that
+ *      queues the molecules entering the lens, a persistent lane-refill lens kernel that queues the
+ *      molecules leaving it, and a streaming walk of the remaining elements (context-owned queues of
+ *      131 bytes per molecule).  CTRAJ_RUN_LANES runs the single-kernel lane-refill walker instead;
+ *      other beamlines, or CTRAJ_RUN_SIMPLE, run one molecule per thread.  All give bit-identical
+ *      results. ---- */
 #define CTRAJ_RUN_STATUS_OR 1 /* OR status bits into run->status instead of overwriting */
 #define CTRAJ_RUN_SIMPLE 2    /* one molecule per thread start to finish (no lane refill) */
+#define CTRAJ_RUN_LANES 4     /* one lens: the single-kernel lane-refill walker instead of the three phases */
 typedef struct ctraj_run {
   int64_t n;             /* molecules */
   int64_t index_base;    /* global index of molecule 0 (for first_index / aoi_index in chunked runs) */
@@ -161,7 +165,7 @@ typedef struct ctraj_run {
                           * 0 = 256, or 768 for the lane walker when the lens tables exceed 40 KB of LDS */
   int32_t grid;          /* blocks (0 = enough for n, capped at 8 per CU x n_cu x 8) */
   int32_t flags;         /* CTRAJ_RUN_* */
-  int32_t refill;        /* lane-refill kernel: idle lanes per wavefront that trigger a refill (0 = 24) */
+  int32_t refill;        /* lane refill: idle lanes per wavefront that trigger a refill (0 = 16 two-phase, 24 single kernel) */
 } ctraj_run;
 int32_t ctraj_propagate(ctraj_ctx *ctx, const ctraj_run *run);
 

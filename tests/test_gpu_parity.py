@@ -218,19 +218,24 @@ def test_beamline_propagate_single_molecule_smoke_kat():
 
 
 def test_lane_refill_kernel_matches_simple_kernel():
-    """The persistent lane-refill walker only reorders work: every refill threshold and the
-    one-thread-per-molecule kernel give identical bits."""
-    for bl, seed in ((configs.config1(), 41), (reference_example_beamline(), 42)):
+    """The three-phase pipeline and the single-kernel lane-refill walker only reorder work: every
+    refill threshold, both of them and the one-thread-per-molecule kernel give identical bits, in
+    configuration mode (one and four lens states) and in reference mode.  Molecules stopped inside
+    the lens are kept too, so the lens phase's own output path is compared."""
+    cases = ((configs.config1(), 41, None), (reference_example_beamline(), 42, None), (configs.config4(), 43, 4))
+    for bl, seed, ns in cases:
         X, V = configs.draw_source(50_000, seed=seed)
         if bl.elements[0].name == "4K shield":
             X[:2] *= 0.05
             V[:2] *= 0.05
-        eng = Engine(bl)
-        base = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], flags=N.RUN_SIMPLE, want_accel=True,
-                             aoi_labels=("Detected",))
-        for refill in (1, 16, 64):
-            res = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], refill=refill, want_accel=True,
-                                aoi_labels=("Detected",))
+        st = None if ns is None else np.random.RandomState(seed).randint(0, ns, X.shape[1]).astype(np.uint8)
+        eng = Engine(bl, n_states=ns)
+        keep = ("Detected", "Inside lens")
+        base = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], state=st, flags=N.RUN_SIMPLE, want_accel=True,
+                             aoi_labels=keep)
+        for flags, refill in ((0, 1), (0, 24), (0, 64), (N.RUN_LANES, 1), (N.RUN_LANES, 16), (N.RUN_LANES, 64)):
+            res = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], state=st, refill=refill, want_accel=True,
+                                aoi_labels=keep, flags=flags)
             np.testing.assert_array_equal(res["label"], base["label"])
             np.testing.assert_array_equal(res["status"], base["status"])
             for k in FIELDS:
@@ -240,7 +245,12 @@ def test_lane_refill_kernel_matches_simple_kernel():
             np.testing.assert_array_equal(res["tally"], base["tally"])
             np.testing.assert_array_equal(res["first_index"], base["first_index"])
             np.testing.assert_array_equal(res["work"], base["work"])
-            np.testing.assert_array_equal(np.sort(res["aoi_index"]), np.sort(base["aoi_index"]))
+            o = np.argsort(res["aoi_index"])
+            ob = np.argsort(base["aoi_index"])
+            np.testing.assert_array_equal(res["aoi_index"][o], base["aoi_index"][ob])
+            np.testing.assert_array_equal(res["aoi_label"][o], base["aoi_label"][ob])
+            for k in FIELDS:
+                np.testing.assert_array_equal(res["aoi_out"][k][o], base["aoi_out"][k][ob])
 
 
 def test_small_and_empty_runs():
