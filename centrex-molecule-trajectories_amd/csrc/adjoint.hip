@@ -16,7 +16,8 @@
 // Workspace of one wavefront (doubles, lane-minor so that every access is one 512-byte row):
 //   faces [n_face_slots][4][64]   state (x, y, vx, vy) on each aperture face and lens entrance
 //   ckpt  [n_ck_slots][4][64]     lens state before steps 0, K, 2K, ...
-//   rec   [K][4][64]              recomputed step inputs of the current segment
+//   sub   [ceil(K/8)][4][64]      state every 8 steps of the segment being reversed
+// and, in LDS, a [8][4][64] window of recomputed step inputs per wavefront.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,8 +40,14 @@ struct SmoothArgs {
   unsigned long long *n_capped;   // may be null
   int64_t n, items_per_cfg, n_items, ws_stride;
   double z_src, sigma_vz, inv_s;
-  int n_params, K, max_steps, ck_per_lens, off_ck, off_rec;  // offsets in rows of 256 doubles
+  int n_params, K, max_steps, ck_per_lens, off_ck, off_sub;  // offsets in rows of 256 doubles
+  int rec_lds;                  // LDS offset (doubles) of the per-wavefront recompute windows
 };
+
+// Inside a checkpoint segment the reverse sweep works in windows of kSub steps: the segment is
+// recomputed once to store the state every kSub steps (HBM rows, cache-resident), then each
+// window, last first, is recomputed into the wavefront's LDS slice and reversed.
+constexpr int kSub = 8;
 
 // sigmoid(w) and 1 - sigmoid(w) without cancellation.
 __device__ __forceinline__ void sigmoid2(double w, double &phi, double &omphi) {
@@ -212,7 +219,7 @@ __device__ __forceinline__ double exit_z(const KBeam &B, const double *P, int e,
 __device__ __forceinline__ int n_params_of(const ctraj_elem &E) { return E.kind == CTRAJ_LENS ? 2 : 1; }
 
 template <bool GRAD>
-__global__ void __launch_bounds__(256) k_smooth(KBeam B, KTab KT, SmoothArgs A) {
+__global__ void __launch_bounds__(512) k_smooth(KBeam B, KTab KT, SmoothArgs A) {
   extern __shared__ double lds[];
   const Tables T = stage_tables(lds, KT);
   __syncthreads();
@@ -353,23 +360,51 @@ __global__ void __launch_bounds__(256) k_smooth(KBeam B, KTab KT, SmoothArgs A) 
             const double hp = Ll / vz - (double)nfull * h;
             const int ns = nfull + 1;
             const int nseg = (ns + K - 1) / K;
+            lds_double *rec = (lds_double *)(lds + A.rec_lds) + (threadIdx.x >> 6) * (kSub * 256) + lane;
             for (int sg = nseg - 1; sg >= 0; --sg) {
               const int s0 = sg * K, s1 = min(s0 + K, ns);
+              const int nsub = (s1 - s0 + kSub - 1) / kSub;
               double x, y, vx, vy;
               get4(W, A.off_ck + cslot + sg, lane, x, y, vx, vy);
               double r2 = __builtin_fma(x, x, y * y);
-              for (int s = s0; s < s1 - 1; ++s) {
-                put4(W, A.off_rec + (s - s0), lane, x, y, vx, vy);
-                rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, s < nfull ? h : hp, x, y, vx, vy, r2, st);
-                r2 = __builtin_fma(x, x, y * y);
+              // the state every kSub steps of the segment
+              for (int q = 0; q < nsub; ++q) {
+                put4(W, A.off_sub + q, lane, x, y, vx, vy);
+                if (q + 1 < nsub) {
+                  for (int s = s0 + q * kSub; s < s0 + (q + 1) * kSub; ++s) {
+                    rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, s < nfull ? h : hp, x, y, vx, vy, r2, st);
+                    r2 = __builtin_fma(x, x, y * y);
+                  }
+                }
               }
-              for (int s = s1 - 1; s >= s0; --s) {
-                if (s < s1 - 1) get4(W, A.off_rec + (s - s0), lane, x, y, vx, vy);
-                if (s == nfull) {
-                  const double hbar = rk4_adjoint<true>(LK, hp, x, y, vx, vy, pbx, pby, vbx, vby, Vbar);
-                  vzbar = vzbar - hbar * ((Ll / vz) / vz);  // hp = L/vz - nfull h
-                } else {
-                  (void)rk4_adjoint<false>(LK, h, x, y, vx, vy, pbx, pby, vbx, vby, Vbar);
+              // windows, last first: recompute into LDS, then reverse
+              for (int q = nsub - 1; q >= 0; --q) {
+                const int t0 = s0 + q * kSub, t1 = min(t0 + kSub, s1);
+                get4(W, A.off_sub + q, lane, x, y, vx, vy);
+                r2 = __builtin_fma(x, x, y * y);
+                for (int s = t0; s < t1 - 1; ++s) {
+                  lds_double *r = rec + (s - t0) * 256;
+                  r[0] = x;
+                  r[64] = y;
+                  r[128] = vx;
+                  r[192] = vy;
+                  rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, s < nfull ? h : hp, x, y, vx, vy, r2, st);
+                  r2 = __builtin_fma(x, x, y * y);
+                }
+                for (int s = t1 - 1; s >= t0; --s) {
+                  if (s < t1 - 1) {
+                    const lds_double *r = rec + (s - t0) * 256;
+                    x = r[0];
+                    y = r[64];
+                    vx = r[128];
+                    vy = r[192];
+                  }
+                  if (s == nfull) {
+                    const double hbar = rk4_adjoint<true>(LK, hp, x, y, vx, vy, pbx, pby, vbx, vby, Vbar);
+                    vzbar = vzbar - hbar * ((Ll / vz) / vz);  // hp = L/vz - nfull h
+                  } else {
+                    (void)rk4_adjoint<false>(LK, h, x, y, vx, vy, pbx, pby, vbx, vby, Vbar);
+                  }
                 }
               }
             }
@@ -487,15 +522,17 @@ int32_t ctraj_smooth_objective(ctraj_ctx *c, const ctraj_smooth *s) {
   if (!(s->softness > 0.0)) return fail(c, CTRAJ_EINVAL, "softness must be > 0");
   if (s->ckpt_every < 1) return fail(c, CTRAJ_EINVAL, "ckpt_every must be >= 1");
   if (s->max_steps < 1) return fail(c, CTRAJ_EINVAL, "max_steps must be >= 1");
-  const int block = s->block ? s->block : 256;
-  if (block % 64 || block < 64 || block > 256) return fail(c, CTRAJ_EINVAL, "block must be 64, 128, 192 or 256");
-  if (nl > 0 && (!c->d_ws || c->ws_states < 1)) return fail(c, CTRAJ_ESTATE, "no W' spline set");
   const bool grad = s->grad != nullptr;
+  // with gradients, 8 wavefronts per workgroup share the tables and keep 16 KB of LDS each
+  const int block = s->block ? s->block : (grad ? 512 : 256);
+  if (block % 64 || block < 64 || block > 512) return fail(c, CTRAJ_EINVAL, "block must be a multiple of 64 up to 512");
+  if (nl > 0 && (!c->d_ws || c->ws_states < 1)) return fail(c, CTRAJ_ESTATE, "no W' spline set");
 
   KTab kt = ctx_ktab(c);
   kt.ws_states = nl > 0 ? 1 : 0;  // state 0 only
   kt.rt_total = 0;
-  const size_t lds = ktab_lds_bytes(kt);
+  const size_t lds_tab = ktab_lds_bytes(kt);
+  const size_t lds = lds_tab + (grad ? (size_t)(block / 64) * kSub * 256 * sizeof(double) : 0);
   const void *kern = grad ? (const void *)k_smooth<true> : (const void *)k_smooth<false>;
   if ((rc = lds_limit(c, kern, lds)) != CTRAJ_OK) return rc;
   if ((rc = block_ok(c, kern, block)) != CTRAJ_OK) return rc;
@@ -515,8 +552,9 @@ int32_t ctraj_smooth_objective(ctraj_ctx *c, const ctraj_smooth *s) {
   A.max_steps = s->max_steps;
   A.ck_per_lens = (s->max_steps + 1 + s->ckpt_every - 1) / s->ckpt_every;
   A.off_ck = nf;
-  A.off_rec = nf + nl * A.ck_per_lens;
-  A.ws_stride = grad ? (int64_t)(A.off_rec + s->ckpt_every) * 256 : 0;
+  A.off_sub = nf + nl * A.ck_per_lens;
+  A.ws_stride = grad ? (int64_t)(A.off_sub + (s->ckpt_every + kSub - 1) / kSub) * 256 : 0;
+  A.rec_lds = (int)(lds_tab / sizeof(double));
 
   const int wpb = block / 64;
   int grid = grid_for(c, kern, block, lds, A.n_items * 64, 0);
