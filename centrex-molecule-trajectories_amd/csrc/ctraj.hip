@@ -105,8 +105,9 @@ __device__ __forceinline__ void load_mol(const ctraj_run &R, const KBeam &B, int
   q.idx = i;
 }
 
-template <int FORCE, int INTEG, int PHASE>
-__global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R, int aoi_lds_off, PipeArgs P) {
+template <int FORCE, int INTEG, int PHASE, int KM>
+__global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R0, int aoi_lds_off, PipeArgs P) {
+  const ctraj_run &R = R0;
   extern __shared__ double lds[];
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
@@ -154,10 +155,11 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
     if (active) {
       if constexpr (PHASE == PHASE_FULL) {
         NullSink sk;
-        walk<FORCE, INTEG>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status, wc, sk);
+        walk<FORCE, INTEG, NullSink, KM>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status,
+                                         wc, sk);
       } else if constexpr (PHASE == PHASE_PRE) {
         label = B.label_detected;
-        if (walk_plain(B.elems, 0, P.lens_e, B.g, m, label, wc)) {
+        if (walk_plain<KM>(B.elems, 0, P.lens_e, B.g, m, label, wc)) {
           const ctraj_elem &L = B.elems[P.lens_e];
           drift_dt(m, time_to(m, L.z0), B.g);  // lens entrance (electrostatic_lens.py:59-64)
           ++wc.faces;
@@ -535,7 +537,7 @@ __global__ void __launch_bounds__(1024) k_scan_u32(const uint32_t *wcount, int n
 // ---------------------------------------------------------------------------------------------
 enum : int { LN_IDLE = 0, LN_LENS = 1, LN_EXIT = 2, LN_FIN = 3, LN_DONE = 4 };
 
-template <int FORCE, int INTEG>
+template <int FORCE, int INTEG, int KM>
 __global__ void __launch_bounds__(768)
 k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refill_min, int chunk, int woff_lds) {
   extern __shared__ double lds[];
@@ -587,7 +589,7 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
             m.az = 0.0;
           }
           label = B.label_detected;
-          walk_plain(opaque(B.elems), P.lens_e + 1, B.n_elems, B.g, m, label, wc);
+          walk_plain<KM>(opaque(B.elems), P.lens_e + 1, B.n_elems, B.g, m, label, wc);
           ph = LN_FIN;
         }
         // record finished molecules
@@ -1066,18 +1068,27 @@ using LensFn = void (*)(KBeam, KTab, const ctraj_run *, PipeArgs, int, int, int)
 using LaneFn = void (*)(KBeam, KTab, const ctraj_run *, LaneArgs);
 using RecFn = void (*)(KBeam, KTab, ctraj_record);
 
-static PropFn prop_kernel(int f, int i, bool has_lens) {
-  if (!has_lens) return k_propagate<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_NONE, PHASE_FULL>;
+// KM_CIRC when every element in [e0, e1) is a circular aperture, else KM_ALL
+static int kinds_mask(const std::vector<ctraj_elem> &el, int e0, int e1) {
+  for (int e = e0; e < e1; ++e)
+    if (el[e].kind != CTRAJ_CIRC && el[e].kind != CTRAJ_LENS) return KM_ALL;
+  return KM_CIRC;
+}
+
+static PropFn prop_kernel(int f, int i, bool has_lens, int km) {
+  if (!has_lens)
+    return km == KM_CIRC ? k_propagate<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_NONE, PHASE_FULL, KM_CIRC>
+                         : k_propagate<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_NONE, PHASE_FULL, KM_ALL>;
 #define X(F, I) \
-  if (f == F && i == I) return k_propagate<F, I, PHASE_FULL>;
+  if (f == F && i == I) return k_propagate<F, I, PHASE_FULL, KM_ALL>;
   CT_FI_LIST(X)
 #undef X
   return nullptr;
 }
 
-static LensFn lens_kernel(int f, int i) {
+static LensFn lens_kernel(int f, int i, int km) {
 #define X(F, I) \
-  if (f == F && i == I) return k_lens<F, I>;
+  if (f == F && i == I) return km == KM_CIRC ? k_lens<F, I, KM_CIRC> : k_lens<F, I, KM_ALL>;
   CT_FI_LIST(X)
 #undef X
   return nullptr;
@@ -1363,13 +1374,15 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     return fail(c, CTRAJ_EINVAL, "block must be a multiple of 64 up to " + std::to_string(max_block));
   HIPCHK(c, hipSetDevice(c->device));
   if (pipeline) {
-    LensFn kl = lens_kernel(c->force, c->integ);
+    LensFn kl = lens_kernel(c->force, c->integ, kinds_mask(c->h_elems, lens_e + 1, c->n_elems));
     if (!kl) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
     // phase 1: walk to the lens and through its entrance; each wavefront queues the molecules that
     // enter the lens into its own region (no atomics)
     const int blk = 256;
     const size_t lds_walk = R->aoi_mask ? (size_t)(blk / 64) * AOI_WAVE * AOI_ENTRY_BYTES : 0;
-    PropFn kpre = k_propagate<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_NONE, PHASE_PRE>;
+    PropFn kpre = kinds_mask(c->h_elems, 0, lens_e) == KM_CIRC
+                      ? k_propagate<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_NONE, PHASE_PRE, KM_CIRC>
+                      : k_propagate<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_NONE, PHASE_PRE, KM_ALL>;
     int rc = lds_limit(c, (const void *)kpre, lds_walk);
     if (rc) return rc;
     int gpre = grid_for(c, (const void *)kpre, blk, lds_walk, R->n, R->grid);
@@ -1436,7 +1449,7 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     const int grid = grid_for(c, (const void *)k, block, lds, R->n, R->grid);
     hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, (const ctraj_run *)c->d_run, LA);
   } else {
-    PropFn k = prop_kernel(c->force, c->integ, n_lens > 0);
+    PropFn k = prop_kernel(c->force, c->integ, n_lens > 0, kinds_mask(c->h_elems, 0, c->n_elems));
     if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
     if (n_lens == 0) kt.ws_states = kt.rt_total = 0;
     size_t lds_tab = n_lens > 0 ? lds : 0;

@@ -403,7 +403,11 @@ __device__ __forceinline__ bool lens_rk4(const ctraj_elem &E, const Tables &T, i
   return alive;
 }
 
-template <int FORCE, int INTEG, class Sink>
+// Element kinds a walker instantiation handles (KM bit mask); the host picks the narrowest mask
+// that covers the beamline, which keeps unused code (and its registers) out of the kernel.
+constexpr int KM_CIRC = 1, KM_RECT = 2, KM_PLATES = 4, KM_ALL = 7;
+
+template <int FORCE, int INTEG, class Sink, int KM = KM_ALL>
 __device__ __forceinline__ void walk(const ctraj_elem *__restrict__ elems, int n_elems, const Tables &T, double g,
                                      int cap, int state, int label_detected, MState &m, int &label, uint32_t &status,
                                      WalkCounters &wc, Sink &sink) {
@@ -422,7 +426,7 @@ __device__ __forceinline__ void walk(const ctraj_elem *__restrict__ elems, int n
       ++wc.faces;
       if constexpr (Sink::kRows) sink.row(m);
       if (rho_of(m.x, m.y) > half_d) { label = E.label_hit; return; }
-    } else if (E.kind == CTRAJ_RECT) {
+    } else if ((KM & KM_RECT) && E.kind == CTRAJ_RECT) {
       // RectangularAperture.propagate_through (apertures.py:165-189): strict box test
       const double x1 = E.p[0], x2 = E.p[1], y1 = E.p[2], y2 = E.p[3];
       drift_dt(m, time_to(m, E.z0), g);
@@ -433,7 +437,7 @@ __device__ __forceinline__ void walk(const ctraj_elem *__restrict__ elems, int n
       ++wc.faces;
       if constexpr (Sink::kRows) sink.row(m);
       if (!((x1 < m.x && m.x < x2) && (y1 < m.y && m.y < y2))) { label = E.label_hit; return; }
-    } else if (E.kind == CTRAJ_PLATES) {
+    } else if ((KM & KM_PLATES) && E.kind == CTRAJ_PLATES) {
       // FieldPlates.propagate_through (apertures.py:227-270): x-bounded plates with wall-hit time
       const double x1 = E.p[0], x2 = E.p[1];
       drift_dt(m, time_to(m, E.z0), g);

@@ -36,6 +36,7 @@ struct LaneMol {
 
 // Walk elements [e0, e1) (no lens among them) from the molecule's current row.  Returns false if
 // the molecule stopped (label set).
+template <int KM = KM_ALL>
 __device__ __forceinline__ bool walk_plain(const ctraj_elem *__restrict__ elems, int e0, int e1, double g, MState &m,
                                            int &label, WalkCounters &wc) {
   NullSink sk;
@@ -49,7 +50,7 @@ __device__ __forceinline__ bool walk_plain(const ctraj_elem *__restrict__ elems,
       drift_dt(m, time_to(m, E.z1), g);
       ++wc.faces;
       if (rho_of(m.x, m.y) > half_d) { label = E.label_hit; return false; }
-    } else if (E.kind == CTRAJ_RECT) {
+    } else if ((KM & KM_RECT) && E.kind == CTRAJ_RECT) {
       const double x1 = E.p[0], x2 = E.p[1], y1 = E.p[2], y2 = E.p[3];
       drift_dt(m, time_to(m, E.z0), g);
       ++wc.faces;
@@ -57,7 +58,7 @@ __device__ __forceinline__ bool walk_plain(const ctraj_elem *__restrict__ elems,
       drift_dt(m, time_to(m, E.z1), g);
       ++wc.faces;
       if (!((x1 < m.x && m.x < x2) && (y1 < m.y && m.y < y2))) { label = E.label_hit; return false; }
-    } else if (E.kind == CTRAJ_PLATES) {
+    } else if ((KM & KM_PLATES) && E.kind == CTRAJ_PLATES) {
       const double x1 = E.p[0], x2 = E.p[1];
       drift_dt(m, time_to(m, E.z0), g);
       ++wc.faces;
