@@ -105,9 +105,8 @@ __device__ __forceinline__ void load_mol(const ctraj_run &R, const KBeam &B, int
   q.idx = i;
 }
 
-template <int FORCE, int INTEG, int PHASE, int KM>
-__global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R0, int aoi_lds_off, PipeArgs P) {
-  const ctraj_run &R = R0;
+template <int FORCE, int INTEG, int PHASE, int KM, bool OUT>
+__global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R, int aoi_lds_off, PipeArgs P) {
   extern __shared__ double lds[];
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
@@ -186,19 +185,21 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
       qn += __popcll(qm);
     }
     if (active && !enq) {
-      if (R.out.x) R.out.x[i] = m.x;
-      if (R.out.y) R.out.y[i] = m.y;
-      if (R.out.z) R.out.z[i] = m.z;
-      if (R.out.vx) R.out.vx[i] = m.vx;
-      if (R.out.vy) R.out.vy[i] = m.vy;
-      if (R.out.vz) R.out.vz[i] = m.vz;
-      if (R.out.t) R.out.t[i] = m.t;
-      if (R.out_a[0]) R.out_a[0][i] = m.ax;
-      if (R.out_a[1]) R.out_a[1][i] = m.ay;
-      if (R.out_a[2]) R.out_a[2][i] = m.az;
-      if (R.label) R.label[i] = label;
-      if (R.status)
-        R.status[i] = (R.flags & CTRAJ_RUN_STATUS_OR) ? (uint8_t)(R.status[i] | status) : (uint8_t)status;
+      if constexpr (OUT) {  // per-molecule outputs (an instantiation without them keeps fewer pointers live)
+        if (R.out.x) R.out.x[i] = m.x;
+        if (R.out.y) R.out.y[i] = m.y;
+        if (R.out.z) R.out.z[i] = m.z;
+        if (R.out.vx) R.out.vx[i] = m.vx;
+        if (R.out.vy) R.out.vy[i] = m.vy;
+        if (R.out.vz) R.out.vz[i] = m.vz;
+        if (R.out.t) R.out.t[i] = m.t;
+        if (R.out_a[0]) R.out_a[0][i] = m.ax;
+        if (R.out_a[1]) R.out_a[1][i] = m.ay;
+        if (R.out_a[2]) R.out_a[2][i] = m.az;
+        if (R.label) R.label[i] = label;
+        if (R.status)
+          R.status[i] = (R.flags & CTRAJ_RUN_STATUS_OR) ? (uint8_t)(R.status[i] | status) : (uint8_t)status;
+      }
       n_oot += (status & CTRAJ_ST_OOT) ? 1u : 0u;
       n_cap += (status & CTRAJ_ST_CAPPED) ? 1u : 0u;
       wave_hist(s_hist, R.first_index ? s_first : nullptr, st * B.n_labels + label, label, R.index_base + i);
@@ -1075,12 +1076,24 @@ static int kinds_mask(const std::vector<ctraj_elem> &el, int e0, int e1) {
   return KM_CIRC;
 }
 
-static PropFn prop_kernel(int f, int i, bool has_lens, int km) {
-  if (!has_lens)
-    return km == KM_CIRC ? k_propagate<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_NONE, PHASE_FULL, KM_CIRC>
-                         : k_propagate<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_NONE, PHASE_FULL, KM_ALL>;
+// true if the run asks for any per-molecule output array
+static bool wants_outputs(const ctraj_run *R) {
+  return R->out.x || R->out.y || R->out.z || R->out.vx || R->out.vy || R->out.vz || R->out.t || R->out_a[0] ||
+         R->out_a[1] || R->out_a[2] || R->label || R->status;
+}
+
+template <int PHASE>
+static PropFn walker_kernel(int km, bool out) {
+  constexpr int F = CTRAJ_FORCE_WSPLINE_HEX, I = CTRAJ_INTEG_NONE;
+  if (km == KM_CIRC)
+    return out ? k_propagate<F, I, PHASE, KM_CIRC, true> : k_propagate<F, I, PHASE, KM_CIRC, false>;
+  return out ? k_propagate<F, I, PHASE, KM_ALL, true> : k_propagate<F, I, PHASE, KM_ALL, false>;
+}
+
+static PropFn prop_kernel(int f, int i, bool has_lens, int km, bool out) {
+  if (!has_lens) return walker_kernel<PHASE_FULL>(km, out);
 #define X(F, I) \
-  if (f == F && i == I) return k_propagate<F, I, PHASE_FULL, KM_ALL>;
+  if (f == F && i == I) return k_propagate<F, I, PHASE_FULL, KM_ALL, true>;
   CT_FI_LIST(X)
 #undef X
   return nullptr;
@@ -1380,9 +1393,7 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     // enter the lens into its own region (no atomics)
     const int blk = 256;
     const size_t lds_walk = R->aoi_mask ? (size_t)(blk / 64) * AOI_WAVE * AOI_ENTRY_BYTES : 0;
-    PropFn kpre = kinds_mask(c->h_elems, 0, lens_e) == KM_CIRC
-                      ? k_propagate<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_NONE, PHASE_PRE, KM_CIRC>
-                      : k_propagate<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_NONE, PHASE_PRE, KM_ALL>;
+    PropFn kpre = walker_kernel<PHASE_PRE>(kinds_mask(c->h_elems, 0, lens_e), wants_outputs(R));
     int rc = lds_limit(c, (const void *)kpre, lds_walk);
     if (rc) return rc;
     int gpre = grid_for(c, (const void *)kpre, blk, lds_walk, R->n, R->grid);
@@ -1449,7 +1460,7 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     const int grid = grid_for(c, (const void *)k, block, lds, R->n, R->grid);
     hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, (const ctraj_run *)c->d_run, LA);
   } else {
-    PropFn k = prop_kernel(c->force, c->integ, n_lens > 0, kinds_mask(c->h_elems, 0, c->n_elems));
+    PropFn k = prop_kernel(c->force, c->integ, n_lens > 0, kinds_mask(c->h_elems, 0, c->n_elems), wants_outputs(R));
     if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
     if (n_lens == 0) kt.ws_states = kt.rt_total = 0;
     size_t lds_tab = n_lens > 0 ? lds : 0;
