@@ -1375,8 +1375,9 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
       ++n_lens;
       lens_e = e;
     }
-  // queue positions are 32-bit: larger single runs use the single-kernel walker
-  const bool pipeline = n_lens == 1 && !(R->flags & (CTRAJ_RUN_SIMPLE | CTRAJ_RUN_LANES)) && R->n < ((int64_t)1 << 32);
+  // the lens-entry queue costs 65 B per molecule: single runs above 2^27 molecules (8.7 GB of
+  // queue) use the single-kernel walker, which needs no queue
+  const bool pipeline = n_lens == 1 && !(R->flags & (CTRAJ_RUN_SIMPLE | CTRAJ_RUN_LANES)) && R->n <= ((int64_t)1 << 27);
   const bool lanes = n_lens == 1 && !(R->flags & CTRAJ_RUN_SIMPLE) && (R->flags & CTRAJ_RUN_LANES);
   // The lane walker keeps its tables once per workgroup: with several lens-state tables (more
   // than ~40 KB) one 768-thread workgroup per CU holds as many wavefronts as its registers allow,

@@ -132,15 +132,15 @@ int32_t ctraj_set_wspline(ctraj_ctx *ctx, int32_t n_states, int32_t n_seg, const
 
 /* ---- forward propagation: replaces run_simulation's per-molecule loop (trajectory_simulator.py:52-78)
  *      and Beamline.propagate_through (beamline.py:20-38).  A beamline with exactly one lens runs in
- *      three phases queued on the stream: a streaming walk to and through the lens entrance that
- *      queues the molecules entering the lens, a persistent lane-refill lens kernel that queues the
- *      molecules leaving it, and a streaming walk of the remaining elements (context-owned queues of
- *      131 bytes per molecule).  CTRAJ_RUN_LANES runs the single-kernel lane-refill walker instead;
+ *      two phases queued on the stream: a streaming walk to and through the lens entrance that queues
+ *      the molecules entering the lens (a context-owned queue of 65 bytes per molecule), then a
+ *      persistent lane-refill kernel that integrates the lens and walks the elements after it.  Runs
+ *      of more than 2^27 molecules, or CTRAJ_RUN_LANES, use the single-kernel lane-refill walker;
  *      other beamlines, or CTRAJ_RUN_SIMPLE, run one molecule per thread.  All give bit-identical
  *      results. ---- */
 #define CTRAJ_RUN_STATUS_OR 1 /* OR status bits into run->status instead of overwriting */
 #define CTRAJ_RUN_SIMPLE 2    /* one molecule per thread start to finish (no lane refill) */
-#define CTRAJ_RUN_LANES 4     /* one lens: the single-kernel lane-refill walker instead of the three phases */
+#define CTRAJ_RUN_LANES 4     /* one lens: the single-kernel lane-refill walker instead of the two phases */
 typedef struct ctraj_run {
   int64_t n;             /* molecules */
   int64_t index_base;    /* global index of molecule 0 (for first_index / aoi_index in chunked runs) */
