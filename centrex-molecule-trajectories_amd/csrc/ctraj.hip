@@ -559,6 +559,8 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
   const ctraj_elem &L0 = B.elems[P.lens_e];
   const Force<FORCE> F = make_force<FORCE>(L0, B.g);
   const double R2 = L0.p[1], z1 = L0.z1;
+  // config-mode step size (uniform) and its derived constants, kept in scalar registers
+  const double Hs = L0.p[2], Hh = 0.5 * Hs, H6 = Hs * kSixth;
   const int label_inside = L0.label_inside;
   const int64_t nq = (int64_t)s_woff[nw];
   WalkCounters wc{0u, 0u};
@@ -597,6 +599,15 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
         if (__ballot(ph == LN_FIN)) {
           const ctraj_run &R = *opaque(Rg);
           if (ph == LN_FIN) {
+            if constexpr (INTEG == CTRAJ_INTEG_RK4) {
+              // a molecule stopped by the bore stores the force at its last position; only the
+              // acceleration outputs see it, so it is evaluated only when they are requested
+              if ((R.out_a[0] || R.out_a[1]) && label == label_inside) {
+                uint32_t st2 = 0;
+                F(T, wsb, m.x, m.y, m.ax, m.ay, st2);
+                m.az = 0.0;
+              }
+            }
             if (R.out.x) R.out.x[idx] = m.x;
             if (R.out.y) R.out.y[idx] = m.y;
             if (R.out.z) R.out.z[idx] = m.z;
@@ -684,11 +695,10 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
             i = 0;
             const ctraj_elem &L = *opaque(B.elems + P.lens_e);
             if constexpr (INTEG == CTRAJ_INTEG_RK4) {
-              const double Hs = L.p[2], Ll = L.p[3];
+              const double Ll = L.p[3];
               nfull = rk4_full_steps(Ll, m.vz, Hs, B.cap, status);
               dzs = m.vz * Hs;
               hp = Ll / m.vz - (double)nfull * Hs;
-              h = nfull == 0 ? hp : Hs;
               r2 = __builtin_fma(m.x, m.x, m.y * m.y);
               ph = LN_LENS;
             } else {
@@ -710,20 +720,20 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
     if (ph == LN_LENS) {
       if constexpr (INTEG == CTRAJ_INTEG_RK4) {
         const bool last = (i == nfull);
-        rk4_step<FORCE>(F, T, wsb, h, m.x, m.y, m.vx, m.vy, r2, status);
-        m.z = last ? z1 : m.z + dzs;
-        m.t = m.t + h;
+        if (!last) {  // full steps: the uniform step size
+          rk4_step_hh<FORCE>(F, T, wsb, Hs, Hh, H6, m.x, m.y, m.vx, m.vy, r2, status);
+          m.z = m.z + dzs;
+          m.t = m.t + Hs;
+        } else {  // the final partial step, once per molecule that reaches the lens end
+          rk4_step<FORCE>(F, T, wsb, hp, m.x, m.y, m.vx, m.vy, r2, status);
+          m.z = z1;
+          m.t = m.t + hp;
+        }
         r2 = __builtin_fma(m.x, m.x, m.y * m.y);
         ++i;
-        h = (i == nfull) ? hp : h;
         const bool dead = r2 > R2;
         label = dead ? label_inside : label;
         ph = dead ? (int)LN_FIN : (last ? (int)LN_EXIT : (int)LN_LENS);
-        if (dead) {
-          uint32_t st2 = 0;
-          F(T, wsb, m.x, m.y, m.ax, m.ay, st2);  // the stopping row stores the force there
-          m.az = 0.0;
-        }
       } else {
         ref_step<FORCE>(F, T, wsb, h, m, status);  // the row stores a = l1
         ++i;

@@ -125,8 +125,18 @@ __device__ __forceinline__ const lds_double *ws_of(const Tables &T, int state) {
 // W'(u) from a state's LDS spline; u = |E|/dE >= 0.  u is clamped to umax, the largest double below
 // nseg, so k = trunc(u) <= nseg - 1 and s = u - k = fract(u) exactly; the force saturates beyond the
 // table end.
+// min(u, umax) as one v_min_f64.  __builtin_fmin would also canonicalise umax on every call (a
+// second FP64 instruction per look-up) because the compiler cannot see that the table bound, a
+// normal double, needs none; u is an arithmetic result and umax a normal number, so the IEEE-mode
+// minimum is fmin's result, including umax for a NaN u.
+__device__ __forceinline__ double clamp_to(double u, double umax) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(u), "v"(umax));
+  return r;
+}
+
 __device__ __forceinline__ double wspline_eval(const lds_double *wsb, double umax, double u) {
-  u = __builtin_fmin(u, umax);
+  u = clamp_to(u, umax);
   const unsigned k = (unsigned)(int)u;
   const double s = __builtin_amdgcn_fract(u);
   // 32-bit LDS byte address k*24 + base (segments are far below 2^24)
@@ -290,10 +300,12 @@ constexpr double kSixth = 1.0 / 6.0;
 //   S_x = ((v1 + 2 v2) + 2 v3) + v4,   S_v = ((a1 + 2 a2) + 2 a3) + a4
 // as soon as each stage is known, which keeps few FP64 values live.  Every FMA here is
 // mirrored by the CPU oracle.
+// rk4_step_hh takes hh and h6 precomputed (a kernel whose step size is uniform keeps them in
+// scalar registers); rk4_step forms them from h.
 template <int FORCE>
-__device__ __forceinline__ void rk4_step(const Force<FORCE> &F, const Tables &T, const lds_double *wsb, double h,
-                                         double &x, double &y, double &vx, double &vy, double r2, uint32_t &status) {
-  const double hh = 0.5 * h, h6 = h * kSixth;
+__device__ __forceinline__ void rk4_step_hh(const Force<FORCE> &F, const Tables &T, const lds_double *wsb, double h,
+                                            double hh, double h6, double &x, double &y, double &vx, double &vy,
+                                            double r2, uint32_t &status) {
   double ax, ay;
   if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX) {
     force_hex_r2(wsb, T.umax, F.ks, F.cu, F.g, x, y, r2, ax, ay);
@@ -332,6 +344,12 @@ __device__ __forceinline__ void rk4_step(const Force<FORCE> &F, const Tables &T,
   y = __builtin_fma(h6, sy, y);
   vx = __builtin_fma(h6, svx, vx);
   vy = __builtin_fma(h6, svy, vy);
+}
+
+template <int FORCE>
+__device__ __forceinline__ void rk4_step(const Force<FORCE> &F, const Tables &T, const lds_double *wsb, double h,
+                                         double &x, double &y, double &vx, double &vy, double r2, uint32_t &status) {
+  rk4_step_hh<FORCE>(F, T, wsb, h, 0.5 * h, h * kSixth, x, y, vx, vy, r2, status);
 }
 
 // Number of full RK4 steps in a lens of length L at axial speed vz: floor(L / (vz dt)), 0 for a
