@@ -380,7 +380,7 @@ def main():
                      "frac": achieved / PEAK_FP64, "traffic": traffic,
                      "kernel": ("k_propagate<WSPLINE_HEX,RK4>" if args.simple or args.config != "lens"
                                 else "k_propagate_lanes<WSPLINE_HEX,RK4>" if args.lanes
-                                else "k_propagate<PRE> + k_scan_u32 + k_lens<WSPLINE_HEX,RK4> (one ctraj_propagate)"),
+                                else "k_propagate<PRE> + k_lens<WSPLINE_HEX,RK4> (one ctraj_propagate)"),
                      "kernel_ms_avg": kavg,
                      "flop_per_launch": flop, "faces_per_launch": faces, "rk4_steps_per_launch": steps_},
         "region_ms_per_step": region_ms / K,

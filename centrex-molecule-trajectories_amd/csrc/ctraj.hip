@@ -73,19 +73,37 @@ struct InMol {
 // is recorded (outputs, tallies, kept-molecule compaction) by the phase in which it finishes.
 enum : int { PHASE_FULL = 0, PHASE_PRE = 1 };
 
-struct PipeQ {
-  double *v[7];                // x, y, z, vx, vy, vz, t
-  int64_t *idx;                // molecule index within the run
-  uint8_t *st;                 // lens state
-};
+// A lens-entry queue entry is 64 bytes, four 16-byte vectors: (x, y) (z, vx) (vy, vz) (t, key) with
+// key = molecule index within the run | lens state << 56 (the pipeline runs at most 2^27
+// molecules).  Each field is kept as its bit pattern.
+__device__ __forceinline__ void queue_put(longlong2 *q, int64_t e, const MState &m, int64_t i, int st) {
+  longlong2 *p = q + 4 * e;
+  p[0] = make_longlong2(__double_as_longlong(m.x), __double_as_longlong(m.y));
+  p[1] = make_longlong2(__double_as_longlong(m.z), __double_as_longlong(m.vx));
+  p[2] = make_longlong2(__double_as_longlong(m.vy), __double_as_longlong(m.vz));
+  p[3] = make_longlong2(__double_as_longlong(m.t), (long long)(i | ((int64_t)st << 56)));
+}
+
+__device__ __forceinline__ void queue_get(const longlong2 *q, int64_t e, MState &m, int64_t &i, int &st) {
+  const longlong2 *p = q + 4 * e;
+  const longlong2 a = p[0], b = p[1], c = p[2], d = p[3];
+  m.x = __longlong_as_double(a.x);
+  m.y = __longlong_as_double(a.y);
+  m.z = __longlong_as_double(b.x);
+  m.vx = __longlong_as_double(b.y);
+  m.vy = __longlong_as_double(c.x);
+  m.vz = __longlong_as_double(c.y);
+  m.t = __longlong_as_double(d.x);
+  i = (int64_t)d.y & (((int64_t)1 << 56) - 1);
+  st = (int)((uint64_t)d.y >> 56);
+}
 
 struct PipeArgs {
-  PipeQ q;                     // lens-entry queue
-  uint32_t *wcount;            // PRE: entries queued by each wavefront
-  uint32_t *woff;              // exclusive prefix of wcount: [n_pre_waves + 1]
-  int64_t region;              // queue entries owned by each PRE wavefront
+  longlong2 *q;                // lens-entry queue, 4 vectors per entry
+  uint32_t *wcount;            // PRE: entries queued by each wavefront (a prefix of its region)
+  int64_t region;              // queue entries owned by each PRE wavefront, a multiple of 64
   int n_pre_waves;
-  unsigned long long *claim;   // lens kernel: next unclaimed queue position
+  unsigned long long *claim;   // lens kernel: next unclaimed 64-entry chunk of the queue
   int lens_e;
 };
 
@@ -171,16 +189,7 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
       // wavefront-private region of the queue: no atomics
       const uint64_t qm = __ballot(enq);
       if (enq) {
-        const int64_t jq = gwave * P.region + qn + lanes_below(qm);
-        P.q.v[0][jq] = m.x;
-        P.q.v[1][jq] = m.y;
-        P.q.v[2][jq] = m.z;
-        P.q.v[3][jq] = m.vx;
-        P.q.v[4][jq] = m.vy;
-        P.q.v[5][jq] = m.vz;
-        P.q.v[6][jq] = m.t;
-        P.q.idx[jq] = i;
-        P.q.st[jq] = (uint8_t)st;
+        queue_put(P.q, gwave * P.region + qn + lanes_below(qm), m, i, st);
       }
       qn += __popcll(qm);
     }
@@ -500,47 +509,25 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
     atomicAdd((unsigned long long *)&R.work[threadIdx.x], s_work[threadIdx.x]);
 }
 
-// Exclusive prefix of the PRE wavefronts' queue counts: woff[w] = sum of wcount[0..w), woff[n] = total.
-// One workgroup of 1024 threads; fixed order.
-__global__ void __launch_bounds__(1024) k_scan_u32(const uint32_t *wcount, int n, uint32_t *woff) {
-  __shared__ uint32_t part[1024];
-  const int per = (n + 1023) / 1024;
-  const int b = threadIdx.x * per, e = min(b + per, n);
-  uint32_t sum = 0;
-  for (int k = b; k < e; ++k) sum += wcount[k];
-  part[threadIdx.x] = sum;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (int k = 0; k < 1024; ++k) {
-      const uint32_t v = part[k];
-      part[k] = run;
-      run += v;
-    }
-    woff[n] = run;
-  }
-  __syncthreads();
-  uint32_t run = part[threadIdx.x];
-  for (int k = b; k < e; ++k) {
-    woff[k] = run;
-    run += wcount[k];
-  }
-}
-
 // ---------------------------------------------------------------------------------------------
 // k_lens: the lens and what follows it.  Persistent wavefronts integrate the lens-entry queue
 // with lane refill: every iteration advances all lanes inside the lens by one step; when at least
 // `refill_min` lanes are idle (or none is in the lens), a convergent section walks the molecules
 // that left the lens through the remaining elements, records every finished molecule (outputs,
-// tallies, kept-molecule compaction) and gives idle lanes the next queued molecules.  Queue
-// position g lives in the region of the PRE wavefront w with woff[w] <= g < woff[w + 1] (binary
-// search of woff, staged in LDS).  The per-molecule arithmetic is the one of walk().
+// tallies, kept-molecule compaction) and gives idle lanes the next queued molecules.  Wavefronts
+// claim the queue in 64-entry chunks, region by region: chunk c is entries [64 (c % cpr), +64) of
+// the region of PRE wavefront c / cpr (cpr = region / 64), of which the first wcount are filled, so
+// a claim needs no search and a chunk past a region's count is simply claimed over.  The
+// per-molecule arithmetic is the one of walk().
 // ---------------------------------------------------------------------------------------------
 enum : int { LN_IDLE = 0, LN_LENS = 1, LN_EXIT = 2, LN_FIN = 3, LN_DONE = 4 };
 
+// Five wavefronts per SIMD (at most 96 VGPRs) for the hexapole RK4 circular-aperture-only
+// instantiation, which fits them without spilling; the others need more registers than that.
 template <int FORCE, int INTEG, int KM>
-__global__ void __launch_bounds__(768)
-k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refill_min, int chunk, int woff_lds) {
+__global__ void __launch_bounds__(768) __attribute__((amdgpu_waves_per_eu(
+    FORCE == CTRAJ_FORCE_WSPLINE_HEX && INTEG == CTRAJ_INTEG_RK4 && KM == KM_CIRC ? 5 : 1)))
+k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refill_min, double Hh, double H6) {
   extern __shared__ double lds[];
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
@@ -550,19 +537,18 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
   for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x) s_first[b] = ~0ull >> 1;
   if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
   const Tables T = stage_tables(lds, KT);
-  uint32_t *s_woff = reinterpret_cast<uint32_t *>(lds + woff_lds);
-  const int nw = P.n_pre_waves;
-  for (int k = threadIdx.x; k <= nw; k += blockDim.x) s_woff[k] = P.woff[k];
   __syncthreads();
+  const uint32_t cpr = (uint32_t)(P.region >> 6);
+  const unsigned long long n_chunks = (unsigned long long)P.n_pre_waves * cpr;
 
   const int lane = lane_id();
   const ctraj_elem &L0 = B.elems[P.lens_e];
   const Force<FORCE> F = make_force<FORCE>(L0, B.g);
   const double R2 = L0.p[1], z1 = L0.z1;
-  // config-mode step size (uniform) and its derived constants, kept in scalar registers
-  const double Hs = L0.p[2], Hh = 0.5 * Hs, H6 = Hs * kSixth;
+  // config-mode step size (uniform); Hh = 0.5 Hs and H6 = Hs (1/6) come formed by the host, so all
+  // three stay in scalar registers
+  const double Hs = L0.p[2];
   const int label_inside = L0.label_inside;
-  const int64_t nq = (int64_t)s_woff[nw];
   WalkCounters wc{0u, 0u};
   uint32_t n_oot = 0, n_cap = 0;
   unsigned long long steps_w = 0;
@@ -657,39 +643,27 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
         }
         if (next >= end) {
           unsigned long long c = 0;
-          if (lane == 0) c = atomicAdd(P.claim, (unsigned long long)chunk);
+          if (lane == 0) c = atomicAdd(P.claim, 1ull);
           c = __shfl(c, 0);
-          next = (int64_t)c;
-          end = next + chunk < nq ? next + chunk : nq;
-          if (next >= end) {
+          if (c >= n_chunks) {
             exhausted = true;
             continue;
           }
+          const uint32_t w = (uint32_t)c / cpr;
+          const int64_t base = (int64_t)((uint32_t)c - w * cpr) * 64;
+          const int64_t filled = (int64_t)P.wcount[w] - base;
+          next = (int64_t)w * P.region + base;
+          end = next + (filled < 64 ? filled : 64);  // next >= end: an empty chunk, claim again
+          continue;
         }
         const int64_t avail = end - next;
         if (ph == LN_IDLE) {
           const int rank = lanes_below(need);
           if (rank < avail) {
-            const uint32_t g = (uint32_t)(next + rank);
-            int lo = 0, hi = nw;  // the PRE wavefront whose region holds queue position g
-            while (hi - lo > 1) {
-              const int mid = (lo + hi) >> 1;
-              if (s_woff[mid] <= g) lo = mid;
-              else hi = mid;
-            }
-            const int64_t j = (int64_t)lo * P.region + (int64_t)(g - s_woff[lo]);
-            m.x = P.q.v[0][j];
-            m.y = P.q.v[1][j];
-            m.z = P.q.v[2][j];
-            m.vx = P.q.v[3][j];
-            m.vy = P.q.v[4][j];
-            m.vz = P.q.v[5][j];
-            m.t = P.q.v[6][j];
+            queue_get(P.q, next + rank, m, idx, st);
             m.ax = 0.0;
             m.ay = -B.g;
             m.az = 0.0;
-            idx = P.q.idx[j];
-            st = P.q.st[j];
             wsb = ws_of(T, st);
             status = 0;
             i = 0;
@@ -1075,7 +1049,7 @@ int ensure_buf(ctraj_ctx *ctx, ctraj_devbuf &b, size_t bytes) {
   X(CTRAJ_FORCE_WSPLINE_LIN, CTRAJ_INTEG_RK4)
 
 using PropFn = void (*)(KBeam, KTab, ctraj_run, int, PipeArgs);
-using LensFn = void (*)(KBeam, KTab, const ctraj_run *, PipeArgs, int, int, int);
+using LensFn = void (*)(KBeam, KTab, const ctraj_run *, PipeArgs, int, double, double);
 using LaneFn = void (*)(KBeam, KTab, const ctraj_run *, LaneArgs);
 using RecFn = void (*)(KBeam, KTab, ctraj_record);
 
@@ -1385,7 +1359,7 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
       ++n_lens;
       lens_e = e;
     }
-  // the lens-entry queue costs 65 B per molecule: single runs above 2^27 molecules (8.7 GB of
+  // the lens-entry queue costs 64 B per molecule: single runs above 2^27 molecules (8.6 GB of
   // queue) use the single-kernel walker, which needs no queue
   const bool pipeline = n_lens == 1 && !(R->flags & (CTRAJ_RUN_SIMPLE | CTRAJ_RUN_LANES)) && R->n <= ((int64_t)1 << 27);
   const bool lanes = n_lens == 1 && !(R->flags & CTRAJ_RUN_SIMPLE) && (R->flags & CTRAJ_RUN_LANES);
@@ -1407,34 +1381,28 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     PropFn kpre = walker_kernel<PHASE_PRE>(kinds_mask(c->h_elems, 0, lens_e), wants_outputs(R));
     int rc = lds_limit(c, (const void *)kpre, lds_walk);
     if (rc) return rc;
-    int gpre = grid_for(c, (const void *)kpre, blk, lds_walk, R->n, R->grid);
-    gpre = std::min(gpre, 1024 / (blk / 64));  // at most 1024 queue regions (their offsets sit in LDS)
+    const int gpre = grid_for(c, (const void *)kpre, blk, lds_walk, R->n, R->grid);
     const int nw = gpre * (blk / 64);
     const int64_t threads = (int64_t)gpre * blk;
     const int64_t region = (R->n + threads - 1) / threads * 64;
     const size_t cap = (size_t)nw * (size_t)region;
-    const size_t bytes = cap * (7 * sizeof(double) + sizeof(int64_t) + 1) + 256;
+    const size_t bytes = cap * 64 + 256;
     if ((rc = ensure_buf(c, c->pipe, bytes)) != CTRAJ_OK) return rc;
-    if ((rc = ensure_buf(c, c->pipe_cnt, 64 + (size_t)(2 * nw + 2) * sizeof(uint32_t))) != CTRAJ_OK) return rc;
+    if ((rc = ensure_buf(c, c->pipe_cnt, 64 + (size_t)nw * sizeof(uint32_t))) != CTRAJ_OK) return rc;
     PipeArgs P{};
     char *b = (char *)c->pipe.p;
-    for (int k = 0; k < 7; ++k) P.q.v[k] = (double *)(b + (size_t)k * cap * 8);
-    P.q.idx = (int64_t *)(b + 7 * cap * 8);
-    P.q.st = (uint8_t *)(b + 8 * cap * 8);
+    P.q = (longlong2 *)b;
     char *cb = (char *)c->pipe_cnt.p;
     P.claim = (unsigned long long *)cb;
     P.wcount = (uint32_t *)(cb + 64);
-    P.woff = P.wcount + nw;
     P.region = region;
     P.n_pre_waves = nw;
     P.lens_e = lens_e;
     if ((rc = ctraj_fill_i64(c, (int64_t *)P.claim, 0, 1)) != CTRAJ_OK) return rc;
     hipLaunchKernelGGL(kpre, dim3(gpre), dim3(blk), lds_walk, c->stream, B, kt, *R, 0, P);
     HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, c->stream, (const uint32_t *)P.wcount, nw, P.woff);
-    HIPCHK(c, hipGetLastError());
     // phase 2: the lens and the elements after it
-    const size_t lds_lens = ((lds + 7) & ~(size_t)7) + (size_t)(nw + 1) * sizeof(uint32_t);
+    const size_t lds_lens = lds;
     if ((rc = lds_limit(c, (const void *)kl, lds_lens)) != CTRAJ_OK) return rc;
     if ((rc = block_ok(c, (const void *)kl, block)) != CTRAJ_OK) return rc;
     if (!c->d_run) HIPCHK(c, hipMalloc(&c->d_run, sizeof(ctraj_run)));
@@ -1445,8 +1413,11 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     }
     const int refill = R->refill > 0 ? std::min(R->refill, 64) : 16;
     const int grid = grid_for(c, (const void *)kl, block, lds_lens, (int64_t)1 << 40, 0);
+    // the RK4 half and sixth steps, rounded as rk4_step forms them
+    const double Hs = c->h_elems[lens_e].p[2];
+    const double Hh = 0.5 * Hs, H6 = Hs * (1.0 / 6.0);
     hipLaunchKernelGGL(kl, dim3(grid), dim3(block), lds_lens, c->stream, B, kt, (const ctraj_run *)c->d_run, P, refill,
-                       64, (int)((lds + 7) / 8));
+                       Hh, H6);
   } else if (lanes) {
     LaneFn k = lane_kernel(c->force, c->integ);
     if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
