@@ -80,14 +80,14 @@ __device__ __forceinline__ double drift_g_adj(double vx, double vy_after, double
 //   f  = ks W'(u), u = min(cu r2, umax)       (a = f (x, y) - g y_hat; identical to force_hex_r2)
 //   d  = 2 ks cu dW'/du  (= 2 df/dr2; 0 where u is clamped)
 //   gv = df/dV = (f + ks cu dW'/du r2) / V    (ks and cu are both proportional to V)
-__device__ __forceinline__ void hex_eval_d(const lds_double *wsb, double umax, double ks, double cu, double kc2,
+__device__ __forceinline__ void hex_eval_d(const WsRef &wsb, double umax, double ks, double cu, double kc2,
                                            double invV, double r2, double &f, double &d, double &gv) {
   const double u0 = r2 * cu;
   const double u = __builtin_fmin(u0, umax);
   const unsigned k = (unsigned)(int)u;
   const double s = __builtin_amdgcn_fract(u);
-  const lds_double *c = (const lds_double *)(size_t)(__umul24(k, 24u) + (unsigned)(size_t)wsb);
-  const double A = c[0], Bc = c[1], C = c[2];
+  const dbl2_t ab = wsb.ab[k];
+  const double A = ab.x, Bc = ab.y, C = wsb.c[k];
   const double P = __builtin_fma(__builtin_fma(A, s, Bc), s, C);
   double dP = __builtin_fma(2.0 * A, s, Bc);
   if (u0 > umax) dP = 0.0;
@@ -97,7 +97,7 @@ __device__ __forceinline__ void hex_eval_d(const lds_double *wsb, double umax, d
 }
 
 struct LensK {
-  const lds_double *wsb;
+  WsRef wsb;
   double umax, ks, cu, kc2, invV, g;
 };
 
@@ -220,10 +220,10 @@ __device__ __forceinline__ int n_params_of(const ctraj_elem &E) { return E.kind 
 
 template <bool GRAD>
 __global__ void __launch_bounds__(512) k_smooth(KBeam B, KTab KT, SmoothArgs A) {
-  extern __shared__ double lds[];
+  extern __shared__ __attribute__((aligned(16))) double lds[];
   const Tables T = stage_tables(lds, KT);
   __syncthreads();
-  const lds_double *wsb = ws_of(T, 0);
+  const WsRef wsb = ws_of(T, 0);
   const int lane = lane_id();
   const int64_t wpb = blockDim.x >> 6;
   const int64_t gwave = (int64_t)blockIdx.x * wpb + (threadIdx.x >> 6);

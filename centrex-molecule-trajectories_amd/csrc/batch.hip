@@ -64,7 +64,7 @@ __device__ __forceinline__ int find_cfg(const int64_t *s_begin, int n_cfg, int64
 }
 
 __global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
-  extern __shared__ double lds[];
+  extern __shared__ __attribute__((aligned(16))) double lds[];
   __shared__ unsigned long long s_work[4];
   const Tables T = stage_tables(lds, KT);
   int64_t *s_begin = reinterpret_cast<int64_t *>(lds + A.off_begin);
@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
   if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
   __syncthreads();
 
-  const lds_double *wsb = ws_of(T, 0);
+  const WsRef wsb = ws_of(T, 0);
   const int lane = lane_id();
   const double g = A.g;
   WalkCounters wc{0u, 0u};

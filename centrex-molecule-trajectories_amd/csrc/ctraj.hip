@@ -125,7 +125,7 @@ __device__ __forceinline__ void load_mol(const ctraj_run &R, const KBeam &B, int
 
 template <int FORCE, int INTEG, int PHASE, int KM, bool OUT>
 __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R, int aoi_lds_off, PipeArgs P) {
-  extern __shared__ double lds[];
+  extern __shared__ __attribute__((aligned(16))) double lds[];
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_work[4];
@@ -331,7 +331,7 @@ __device__ __forceinline__ void record_finished(const KBeam &B, const ctraj_run 
 template <int FORCE, int INTEG>
 __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg,
                                                          LaneArgs LA) {
-  extern __shared__ double lds[];
+  extern __shared__ __attribute__((aligned(16))) double lds[];
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_work[4];
@@ -528,7 +528,7 @@ template <int FORCE, int INTEG, int KM>
 __global__ void __launch_bounds__(768) __attribute__((amdgpu_waves_per_eu(
     FORCE == CTRAJ_FORCE_WSPLINE_HEX && INTEG == CTRAJ_INTEG_RK4 && KM == KM_CIRC ? 5 : 1)))
 k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refill_min, double Hh, double H6) {
-  extern __shared__ double lds[];
+  extern __shared__ __attribute__((aligned(16))) double lds[];
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_work[4];
@@ -561,7 +561,7 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
   int i = 0, nfull = 0, st = 0, ph = LN_IDLE, label = 0;
   uint32_t status = 0;
   int64_t idx = 0;
-  const lds_double *wsb = ws_of(T, 0);
+  WsRef wsb = ws_of(T, 0);
 
   for (;;) {
     const uint64_t lensm = __ballot(ph == LN_LENS);
@@ -744,7 +744,7 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
 // ---------------------------------------------------------------------------------------------
 template <int FORCE, int INTEG, int MODE>
 __global__ void __launch_bounds__(256) k_record(KBeam B, KTab KT, ctraj_record RC) {
-  extern __shared__ double lds[];
+  extern __shared__ __attribute__((aligned(16))) double lds[];
   const Tables T = stage_tables(lds, KT);
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1411,7 +1411,7 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
       c->last_run = *R;
       c->have_last_run = true;
     }
-    const int refill = R->refill > 0 ? std::min(R->refill, 64) : 16;
+    const int refill = R->refill > 0 ? std::min(R->refill, 64) : 12;
     const int grid = grid_for(c, (const void *)kl, block, lds_lens, (int64_t)1 << 40, 0);
     // the RK4 half and sixth steps, rounded as rk4_step forms them
     const double Hs = c->h_elems[lens_e].p[2];

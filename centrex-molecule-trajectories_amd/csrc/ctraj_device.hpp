@@ -27,10 +27,17 @@ struct MState {
 
 // LDS-qualified doubles: arithmetic on these pointers stays 32-bit and loads become ds_read.
 typedef __attribute__((address_space(3))) double lds_double;
+typedef double dbl2_t __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) dbl2_t lds_dbl2;
 
-// Lens tables as seen from one workgroup (staged in LDS).
+// Lens tables as seen from one workgroup (staged in LDS).  The W' spline segment k of a state is
+// W'(k + s) = (A s + B) s + C.  (A, B) pairs and C sit in two arrays so that a look-up is one
+// 16-byte and one 8-byte LDS read at 16- and 8-byte strides: random segments across a wavefront
+// then cost about what conflict-free reads do, where one 24-byte record per segment (two 8-byte
+// reads and one 8-byte read) costs half again as much.
 struct Tables {
-  const lds_double *ws;  // W' spline: [state][nseg][3] = {A, B, C}
+  const lds_dbl2 *ws_ab;   // W' spline (A, B): [state][nseg], 16-byte aligned
+  const lds_double *ws_c;  // W' spline C: [state][nseg]
   int nseg;
   double umax;           // largest double below nseg: |E| beyond the table is clamped there
   const lds_double *rt;  // r tables: [knot][4] = {r, a, slope, 0}, all tables back to back
@@ -119,8 +126,16 @@ __device__ __forceinline__ void force_rtable(const Tables &T, const RDesc &D, do
   ay = fy - g;
 }
 
-// LDS base of one state's W' spline.
-__device__ __forceinline__ const lds_double *ws_of(const Tables &T, int state) { return T.ws + 3 * (state * T.nseg); }
+// One lens state's W' spline in LDS.
+struct WsRef {
+  const lds_dbl2 *ab;
+  const lds_double *c;
+};
+
+__device__ __forceinline__ WsRef ws_of(const Tables &T, int state) {
+  const int o = state * T.nseg;
+  return WsRef{T.ws_ab + o, T.ws_c + o};
+}
 
 // W'(u) from a state's LDS spline; u = |E|/dE >= 0.  u is clamped to umax, the largest double below
 // nseg, so k = trunc(u) <= nseg - 1 and s = u - k = fract(u) exactly; the force saturates beyond the
@@ -135,18 +150,18 @@ __device__ __forceinline__ double clamp_to(double u, double umax) {
   return r;
 }
 
-__device__ __forceinline__ double wspline_eval(const lds_double *wsb, double umax, double u) {
+__device__ __forceinline__ double wspline_eval(const WsRef &w, double umax, double u) {
   u = clamp_to(u, umax);
   const unsigned k = (unsigned)(int)u;
   const double s = __builtin_amdgcn_fract(u);
-  // 32-bit LDS byte address k*24 + base (segments are far below 2^24)
-  const lds_double *c = (const lds_double *)(size_t)(__umul24(k, 24u) + (unsigned)(size_t)wsb);
-  return __builtin_fma(__builtin_fma(c[0], s, c[1]), s, c[2]);
+  const dbl2_t ab = w.ab[k];
+  const double c = w.c[k];
+  return __builtin_fma(__builtin_fma(ab.x, s, ab.y), s, c);
 }
 
 // Hexapole: |E| = cE r^2, a = -W'(|E|) 2 cE (x, y) / m - g y_hat.  r2 = x^2 + y^2 is passed in
 // so that the bore test of the previous step can share it.
-__device__ __forceinline__ void force_hex_r2(const lds_double *wsb, double umax, double ks, double cu, double g, double x,
+__device__ __forceinline__ void force_hex_r2(const WsRef &wsb, double umax, double ks, double cu, double g, double x,
                                              double y, double r2, double &ax, double &ay) {
   const double f = wspline_eval(wsb, umax, r2 * cu) * ks;
   ax = f * x;
@@ -155,7 +170,7 @@ __device__ __forceinline__ void force_hex_r2(const lds_double *wsb, double umax,
 
 // Linear-field model |E| = cE r (the reference's field assumption, electrostatic_lens.py:199-200):
 // a = -W'(|E|) cE (x, y) / (m r) - g y_hat, and (0, -g) on axis.
-__device__ __forceinline__ void force_lin(const lds_double *wsb, double umax, double ks, double cu, double g, double x,
+__device__ __forceinline__ void force_lin(const WsRef &wsb, double umax, double ks, double cu, double g, double x,
                                           double y, double &ax, double &ay) {
   const double r = rho_of(x, y);
   double fx = 0.0, fy = 0.0;
@@ -176,7 +191,7 @@ template <>
 struct Force<CTRAJ_FORCE_RTABLE> {
   RDesc d;
   double g;
-  __device__ __forceinline__ void operator()(const Tables &T, const lds_double * /*wsb*/, double x, double y, double &ax,
+  __device__ __forceinline__ void operator()(const Tables &T, const WsRef & /*wsb*/, double x, double y, double &ax,
                                              double &ay, uint32_t &st) const {
     force_rtable(T, d, g, x, y, ax, ay, st);
   }
@@ -185,7 +200,7 @@ struct Force<CTRAJ_FORCE_RTABLE> {
 template <>
 struct Force<CTRAJ_FORCE_WSPLINE_HEX> {
   double ks, cu, g;
-  __device__ __forceinline__ void operator()(const Tables &T, const lds_double *wsb, double x, double y, double &ax,
+  __device__ __forceinline__ void operator()(const Tables &T, const WsRef &wsb, double x, double y, double &ax,
                                              double &ay, uint32_t &) const {
     force_hex_r2(wsb, T.umax, ks, cu, g, x, y, __builtin_fma(x, x, y * y), ax, ay);
   }
@@ -194,7 +209,7 @@ struct Force<CTRAJ_FORCE_WSPLINE_HEX> {
 template <>
 struct Force<CTRAJ_FORCE_WSPLINE_LIN> {
   double ks, cu, g;
-  __device__ __forceinline__ void operator()(const Tables &T, const lds_double *wsb, double x, double y, double &ax,
+  __device__ __forceinline__ void operator()(const Tables &T, const WsRef &wsb, double x, double y, double &ax,
                                              double &ay, uint32_t &) const {
     force_lin(wsb, T.umax, ks, cu, g, x, y, ax, ay);
   }
@@ -239,7 +254,7 @@ __device__ __forceinline__ Force<CTRAJ_FORCE_WSPLINE_LIN> make_force<CTRAJ_FORCE
 // One step of the reference's lens integrator (electrostatic_lens.py:91-111): RK4 with the second
 // stage at the full Euler point x + dt k1; the new row stores a = l1.
 template <int FORCE>
-__device__ __forceinline__ void ref_step(const Force<FORCE> &F, const Tables &T, const lds_double *wsb, double dt,
+__device__ __forceinline__ void ref_step(const Force<FORCE> &F, const Tables &T, const WsRef &wsb, double dt,
                                          MState &m, uint32_t &status) {
   const double x = m.x, y = m.y, z = m.z;
   const double k1x = m.vx, k1y = m.vy, k1z = m.vz;
@@ -278,7 +293,7 @@ template <int FORCE, class Sink>
 __device__ __forceinline__ bool lens_ref(const ctraj_elem &E, const Tables &T, int state, double g, MState &m,
                                          uint32_t &status, WalkCounters &wc, Sink &sink) {
   const Force<FORCE> F = make_force<FORCE>(E, g);
-  const lds_double *wsb = ws_of(T, state);
+  const WsRef &wsb = ws_of(T, state);
   const double half_d = E.p[0];
   const int N = E.n_steps;
   const double dt = E.p[2] / m.vz;
@@ -303,7 +318,7 @@ constexpr double kSixth = 1.0 / 6.0;
 // rk4_step_hh takes hh and h6 precomputed (a kernel whose step size is uniform keeps them in
 // scalar registers); rk4_step forms them from h.
 template <int FORCE>
-__device__ __forceinline__ void rk4_step_hh(const Force<FORCE> &F, const Tables &T, const lds_double *wsb, double h,
+__device__ __forceinline__ void rk4_step_hh(const Force<FORCE> &F, const Tables &T, const WsRef &wsb, double h,
                                             double hh, double h6, double &x, double &y, double &vx, double &vy,
                                             double r2, uint32_t &status) {
   double ax, ay;
@@ -347,7 +362,7 @@ __device__ __forceinline__ void rk4_step_hh(const Force<FORCE> &F, const Tables 
 }
 
 template <int FORCE>
-__device__ __forceinline__ void rk4_step(const Force<FORCE> &F, const Tables &T, const lds_double *wsb, double h,
+__device__ __forceinline__ void rk4_step(const Force<FORCE> &F, const Tables &T, const WsRef &wsb, double h,
                                          double &x, double &y, double &vx, double &vy, double r2, uint32_t &status) {
   rk4_step_hh<FORCE>(F, T, wsb, h, 0.5 * h, h * kSixth, x, y, vx, vy, r2, status);
 }
@@ -371,7 +386,7 @@ template <int FORCE, class Sink>
 __device__ __forceinline__ bool lens_rk4(const ctraj_elem &E, const Tables &T, int state, double g, int cap,
                                          MState &m, uint32_t &status, WalkCounters &wc, Sink &sink) {
   const Force<FORCE> F = make_force<FORCE>(E, g);
-  const lds_double *wsb = ws_of(T, state);
+  const WsRef &wsb = ws_of(T, state);
   const double R2 = E.p[1];
   const double h = E.p[2];
   const double L = E.p[3];

@@ -23,7 +23,7 @@ enum : int { PH_NEED = 0, PH_LENS = 1, PH_POST = 2, PH_DONE = 3 };
 struct LaneMol {
   MState m;
   int64_t gidx;
-  const lds_double *wsb;  // LDS spline of the molecule's state
+  WsRef wsb;  // LDS spline of the molecule's state
   int st;
   int label;
   uint32_t status;

@@ -9,26 +9,27 @@
 // ---------------------------------------------------------------------------------------------
 namespace ctraj {
 
-// The W' spline is kept in LDS as [state][seg][3] = {A, B, C}: a 24-byte segment stride spreads
-// the lanes' random segments over the LDS banks far better than the 32-byte stride of the global
-// table (which puts every segment on one of 8 bank groups).  r tables keep [knot][4].
+// The W' spline is kept in LDS as two arrays over all segments of all states: (A, B) pairs
+// [state][seg] (16-byte stride, at the LDS base) and C [state][seg] after them; the global table is
+// [seg][4] = {A, B, C, 0}.  r tables keep [knot][4].  The LDS base must be 16-byte aligned: every
+// kernel declares its dynamic LDS with that alignment.
 __device__ inline Tables stage_tables(double *lds, const KTab &kt) {
   Tables T;
   const int nseg_all = kt.ws_states * kt.ws_nseg;
   const int nws = nseg_all * 3;
   const int nrt = kt.rt_total * 4;
+  double2 *lds_ab = reinterpret_cast<double2 *>(lds);
+  double *lds_c = lds + 2 * nseg_all;
   for (int i = threadIdx.x; i < nseg_all; i += blockDim.x) {
-    const double2 ab = reinterpret_cast<const double2 *>(kt.ws)[2 * i];
-    const double c = kt.ws[4 * i + 2];
-    lds[3 * i] = ab.x;
-    lds[3 * i + 1] = ab.y;
-    lds[3 * i + 2] = c;
+    lds_ab[i] = reinterpret_cast<const double2 *>(kt.ws)[2 * i];
+    lds_c[i] = kt.ws[4 * i + 2];
   }
   const double2 *src_rt = reinterpret_cast<const double2 *>(kt.rt);
   double2 *dst_rt = reinterpret_cast<double2 *>(lds + nws + (nws & 1));
   for (int i = threadIdx.x; i < nrt / 2; i += blockDim.x) dst_rt[i] = src_rt[i];
   const lds_double *base = (const lds_double *)lds;
-  T.ws = base;
+  T.ws_ab = (const lds_dbl2 *)lds_ab;
+  T.ws_c = base + 2 * nseg_all;
   T.nseg = kt.ws_nseg;
   T.umax = kt.umax;
   T.rt = base + nws + (nws & 1);

@@ -165,7 +165,7 @@ typedef struct ctraj_run {
                           * 0 = 256, or 768 for the lane walker when the lens tables exceed 40 KB of LDS */
   int32_t grid;          /* blocks (0 = enough for n, capped at 8 per CU x n_cu x 8) */
   int32_t flags;         /* CTRAJ_RUN_* */
-  int32_t refill;        /* lane refill: idle lanes per wavefront that trigger a refill (0 = 16 two-phase, 24 single kernel) */
+  int32_t refill;        /* lane refill: idle lanes per wavefront that trigger a refill (0 = 12 two-phase, 24 single kernel) */
 } ctraj_run;
 int32_t ctraj_propagate(ctraj_ctx *ctx, const ctraj_run *run);
 
