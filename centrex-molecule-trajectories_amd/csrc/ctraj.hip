@@ -525,7 +525,7 @@ enum : int { LN_IDLE = 0, LN_LENS = 1, LN_EXIT = 2, LN_FIN = 3, LN_DONE = 4 };
 // Five wavefronts per SIMD (at most 96 VGPRs) for the hexapole RK4 circular-aperture-only
 // instantiation, which fits them without spilling; the others need more registers than that.
 template <int FORCE, int INTEG, int KM>
-__global__ void __launch_bounds__(768) __attribute__((amdgpu_waves_per_eu(
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
     FORCE == CTRAJ_FORCE_WSPLINE_HEX && INTEG == CTRAJ_INTEG_RK4 && KM == KM_CIRC ? 5 : 1)))
 k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refill_min, double Hh, double H6) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
@@ -1362,12 +1362,13 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
   // the lens-entry queue costs 64 B per molecule: single runs above 2^27 molecules (8.6 GB of
   // queue) use the single-kernel walker, which needs no queue
   const bool pipeline = n_lens == 1 && !(R->flags & (CTRAJ_RUN_SIMPLE | CTRAJ_RUN_LANES)) && R->n <= ((int64_t)1 << 27);
-  const bool lanes = n_lens == 1 && !(R->flags & CTRAJ_RUN_SIMPLE) && (R->flags & CTRAJ_RUN_LANES);
-  // The lane walker keeps its tables once per workgroup: with several lens-state tables (more
-  // than ~40 KB) one 768-thread workgroup per CU holds as many wavefronts as its registers allow,
+  const bool lanes = n_lens == 1 && !(R->flags & CTRAJ_RUN_SIMPLE) && !pipeline;
+  // The lane-refill kernels keep their tables once per workgroup: with several lens-state tables
+  // (more than ~40 KB) one large workgroup per CU holds as many wavefronts as its registers allow,
   // where 256-thread workgroups would be limited by LDS to one per CU.
-  const int block = R->block ? R->block : ((lanes || pipeline) && lds > 40960 ? 768 : 256);
-  const int max_block = (lanes || pipeline) ? 768 : 256;
+  // (one 1024-thread two-phase lens workgroup per CU holds 16 wavefronts beside several state tables)
+  const int block = R->block ? R->block : (pipeline && lds > 40960 ? 1024 : (lanes && lds > 40960 ? 768 : 256));
+  const int max_block = pipeline ? 1024 : (lanes ? 768 : 256);
   if (block < 64 || block > max_block || block % 64)
     return fail(c, CTRAJ_EINVAL, "block must be a multiple of 64 up to " + std::to_string(max_block));
   HIPCHK(c, hipSetDevice(c->device));

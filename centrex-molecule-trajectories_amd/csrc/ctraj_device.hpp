@@ -143,7 +143,8 @@ __device__ __forceinline__ WsRef ws_of(const Tables &T, int state) {
 // min(u, umax) as one v_min_f64.  __builtin_fmin would also canonicalise umax on every call (a
 // second FP64 instruction per look-up) because the compiler cannot see that the table bound, a
 // normal double, needs none; u is an arithmetic result and umax a normal number, so the IEEE-mode
-// minimum is fmin's result, including umax for a NaN u.
+// minimum is fmin's result, including umax for a NaN u.  (umax stays a VGPR operand: pinning it
+// to scalar registers costs the lens kernel a spill.)
 __device__ __forceinline__ double clamp_to(double u, double umax) {
   double r;
   asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(u), "v"(umax));

@@ -161,8 +161,9 @@ typedef struct ctraj_run {
   int32_t *aoi_label;    /* [aoi_cap] their labels (optional) */
   ctraj_soa aoi_out;     /* [aoi_cap] their final phase space (optional fields) */
   uint64_t *work;        /* [CTRAJ_WORK_N] counters, accumulated (optional) */
-  int32_t block;         /* threads per block, multiple of 64: up to 768 for the lane walker (one lens), else 256;
-                          * 0 = 256, or 768 for the lane walker when the lens tables exceed 40 KB of LDS */
+  int32_t block;         /* threads per block, multiple of 64: up to 1024 for the two-phase lens kernel and 768
+                          * for the lane walker (one lens), else 256; 0 = 256, or 1024 / 768 when the lens
+                          * tables exceed 40 KB of LDS */
   int32_t grid;          /* blocks (0 = enough for n, capped at 8 per CU x n_cu x 8) */
   int32_t flags;         /* CTRAJ_RUN_* */
   int32_t refill;        /* lane refill: idle lanes per wavefront that trigger a refill (0 = 12 two-phase, 24 single kernel) */
