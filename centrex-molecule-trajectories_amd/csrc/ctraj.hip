@@ -103,7 +103,7 @@ struct PipeArgs {
   uint32_t *wcount;            // PRE: entries queued by each wavefront (a prefix of its region)
   int64_t region;              // queue entries owned by each PRE wavefront, a multiple of 64
   int n_pre_waves;
-  unsigned long long *claim;   // lens kernel: next unclaimed 64-entry chunk of the queue
+  unsigned long long *claim;   // lens kernel: next unclaimed 64-entry chunk of each region group, [8][32]
   int lens_e;
 };
 
@@ -515,10 +515,13 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
 // `refill_min` lanes are idle (or none is in the lens), a convergent section walks the molecules
 // that left the lens through the remaining elements, records every finished molecule (outputs,
 // tallies, kept-molecule compaction) and gives idle lanes the next queued molecules.  Wavefronts
-// claim the queue in 64-entry chunks, region by region: chunk c is entries [64 (c % cpr), +64) of
-// the region of PRE wavefront c / cpr (cpr = region / 64), of which the first wcount are filled, so
-// a claim needs no search and a chunk past a region's count is simply claimed over.  The
-// per-molecule arithmetic is the one of walk().
+// claim the queue in 64-entry chunks, region by region, from eight counters: counter g serves the
+// regions of the PRE workgroups dispatched to XCD g (workgroup b runs on XCD b mod 8), and a
+// wavefront starts with its own XCD's counter and moves to the next one when it runs dry, which
+// spreads the claim atomics over eight addresses.  Chunk c of group g is entries
+// [64 (c % cpr), +64) of the group's region c / cpr (cpr = region / 64), of which the first wcount
+// are filled, so a claim needs no search and a chunk past a region's count is simply claimed over.
+// The per-molecule arithmetic is the one of walk().
 // ---------------------------------------------------------------------------------------------
 enum : int { LN_IDLE = 0, LN_LENS = 1, LN_EXIT = 2, LN_FIN = 3, LN_DONE = 4 };
 
@@ -539,7 +542,9 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
   const Tables T = stage_tables(lds, KT);
   __syncthreads();
   const uint32_t cpr = (uint32_t)(P.region >> 6);
-  const unsigned long long n_chunks = (unsigned long long)P.n_pre_waves * cpr;
+  // eight claim counters, one per group of PRE regions written by one XCD's workgroups
+  const unsigned long long n_chunks = (unsigned long long)(P.n_pre_waves >> 3) * cpr;
+  int grp = blockIdx.x & 7, tried = 0;
 
   const int lane = lane_id();
   const ctraj_elem &L0 = B.elems[P.lens_e];
@@ -643,14 +648,16 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
         }
         if (next >= end) {
           unsigned long long c = 0;
-          if (lane == 0) c = atomicAdd(P.claim, 1ull);
+          if (lane == 0) c = atomicAdd(P.claim + 32 * grp, 1ull);
           c = __shfl(c, 0);
           if (c >= n_chunks) {
-            exhausted = true;
+            if (++tried == 8) exhausted = true;
+            grp = (grp + 1) & 7;
             continue;
           }
-          const uint32_t w = (uint32_t)c / cpr;
-          const int64_t base = (int64_t)((uint32_t)c - w * cpr) * 64;
+          const uint32_t r = (uint32_t)c / cpr;
+          const uint32_t w = ((r >> 2) << 5) | ((uint32_t)grp << 2) | (r & 3u);
+          const int64_t base = (int64_t)((uint32_t)c - r * cpr) * 64;
           const int64_t filled = (int64_t)P.wcount[w] - base;
           next = (int64_t)w * P.region + base;
           end = next + (filled < 64 ? filled : 64);  // next >= end: an empty chunk, claim again
@@ -1382,24 +1389,24 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     PropFn kpre = walker_kernel<PHASE_PRE>(kinds_mask(c->h_elems, 0, lens_e), wants_outputs(R));
     int rc = lds_limit(c, (const void *)kpre, lds_walk);
     if (rc) return rc;
-    const int gpre = grid_for(c, (const void *)kpre, blk, lds_walk, R->n, R->grid);
+    const int gpre = (grid_for(c, (const void *)kpre, blk, lds_walk, R->n, R->grid) + 7) & ~7;  // whole XCD groups
     const int nw = gpre * (blk / 64);
     const int64_t threads = (int64_t)gpre * blk;
     const int64_t region = (R->n + threads - 1) / threads * 64;
     const size_t cap = (size_t)nw * (size_t)region;
     const size_t bytes = cap * 64 + 256;
     if ((rc = ensure_buf(c, c->pipe, bytes)) != CTRAJ_OK) return rc;
-    if ((rc = ensure_buf(c, c->pipe_cnt, 64 + (size_t)nw * sizeof(uint32_t))) != CTRAJ_OK) return rc;
+    if ((rc = ensure_buf(c, c->pipe_cnt, 8 * 256 + (size_t)nw * sizeof(uint32_t))) != CTRAJ_OK) return rc;
     PipeArgs P{};
     char *b = (char *)c->pipe.p;
     P.q = (longlong2 *)b;
     char *cb = (char *)c->pipe_cnt.p;
     P.claim = (unsigned long long *)cb;
-    P.wcount = (uint32_t *)(cb + 64);
+    P.wcount = (uint32_t *)(cb + 8 * 256);
     P.region = region;
     P.n_pre_waves = nw;
     P.lens_e = lens_e;
-    if ((rc = ctraj_fill_i64(c, (int64_t *)P.claim, 0, 1)) != CTRAJ_OK) return rc;
+    if ((rc = ctraj_fill_i64(c, (int64_t *)P.claim, 0, 8 * 32)) != CTRAJ_OK) return rc;
     hipLaunchKernelGGL(kpre, dim3(gpre), dim3(blk), lds_walk, c->stream, B, kt, *R, 0, P);
     HIPCHK(c, hipGetLastError());
     // phase 2: the lens and the elements after it
