@@ -187,8 +187,8 @@ def bench_adjoint(ctx, n: int, reps: int = 5) -> dict:
 def bench_batch(ctx, reps: int = 5, tune_reps: int = 2) -> dict:
     """The batched scan: 256 ES-lens configurations (V0, lens z0, detector radius, molecule count
     drawn from RandomState(2)), molecules drawn on the device, one persistent graph-replayed launch.
-    The autotuner picks (block, chunk, refill); the same configurations run one by one (device draw +
-    ctraj_propagate each) give the unbatched time."""
+    The autotuner picks the path (two phases or one fused kernel) and its (block, chunk, refill); the
+    same configurations run one by one (device draw + ctraj_propagate each) give the unbatched time."""
     from trajectories import configs
     from trajectories.batch import BatchScan
     from trajectories.engine import Engine
@@ -196,7 +196,8 @@ def bench_batch(ctx, reps: int = 5, tune_reps: int = 2) -> dict:
     bls, n, seeds = configs.config2()
     scan = BatchScan(bls, n, seeds, configs.SOURCE_MEAN, configs.SOURCE_SIGMA, ctx=ctx)
     t0 = time.perf_counter()
-    best = scan.autotune(reps=tune_reps)
+    best = scan.autotune(blocks=(256, 512, 1024), chunks=(128, 256), refills=(8, 12, 16, 24, 32), fused=(0, 1),
+                         reps=tune_reps)
     t_tune = time.perf_counter() - t0
     scan.run(best)
     runs = [scan.run(best) for _ in range(reps)]
@@ -221,18 +222,20 @@ def bench_batch(ctx, reps: int = 5, tune_reps: int = 2) -> dict:
     ctx.sync()
     seq_ms = ctx.event_elapsed_ms(0, 1)
     seq_wall = (time.perf_counter() - wall0) * 1e3
-    top = [(b, c, r, t) for t, b, c, r in scan.tuning[:5]]
+    top = [(b, c, r, f, t) for t, b, c, r, f in scan.tuning[:5]]
     return {
         "workload": "256 ES-lens configurations: V0~U[10,40] kV, lens z0~U[0.7,0.9] m, detector r~U[2,5] mm, "
                     "n~round(10^U[3,6]) from RandomState(2); device Philox source per configuration",
         "configurations": len(bls), "molecules": int(n.sum()),
         "ms": ms, "molecules_per_s": float(n.sum()) / (ms * 1e-3),
-        "tune": {"block": best.block, "chunk": best.chunk, "refill": best.refill, "graph": bool(best.use_graph),
-                 "candidates": len(scan.tuning), "tuning_s": t_tune, "top5_block_chunk_refill_ms": top},
+        "tune": {"block": best.block, "chunk": best.chunk, "refill": best.refill, "fused": bool(best.fused),
+                 "graph": bool(best.use_graph), "candidates": len(scan.tuning), "tuning_s": t_tune,
+                 "top5_block_chunk_refill_fused_ms": top},
         "unbatched_device_ms": seq_ms, "unbatched_wall_ms": seq_wall, "speedup_vs_unbatched": seq_ms / ms,
         "detected_fraction": {"min": float(frac.min()), "median": float(np.median(frac)), "max": float(frac.max())},
         "roofline": {"bound": "mfma", "engine": ENGINE, "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "kernel": "k_batch_lanes", "flop_per_launch": flop},
+                     "frac": achieved / PEAK_FP64,
+                     "kernel": "k_batch_lanes" if best.fused else "k_batch_pre + k_batch_lens", "flop_per_launch": flop},
     }
 
 

@@ -28,6 +28,9 @@ using namespace ctraj;
 namespace {
 
 constexpr int kMaxBins = 4096;  // configurations x labels tallied in LDS
+constexpr int64_t kPipeMax = (int64_t)1 << 27;  // larger batches run the single-kernel walker (64 B of queue each)
+
+enum : int { LN_IDLE = 0, LN_LENS = 1, LN_EXIT = 2, LN_FIN = 3, LN_DONE = 4 };
 
 struct BCfg {
   int64_t begin, n;
@@ -36,8 +39,15 @@ struct BCfg {
   int32_t e0, ne, lens, pad;  // element slice; lens index within the slice (-1: none)
 };
 
+// Lens constants of one configuration, staged in the two-phase lens kernel's LDS.
+struct BLensC {
+  double ks, cu, R2, z1, Hs, Ll;
+  int32_t label_inside, e0, ne, lens;  // element slice and the lens's index in it
+};
+
 struct BArgs {
   const BCfg *cfgs;
+  const BLensC *lensk;
   const ctraj_elem *elems;
   unsigned long long *counter;
   int64_t *tally;               // [n_cfg][n_labels]
@@ -45,7 +55,18 @@ struct BArgs {
   int64_t total;
   double g;
   int n_cfg, n_labels, label_detected, cap, chunk, refill_min;
-  int off_begin, off_hist;      // in doubles from the LDS base
+  int off_begin, off_hist;      // in doubles from the LDS base (off_begin: the lens constants in the lens kernel)
+};
+
+// Two-phase scan: the PRE kernel queues the molecules that enter their configuration's lens into
+// per-wavefront regions (the single-run queue layout, key = configuration index); the lens kernel
+// claims 64-entry chunks from eight per-XCD counters.
+struct BPipe {
+  longlong2 *q;
+  uint32_t *wcount;             // entries queued by each PRE wavefront
+  unsigned long long *claim;    // [8][32]
+  int64_t region;
+  int n_pre_waves;
 };
 
 struct BLane {
@@ -61,6 +82,51 @@ __device__ __forceinline__ int find_cfg(const int64_t *s_begin, int n_cfg, int64
     else hi = mid;
   }
   return lo;
+}
+
+// Molecule j of configuration C from its Philox stream, walked through the configuration's
+// elements `el` before its lens and through the lens entrance.  Returns true if it enters the lens;
+// otherwise it has finished with label `lab`.
+__device__ __forceinline__ bool batch_start(const BCfg &C, const ctraj_elem *el, uint64_t j, double g,
+                                            int label_detected, MState &m, int &lab, WalkCounters &wc) {
+  double z[6];
+  gaussian_pair(C.seed, j, 0u, z[0], z[1]);
+  gaussian_pair(C.seed, j, 1u, z[2], z[3]);
+  gaussian_pair(C.seed, j, 2u, z[4], z[5]);
+  m.x = C.mean[0] + C.sigma[0] * z[0];
+  m.y = C.mean[1] + C.sigma[1] * z[1];
+  m.z = C.mean[2] + C.sigma[2] * z[2];
+  m.vx = C.mean[3] + C.sigma[3] * z[3];
+  m.vy = C.mean[4] + C.sigma[4] * z[4];
+  m.vz = C.mean[5] + C.sigma[5] * z[5];
+  m.t = 0.0;
+  m.ax = 0.0;
+  m.ay = -g;
+  m.az = 0.0;
+  lab = label_detected;
+  const bool ok = walk_plain(el, 0, C.lens >= 0 ? C.lens : C.ne, g, m, lab, wc);
+  if (!ok || C.lens < 0) return false;
+  const ctraj_elem &L = el[C.lens];
+  drift_dt(m, time_to(m, L.z0), g);
+  ++wc.faces;
+  if (rho_of(m.x, m.y) > L.p[0]) {
+    lab = L.label_hit;
+    return false;
+  }
+  return true;
+}
+
+// (configuration, label) tally of the lanes with `fin` set into the workgroup's LDS histogram:
+// one atomic per distinct key in the wavefront.  Every lane takes part.
+__device__ __forceinline__ void batch_hist(uint32_t *s_hist, bool fin, int key, int lane) {
+  uint64_t todo = __ballot(fin);
+  while (todo) {
+    const int leader = __ffsll((unsigned long long)todo) - 1;
+    const int lk = __shfl(key, leader);
+    const uint64_t same = __ballot(key == lk) & todo;
+    if (lane == leader) atomicAdd(&s_hist[lk], (uint32_t)__popcll(same));
+    todo &= ~same;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
@@ -111,16 +177,8 @@ __global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
             n_oot += (s.status & CTRAJ_ST_OOT) ? 1u : 0u;
             n_cap += (s.status & CTRAJ_ST_CAPPED) ? 1u : 0u;
           }
-          // every lane takes part; lanes without a finished molecule add to a key no one else uses
-          const int key = s.fin ? b.c * A.n_labels + s.label : nbins + lane;
-          uint64_t todo = __ballot(s.fin != 0);
-          while (todo) {
-            const int leader = __ffsll((unsigned long long)todo) - 1;
-            const int lk = __shfl(key, leader);
-            const uint64_t same = __ballot(key == lk) & todo;
-            if (lane == leader) atomicAdd(&s_hist[lk], (uint32_t)__popcll(same));
-            todo &= ~same;
-          }
+          // lanes without a finished molecule carry a key no one else uses
+          batch_hist(s_hist, s.fin != 0, s.fin ? b.c * A.n_labels + s.label : nbins + lane, lane);
           s.fin = 0;
         }
         const uint64_t need = __ballot(s.phase == PH_NEED);
@@ -146,52 +204,29 @@ __global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
             const int64_t gi = next + rank;
             const int c = find_cfg(s_begin, A.n_cfg, gi);
             const BCfg &C = opaque(A.cfgs)[c];
-            const uint64_t j = (uint64_t)(gi - C.begin);
-            double z[6];
-            gaussian_pair(C.seed, j, 0u, z[0], z[1]);
-            gaussian_pair(C.seed, j, 1u, z[2], z[3]);
-            gaussian_pair(C.seed, j, 2u, z[4], z[5]);
             MState &m = s.m;
-            m.x = C.mean[0] + C.sigma[0] * z[0];
-            m.y = C.mean[1] + C.sigma[1] * z[1];
-            m.z = C.mean[2] + C.sigma[2] * z[2];
-            m.vx = C.mean[3] + C.sigma[3] * z[3];
-            m.vy = C.mean[4] + C.sigma[4] * z[4];
-            m.vz = C.mean[5] + C.sigma[5] * z[5];
-            m.t = 0.0;
-            m.ax = 0.0;
-            m.ay = -g;
-            m.az = 0.0;
             b.c = c;
             b.e0 = C.e0;
             b.ne = C.ne;
             b.lens = C.lens;
             s.status = 0;
             const ctraj_elem *el = opaque(A.elems) + b.e0;
-            int lab = A.label_detected;
-            bool ok = walk_plain(el, 0, b.lens >= 0 ? b.lens : b.ne, g, m, lab, wc);
-            if (ok && b.lens >= 0) {
+            int lab;
+            if (batch_start(C, el, (uint64_t)(gi - C.begin), g, A.label_detected, m, lab, wc)) {
               const ctraj_elem &L = el[b.lens];
-              drift_dt(m, time_to(m, L.z0), g);
-              ++wc.faces;
-              if (rho_of(m.x, m.y) > L.p[0]) {
-                ok = false;
-                lab = L.label_hit;
-              } else {
-                const double Hs = L.p[2], Ll = L.p[3];
-                b.ks = L.p[4];
-                b.cu = L.p[5];
-                b.R2 = L.p[1];
-                b.z1 = L.z1;
-                b.label_inside = L.label_inside;
-                s.i = 0;
-                s.nfull = rk4_full_steps(Ll, m.vz, Hs, A.cap, s.status);
-                s.dzs = m.vz * Hs;
-                s.hp = Ll / m.vz - (double)s.nfull * Hs;
-                s.h = s.nfull == 0 ? s.hp : Hs;
-                s.r2 = __builtin_fma(m.x, m.x, m.y * m.y);
-                s.phase = PH_LENS;
-              }
+              const double Hs = L.p[2], Ll = L.p[3];
+              b.ks = L.p[4];
+              b.cu = L.p[5];
+              b.R2 = L.p[1];
+              b.z1 = L.z1;
+              b.label_inside = L.label_inside;
+              s.i = 0;
+              s.nfull = rk4_full_steps(Ll, m.vz, Hs, A.cap, s.status);
+              s.dzs = m.vz * Hs;
+              s.hp = Ll / m.vz - (double)s.nfull * Hs;
+              s.h = s.nfull == 0 ? s.hp : Hs;
+              s.r2 = __builtin_fma(m.x, m.x, m.y * m.y);
+              s.phase = PH_LENS;
             }
             if (s.phase != PH_LENS) {
               s.label = lab;
@@ -232,6 +267,194 @@ __global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
   if (threadIdx.x < 4 && s_work[threadIdx.x]) atomicAdd(&A.work[threadIdx.x], s_work[threadIdx.x]);
 }
 
+// Two-phase scan, phase 1: every molecule is drawn and walked to and through its configuration's
+// lens entrance (grid-stride, wave-uniform trip count); the molecules that enter a lens are queued
+// in the wavefront's region, the others are tallied.
+__global__ void __launch_bounds__(256) k_batch_pre(BArgs A, BPipe P) {
+  extern __shared__ __attribute__((aligned(16))) double lds[];
+  __shared__ unsigned long long s_work[4];
+  int64_t *s_begin = reinterpret_cast<int64_t *>(lds);
+  uint32_t *s_hist = reinterpret_cast<uint32_t *>(lds + A.n_cfg);
+  const int nbins = A.n_cfg * A.n_labels;
+  for (int i = threadIdx.x; i < A.n_cfg; i += blockDim.x) s_begin[i] = A.cfgs[i].begin;
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0;
+  if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
+  __syncthreads();
+
+  const int lane = lane_id();
+  const double g = A.g;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t gwave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  WalkCounters wc{0u, 0u};
+  int64_t qn = 0;
+  for (int64_t wbase = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < A.total; wbase += stride) {
+    const int64_t gi = wbase + lane;
+    const bool active = gi < A.total;
+    MState m;
+    int lab = 0, c = 0;
+    bool enq = false;
+    if (active) {
+      c = find_cfg(s_begin, A.n_cfg, gi);
+      const BCfg &C = opaque(A.cfgs)[c];
+      enq = batch_start(C, opaque(A.elems) + C.e0, (uint64_t)(gi - C.begin), g, A.label_detected, m, lab, wc);
+    }
+    const uint64_t qm = __ballot(enq);
+    if (enq) queue_put(P.q, gwave * P.region + qn + lanes_below(qm), m, c, 0);
+    qn += __popcll(qm);
+    const bool fin = active && !enq;
+    if (__ballot(fin)) batch_hist(s_hist, fin, fin ? c * A.n_labels + lab : nbins + lane, lane);
+  }
+  if (lane == 0) P.wcount[gwave] = (uint32_t)qn;
+  if (wc.faces) atomicAdd(&s_work[0], (unsigned long long)wc.faces);
+  __syncthreads();
+  for (int k = threadIdx.x; k < nbins; k += blockDim.x)
+    if (s_hist[k]) atomicAdd((unsigned long long *)&A.tally[k], (unsigned long long)s_hist[k]);
+  if (threadIdx.x < 4 && s_work[threadIdx.x]) atomicAdd(&A.work[threadIdx.x], s_work[threadIdx.x]);
+}
+
+// Two-phase scan, phase 2: persistent wavefronts integrate the queued molecules through their
+// configuration's lens with lane refill (the configuration's lens constants staged in LDS, its field
+// constants and step in the lane's registers), walk the survivors through the elements after the
+// lens and tally.  The arithmetic per molecule is the one of k_batch_lanes.
+__global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT) {
+  extern __shared__ __attribute__((aligned(16))) double lds[];
+  __shared__ unsigned long long s_work[4];
+  const Tables T = stage_tables(lds, KT);
+  BLensC *s_cfg = reinterpret_cast<BLensC *>(lds + A.off_begin);
+  uint32_t *s_hist = reinterpret_cast<uint32_t *>(lds + A.off_hist);
+  const int nbins = A.n_cfg * A.n_labels;
+  for (int i = threadIdx.x; i < A.n_cfg; i += blockDim.x) s_cfg[i] = A.lensk[i];
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0;
+  if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
+  __syncthreads();
+
+  const WsRef wsb = ws_of(T, 0);
+  const int lane = lane_id();
+  const double g = A.g;
+  const uint32_t cpr = (uint32_t)(P.region >> 6);
+  const unsigned long long n_chunks = (unsigned long long)(P.n_pre_waves >> 3) * cpr;
+  int grp = blockIdx.x & 7, tried = 0;
+  WalkCounters wc{0u, 0u};
+  uint32_t n_oot = 0, n_cap = 0;
+  unsigned long long steps_w = 0;
+  int64_t next = 0, end = 0;
+  bool exhausted = false;
+
+  MState m;
+  m.ax = 0.0; m.ay = -g; m.az = 0.0;
+  int c = 0, ph = LN_IDLE, label = 0, i = 0, nfull = 0;
+  uint32_t status = 0;
+  double ks = 0.0, cu = 0.0, R2 = 0.0, Hs = 0.0, dzs = 0.0, r2 = 0.0;
+
+  for (;;) {
+    const uint64_t lensm = __ballot(ph == LN_LENS);
+    if (lensm == 0 || 64 - __popcll(lensm) >= A.refill_min) {
+      for (;;) {
+        if (ph == LN_EXIT) {  // the lens exit row stores (0, -g, 0); then the elements after the lens
+          m.ax = 0.0;
+          m.ay = -g;
+          m.az = 0.0;
+          const BLensC &K = s_cfg[c];
+          label = A.label_detected;
+          walk_plain(opaque(A.elems) + K.e0, K.lens + 1, K.ne, g, m, label, wc);
+          ph = LN_FIN;
+        }
+        if (__ballot(ph == LN_FIN)) {
+          const bool fin = ph == LN_FIN;
+          if (fin) {
+            n_oot += (status & CTRAJ_ST_OOT) ? 1u : 0u;
+            n_cap += (status & CTRAJ_ST_CAPPED) ? 1u : 0u;
+          }
+          batch_hist(s_hist, fin, fin ? c * A.n_labels + label : nbins + lane, lane);
+          if (fin) ph = LN_IDLE;
+        }
+        const uint64_t need = __ballot(ph == LN_IDLE);
+        if (!need || exhausted) {
+          if (ph == LN_IDLE && exhausted) ph = LN_DONE;
+          break;
+        }
+        if (next >= end) {
+          unsigned long long cl = 0;
+          if (lane == 0) cl = atomicAdd(P.claim + 32 * grp, 1ull);
+          cl = __shfl(cl, 0);
+          if (cl >= n_chunks) {
+            if (++tried == 8) exhausted = true;
+            grp = (grp + 1) & 7;
+            continue;
+          }
+          const uint32_t r = (uint32_t)cl / cpr;
+          const uint32_t w = ((r >> 2) << 5) | ((uint32_t)grp << 2) | (r & 3u);
+          const int64_t base = (int64_t)((uint32_t)cl - r * cpr) * 64;
+          const int64_t filled = (int64_t)P.wcount[w] - base;
+          next = (int64_t)w * P.region + base;
+          end = next + (filled < 64 ? filled : 64);  // next >= end: an empty chunk, claim again
+          continue;
+        }
+        const int64_t avail = end - next;
+        if (ph == LN_IDLE) {
+          const int rank = lanes_below(need);
+          if (rank < avail) {
+            int64_t key;
+            int st;
+            queue_get(P.q, next + rank, m, key, st);
+            c = (int)key;
+            const BLensC &K = s_cfg[c];
+            ks = K.ks;
+            cu = K.cu;
+            R2 = K.R2;
+            Hs = K.Hs;
+            m.ax = 0.0;
+            m.ay = -g;
+            m.az = 0.0;
+            status = 0;
+            i = 0;
+            nfull = rk4_full_steps(K.Ll, m.vz, Hs, A.cap, status);
+            dzs = m.vz * Hs;
+            r2 = __builtin_fma(m.x, m.x, m.y * m.y);
+            ph = LN_LENS;
+          }
+        }
+        next += avail < (int64_t)__popcll(need) ? avail : (int64_t)__popcll(need);
+      }
+      if (!__ballot(ph == LN_LENS)) {
+        if (!__ballot(ph == LN_EXIT || ph == LN_FIN)) break;
+        continue;
+      }
+    }
+
+    steps_w += (unsigned long long)__popcll(__ballot(ph == LN_LENS));
+    if (ph == LN_LENS) {
+      const Force<CTRAJ_FORCE_WSPLINE_HEX> F{ks, cu, g};
+      const bool last = (i == nfull);
+      if (!last) {
+        rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, Hs, m.x, m.y, m.vx, m.vy, r2, status);
+        m.z = m.z + dzs;
+        m.t = m.t + Hs;
+      } else {  // the final partial step, formed here, once per molecule that reaches the lens end
+        const BLensC &K = s_cfg[c];
+        const double hp = K.Ll / m.vz - (double)nfull * Hs;
+        rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, hp, m.x, m.y, m.vx, m.vy, r2, status);
+        m.z = K.z1;
+        m.t = m.t + hp;
+      }
+      r2 = __builtin_fma(m.x, m.x, m.y * m.y);
+      ++i;
+      const bool dead = r2 > R2;
+      if (dead) label = s_cfg[c].label_inside;
+      ph = dead ? (int)LN_FIN : (last ? (int)LN_EXIT : (int)LN_LENS);
+    }
+  }
+
+  if (wc.faces) atomicAdd(&s_work[0], (unsigned long long)wc.faces);
+  if (lane == 0 && steps_w) atomicAdd(&s_work[1], steps_w);
+  if (n_oot) atomicAdd(&s_work[2], (unsigned long long)n_oot);
+  if (n_cap) atomicAdd(&s_work[3], (unsigned long long)n_cap);
+  __syncthreads();
+  for (int k = threadIdx.x; k < nbins; k += blockDim.x)
+    if (s_hist[k]) atomicAdd((unsigned long long *)&A.tally[k], (unsigned long long)s_hist[k]);
+  if (threadIdx.x < 4 && s_work[threadIdx.x]) atomicAdd(&A.work[threadIdx.x], s_work[threadIdx.x]);
+}
+
 __global__ void __launch_bounds__(256) k_zero_u64(unsigned long long *p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
 }
@@ -249,12 +472,17 @@ struct ctraj_batch {
   int64_t *d_tally = nullptr;
   double *d_ws = nullptr;  // the batch's own copy of the W' spline (state 0), [nseg][4]
   int ws_nseg = 0;
+  // two-phase path: per-configuration lens constants, the lens-entry queue and its counters
+  BLensC *d_lensk = nullptr;
+  void *d_pipe = nullptr, *d_pipe_cnt = nullptr;
+  int pre_grid = 0, n_pre_waves = 0;
+  int64_t region = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // captured launches by (block, chunk, grid, refill)
-  std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
+  // captured launches by (block, chunk, grid, refill, fused)
+  std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> graphs;
 };
 
-static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refill) {
+static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refill, bool fused) {
   ctraj_ctx *c = b->ctx;
   KTab kt{};
   kt.ws = b->d_ws;
@@ -266,6 +494,7 @@ static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refi
   const size_t tab = ktab_lds_bytes(kt) / sizeof(double);
   BArgs A{};
   A.cfgs = b->d_cfgs;
+  A.lensk = b->d_lensk;
   A.elems = b->d_elems;
   A.counter = b->d_counter;
   A.work = b->d_counter + 1;
@@ -278,19 +507,50 @@ static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refi
   A.cap = b->cap;
   A.chunk = chunk;
   A.refill_min = refill;
-  A.off_begin = (int)tab;
-  A.off_hist = (int)(tab + b->n_cfg);
-  const size_t lds = (tab + b->n_cfg) * sizeof(double) + (size_t)b->n_cfg * b->n_labels * sizeof(uint32_t);
-  int rc = lds_limit(c, (const void *)k_batch_lanes, lds);
-  if (rc != CTRAJ_OK) return rc;
-  if ((rc = block_ok(c, (const void *)k_batch_lanes, block)) != CTRAJ_OK) return rc;
-  if (grid <= 0) grid = grid_for(c, (const void *)k_batch_lanes, block, lds, (int64_t)1 << 40, 0);
+  const size_t hist_bytes = (size_t)b->n_cfg * b->n_labels * sizeof(uint32_t);
   const int64_t nz = 1 + CTRAJ_WORK_N;
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, b->d_counter, nz);
   const int64_t nt = (int64_t)b->n_cfg * b->n_labels;
+  int rc = CTRAJ_OK;
+  if (fused || !b->d_pipe) {
+    A.off_begin = (int)tab;
+    A.off_hist = (int)(tab + b->n_cfg);
+    const size_t lds = (tab + b->n_cfg) * sizeof(double) + hist_bytes;
+    if ((rc = lds_limit(c, (const void *)k_batch_lanes, lds)) != CTRAJ_OK) return rc;
+    if ((rc = block_ok(c, (const void *)k_batch_lanes, block)) != CTRAJ_OK) return rc;
+    if (grid <= 0) grid = grid_for(c, (const void *)k_batch_lanes, block, lds, (int64_t)1 << 40, 0);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, b->d_counter, nz);
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<int64_t>((nt + 255) / 256, 1024)), dim3(256), 0,
+                       c->stream, (unsigned long long *)b->d_tally, nt);
+    hipLaunchKernelGGL(k_batch_lanes, dim3(grid), dim3(block), lds, c->stream, A, kt);
+    HIPCHK(c, hipGetLastError());
+    return CTRAJ_OK;
+  }
+  BPipe P{};
+  P.q = (longlong2 *)b->d_pipe;
+  P.claim = (unsigned long long *)b->d_pipe_cnt;
+  P.wcount = (uint32_t *)((char *)b->d_pipe_cnt + 8 * 256);
+  P.region = b->region;
+  P.n_pre_waves = b->n_pre_waves;
+  // phase 1: LDS = configuration offsets + histogram
+  const size_t lds_pre = (size_t)b->n_cfg * sizeof(int64_t) + hist_bytes;
+  if ((rc = lds_limit(c, (const void *)k_batch_pre, lds_pre)) != CTRAJ_OK) return rc;
+  // phase 2: LDS = tables + lens constants + histogram
+  const size_t cfg_dbl = (size_t)b->n_cfg * (sizeof(BLensC) / sizeof(double));
+  A.off_begin = (int)tab;
+  A.off_hist = (int)(tab + cfg_dbl);
+  const size_t lds_lens = (tab + cfg_dbl) * sizeof(double) + hist_bytes;
+  if ((rc = lds_limit(c, (const void *)k_batch_lens, lds_lens)) != CTRAJ_OK) return rc;
+  if ((rc = block_ok(c, (const void *)k_batch_lens, block)) != CTRAJ_OK) return rc;
+  if (grid <= 0) grid = grid_for(c, (const void *)k_batch_lens, block, lds_lens, (int64_t)1 << 40, 0);
+  // queue counters and tallies are reset by fill kernels (no memset nodes in the captured graph)
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, c->stream, (unsigned long long *)b->d_pipe_cnt,
+                     (int64_t)8 * 32);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, b->d_counter, nz);
   hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<int64_t>((nt + 255) / 256, 1024)), dim3(256), 0, c->stream,
                      (unsigned long long *)b->d_tally, nt);
-  hipLaunchKernelGGL(k_batch_lanes, dim3(grid), dim3(block), lds, c->stream, A, kt);
+  hipLaunchKernelGGL(k_batch_pre, dim3(b->pre_grid), dim3(256), lds_pre, c->stream, A, P);
+  HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(k_batch_lens, dim3(grid), dim3(block), lds_lens, c->stream, A, P, kt);
   HIPCHK(c, hipGetLastError());
   return CTRAJ_OK;
 }
@@ -368,7 +628,45 @@ int32_t ctraj_batch_create(ctraj_ctx *c, const ctraj_elem *elems, int32_t n_elem
       return fail(c, CTRAJ_ENOMEM, "batch allocation failed");
     }
   }
+  // two-phase path: lens constants per configuration and a lens-entry queue sized for the batch
+  std::vector<BLensC> hl(n_cfgs);
+  for (int k = 0; k < n_cfgs; ++k) {
+    BLensC &K = hl[k];
+    K = BLensC{};
+    K.e0 = hc[k].e0;
+    K.ne = hc[k].ne;
+    K.lens = hc[k].lens;
+    if (hc[k].lens >= 0) {
+      const ctraj_elem &L = elems[hc[k].e0 + hc[k].lens];
+      K.ks = L.p[4];
+      K.cu = L.p[5];
+      K.R2 = L.p[1];
+      K.z1 = L.z1;
+      K.Hs = L.p[2];
+      K.Ll = L.p[3];
+      K.label_inside = L.label_inside;
+    }
+  }
+  if (any_lens && total > 0 && total <= kPipeMax) {
+    const size_t lds_pre = (size_t)n_cfgs * sizeof(int64_t) + nt * sizeof(uint32_t);
+    int gpre = grid_for(c, (const void *)k_batch_pre, 256, lds_pre, total, 0);
+    gpre = (gpre + 7) & ~7;  // whole groups of eight workgroups: one queue-region group per XCD
+    const int64_t threads = (int64_t)gpre * 256;
+    b->pre_grid = gpre;
+    b->n_pre_waves = gpre * 4;
+    b->region = (total + threads - 1) / threads * 64;
+    const size_t qbytes = (size_t)b->n_pre_waves * (size_t)b->region * 64;
+    if (hipMalloc((void **)&b->d_lensk, sizeof(BLensC) * n_cfgs) != hipSuccess ||
+        hipMalloc(&b->d_pipe, qbytes) != hipSuccess ||
+        hipMalloc(&b->d_pipe_cnt, 8 * 256 + (size_t)b->n_pre_waves * sizeof(uint32_t)) != hipSuccess) {
+      // not enough memory for the queue: the single-kernel walker needs none
+      (void)hipGetLastError();
+      if (b->d_pipe) (void)hipFree(b->d_pipe);
+      b->d_pipe = nullptr;
+    }
+  }
   if (hipStreamSynchronize(c->stream) != hipSuccess ||
+      (b->d_lensk && hipMemcpy(b->d_lensk, hl.data(), sizeof(BLensC) * n_cfgs, hipMemcpyHostToDevice) != hipSuccess) ||
       (b->d_ws && hipMemcpy(b->d_ws, c->d_ws, sizeof(double) * 4 * b->ws_nseg, hipMemcpyDeviceToDevice) != hipSuccess) ||
       hipMemcpy(b->d_cfgs, hc.data(), sizeof(BCfg) * n_cfgs, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_elems, elems, sizeof(ctraj_elem) * n_elems_total, hipMemcpyHostToDevice) != hipSuccess) {
@@ -385,24 +683,26 @@ int32_t ctraj_batch_run(ctraj_batch *b, const ctraj_batch_tune *tune, int64_t *t
   if (!b) return fail(nullptr, CTRAJ_EINVAL, "null batch");
   ctraj_ctx *c = b->ctx;
   HIPCHK(c, hipSetDevice(c->device));
-  const int block = tune && tune->block ? tune->block : 256;
+  const bool fused = (tune && tune->fused) || !b->d_pipe;
+  const int block = tune && tune->block ? tune->block : (fused ? 256 : 1024);
   const int chunk = tune && tune->chunk ? tune->chunk : 128;
   const int grid = tune ? tune->grid : 0;
   const int refill = tune && tune->refill ? tune->refill : 16;
   const bool use_graph = tune && tune->use_graph;
-  if (block % 64 || block < 64 || block > 256) return fail(c, CTRAJ_EINVAL, "block must be 64, 128, 192 or 256");
+  if (block % 64 || block < 64 || block > (fused ? 256 : 1024))
+    return fail(c, CTRAJ_EINVAL, fused ? "block must be 64, 128, 192 or 256" : "block must be a multiple of 64 up to 1024");
   if (chunk < 1 || chunk > (1 << 20)) return fail(c, CTRAJ_EINVAL, "chunk must be 1..2^20");
   if (refill < 1 || refill > 64) return fail(c, CTRAJ_EINVAL, "refill must be 1..64");
   if (grid < 0) return fail(c, CTRAJ_EINVAL, "grid < 0");
   HIPCHK(c, hipEventRecord(b->ev0, c->stream));
   int rc = CTRAJ_OK;
   if (use_graph) {
-    const auto key = std::make_tuple(block, chunk, grid, refill);
+    const auto key = std::make_tuple(block, chunk, grid, refill, fused ? 1 : 0);
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
       hipGraph_t graph = nullptr;
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      rc = batch_launch(b, block, chunk, grid, refill);
+      rc = batch_launch(b, block, chunk, grid, refill, fused);
       const hipError_t e = hipStreamEndCapture(c->stream, &graph);
       if (rc != CTRAJ_OK) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -417,7 +717,7 @@ int32_t ctraj_batch_run(ctraj_batch *b, const ctraj_batch_tune *tune, int64_t *t
     }
     HIPCHK(c, hipGraphLaunch(it->second, c->stream));
   } else {
-    rc = batch_launch(b, block, chunk, grid, refill);
+    rc = batch_launch(b, block, chunk, grid, refill, fused);
     if (rc != CTRAJ_OK) return rc;
   }
   HIPCHK(c, hipEventRecord(b->ev1, c->stream));
@@ -445,6 +745,9 @@ int32_t ctraj_batch_destroy(ctraj_batch *b) {
   if (b->d_counter) (void)hipFree(b->d_counter);
   if (b->d_tally) (void)hipFree(b->d_tally);
   if (b->d_ws) (void)hipFree(b->d_ws);
+  if (b->d_lensk) (void)hipFree(b->d_lensk);
+  if (b->d_pipe) (void)hipFree(b->d_pipe);
+  if (b->d_pipe_cnt) (void)hipFree(b->d_pipe_cnt);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   delete b;

@@ -42,6 +42,31 @@ __device__ __forceinline__ int lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// A lens-entry queue entry is 64 bytes, four 16-byte vectors: (x, y) (z, vx) (vy, vz) (t, key) with
+// key = molecule index within the run | lens state << 56 (the pipeline runs at most 2^27
+// molecules).  Each field is kept as its bit pattern.
+__device__ __forceinline__ void queue_put(longlong2 *q, int64_t e, const MState &m, int64_t i, int st) {
+  longlong2 *p = q + 4 * e;
+  p[0] = make_longlong2(__double_as_longlong(m.x), __double_as_longlong(m.y));
+  p[1] = make_longlong2(__double_as_longlong(m.z), __double_as_longlong(m.vx));
+  p[2] = make_longlong2(__double_as_longlong(m.vy), __double_as_longlong(m.vz));
+  p[3] = make_longlong2(__double_as_longlong(m.t), (long long)(i | ((int64_t)st << 56)));
+}
+
+__device__ __forceinline__ void queue_get(const longlong2 *q, int64_t e, MState &m, int64_t &i, int &st) {
+  const longlong2 *p = q + 4 * e;
+  const longlong2 a = p[0], b = p[1], c = p[2], d = p[3];
+  m.x = __longlong_as_double(a.x);
+  m.y = __longlong_as_double(a.y);
+  m.z = __longlong_as_double(b.x);
+  m.vx = __longlong_as_double(b.y);
+  m.vy = __longlong_as_double(c.x);
+  m.vz = __longlong_as_double(c.y);
+  m.t = __longlong_as_double(d.x);
+  i = (int64_t)d.y & (((int64_t)1 << 56) - 1);
+  st = (int)((uint64_t)d.y >> 56);
+}
+
 // Wave-aggregated histogram: one LDS atomic per distinct key in the wavefront (its lowest lane
 // adds the group's count).  The first molecule index of each label is an LDS atomic minimum per
 // lane, so lanes need not hold their molecules in index order (the queue-fed phases do not).

@@ -131,7 +131,7 @@ class BatchTune(C.Structure):
         ("grid", C.c_int32),
         ("refill", C.c_int32),
         ("use_graph", C.c_int32),
-        ("reserved", C.c_int32),
+        ("fused", C.c_int32),
     ]
 
 

@@ -76,7 +76,7 @@ class BatchScan:
         self._wspline = reg.wsplines[0] if reg.wsplines else None
         self.n_cfg = n_cfg
         self.total = int(counts.sum())
-        self.tune = N.BatchTune(block=256, chunk=128, grid=0, refill=16, use_graph=1)
+        self.tune = N.BatchTune(block=0, chunk=128, grid=0, refill=16, use_graph=1, fused=0)
 
     def run(self, tune: N.BatchTune | None = None, graph: bool | None = None):
         """One launch over every configuration.  Returns (tally [n_cfg][n_labels], work, device ms)."""
@@ -97,36 +97,41 @@ class BatchScan:
     def signature(self) -> str:
         return f"cfg{self.n_cfg}-n{self.total}-labels{len(self.labels)}"
 
-    def autotune(self, blocks=(64, 128, 256), chunks=(32, 64, 128, 256), refills=(8, 16, 32), reps: int = 2,
+    def autotune(self, blocks=(256, 512, 1024), chunks=(128,), refills=(8, 12, 16, 24), fused=(0,), reps: int = 2,
                  cache: str | Path | None = None, verbose: bool = False) -> N.BatchTune:
-        """Time every (block, chunk, refill) candidate (median of ``reps`` device-timed launches after
-        one warm-up) and keep the fastest as ``self.tune`` (graph replay on).  With ``cache`` (a JSON
-        file), a previous result for the same device and batch signature is reused."""
+        """Time every (block, chunk, refill, fused) candidate (median of ``reps`` device-timed launches
+        after one warm-up) and keep the fastest as ``self.tune`` (graph replay on).  ``fused`` lists the
+        paths to try: 0 the two phases (``block`` is the lens kernel's workgroup, up to 1024; ``chunk``
+        is unused), 1 the single-kernel walker (``block`` up to 256).  With ``cache`` (a JSON file), a
+        previous result for the same device and batch signature is reused."""
         key = f"{self.ctx.arch}:{self.signature()}"
         if cache is not None:
             p = Path(cache)
             if p.exists():
                 known = json.loads(p.read_text())
                 if key in known:
-                    b, c, r = known[key]["block"], known[key]["chunk"], known[key]["refill"]
-                    self.tune = N.BatchTune(block=b, chunk=c, grid=0, refill=r, use_graph=1)
+                    k = known[key]
+                    self.tune = N.BatchTune(block=k["block"], chunk=k["chunk"], grid=0, refill=k["refill"],
+                                            use_graph=1, fused=k.get("fused", 0))
                     return self.tune
         results = []
-        for b, c, r in itertools.product(blocks, chunks, refills):
-            t = N.BatchTune(block=b, chunk=c, grid=0, refill=r, use_graph=0)
+        for f, b, c, r in itertools.product(fused, blocks, chunks, refills):
+            if (f and b > 256) or (not f and c != chunks[0]):  # the two phases take no chunk
+                continue
+            t = N.BatchTune(block=b, chunk=c, grid=0, refill=r, use_graph=0, fused=f)
             self.run(t)
             times = sorted(self.run(t)[2] for _ in range(reps))
-            results.append((times[len(times) // 2], b, c, r))
+            results.append((times[len(times) // 2], b, c, r, f))
             if verbose:
-                print(f"block {b:3d} chunk {c:4d} refill {r:2d}: {times[len(times) // 2]:.3f} ms", flush=True)
+                print(f"fused {f} block {b:4d} chunk {c:4d} refill {r:2d}: {times[len(times) // 2]:.3f} ms", flush=True)
         results.sort()
-        _, b, c, r = results[0]
-        self.tune = N.BatchTune(block=b, chunk=c, grid=0, refill=r, use_graph=1)
+        _, b, c, r, f = results[0]
+        self.tune = N.BatchTune(block=b, chunk=c, grid=0, refill=r, use_graph=1, fused=f)
         self.tuning = results
         if cache is not None:
             p = Path(cache)
             known = json.loads(p.read_text()) if p.exists() else {}
-            known[key] = {"block": b, "chunk": c, "refill": r, "ms": results[0][0]}
+            known[key] = {"block": b, "chunk": c, "refill": r, "fused": f, "ms": results[0][0]}
             p.write_text(json.dumps(known, indent=1))
         return self.tune
 

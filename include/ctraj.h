@@ -234,13 +234,18 @@ typedef struct ctraj_batch_cfg {
   uint64_t seed;               /* device source stream of this configuration */
   double mean[6], sigma[6];    /* Gaussian source for x, y, z, vx, vy, vz */
 } ctraj_batch_cfg;
+/* A batch of at most 2^27 molecules with a lens runs in two phases, like ctraj_propagate: every
+ * molecule is drawn and walked to its lens entrance, the molecules entering a lens are queued (64 B
+ * each, batch-owned), then persistent wavefronts integrate the queue with lane refill.  Larger
+ * batches, or fused = 1, run one lane-refill kernel that draws, walks and integrates. */
 typedef struct ctraj_batch_tune {
-  int32_t block;     /* threads per block: 64, 128, 192 or 256 (0 = 256) */
-  int32_t chunk;     /* molecules a wavefront claims at a time from the global queue (0 = 128) */
+  int32_t block;     /* threads per block of the lens kernel: a multiple of 64 up to 1024 (0 = 1024); with
+                      * fused: 64, 128, 192 or 256 (0 = 256) */
+  int32_t chunk;     /* fused: molecules a wavefront claims at a time from the global queue (0 = 128) */
   int32_t grid;      /* persistent blocks (0 = as many as are resident) */
   int32_t refill;    /* refill a wavefront when at least this many lanes are idle, 1..64 (0 = 16) */
-  int32_t use_graph; /* 1: capture the launch (queue reset, tally reset, kernel) once as a hipGraph and replay it */
-  int32_t reserved;
+  int32_t use_graph; /* 1: capture the launches (counter and tally resets, kernels) once as a hipGraph and replay */
+  int32_t fused;     /* 1: the single-kernel walker instead of the two phases */
 } ctraj_batch_tune;
 /* host arrays; the context's W' spline (ctraj_set_wspline, state 0) at creation is copied into the batch and
  * shared by all configurations */

@@ -81,12 +81,24 @@ def test_graph_replays_match_eager_runs(ctx):
 def test_every_tuning_gives_the_same_tallies(ctx):
     bls, n, seeds, scan = small_scan(ctx)
     ref, _, _ = scan.run(graph=False)
-    best = scan.autotune(blocks=(64, 256), chunks=(32, 256), refills=(1, 16, 64), reps=1)
-    for _, b, c, r in scan.tuning:
-        t, _, _ = scan.run(scan.tune.__class__(block=b, chunk=c, grid=0, refill=r, use_graph=0))
+    best = scan.autotune(blocks=(64, 256, 1024), chunks=(32, 256), refills=(1, 16, 64), fused=(0, 1), reps=1)
+    assert {f for *_, f in scan.tuning} == {0, 1}
+    for _, b, c, r, f in scan.tuning:
+        t, w, _ = scan.run(scan.tune.__class__(block=b, chunk=c, grid=0, refill=r, use_graph=0, fused=f))
         np.testing.assert_array_equal(t, ref)
     t, _, _ = scan.run(best)
     np.testing.assert_array_equal(t, ref)
+
+
+def test_two_phase_and_single_kernel_work_counters_agree(ctx):
+    """Both batch paths walk the same faces and take the same RK4 steps."""
+    bls, n, seeds, scan = small_scan(ctx)
+    T = scan.tune.__class__
+    t2, w2, _ = scan.run(T(block=0, chunk=128, grid=0, refill=16, use_graph=0, fused=0))
+    t1, w1, _ = scan.run(T(block=256, chunk=128, grid=0, refill=16, use_graph=0, fused=1))
+    np.testing.assert_array_equal(t1, t2)
+    np.testing.assert_array_equal(w1[:2], w2[:2])
+    assert w1[1] > 0
 
 
 def test_config2_scan_runs_and_matches_sampled_configurations(ctx):
