@@ -196,7 +196,7 @@ def bench_batch(ctx, reps: int = 5, tune_reps: int = 2) -> dict:
     bls, n, seeds = configs.config2()
     scan = BatchScan(bls, n, seeds, configs.SOURCE_MEAN, configs.SOURCE_SIGMA, ctx=ctx)
     t0 = time.perf_counter()
-    best = scan.autotune(blocks=(256, 512, 1024), chunks=(128, 256), refills=(8, 12, 16, 24, 32), fused=(0, 1),
+    best = scan.autotune(blocks=(256, 512, 640, 1024), chunks=(128, 256), refills=(8, 12, 16, 24, 32), fused=(0, 1),
                          reps=tune_reps)
     t_tune = time.perf_counter() - t0
     scan.run(best)

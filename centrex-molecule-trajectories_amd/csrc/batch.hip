@@ -55,6 +55,7 @@ struct BArgs {
   int64_t total;
   double g;
   int n_cfg, n_labels, label_detected, cap, chunk, refill_min;
+  double Hs;                    // the lens dt shared by every configuration (two-phase lens kernel, UH)
   int off_begin, off_hist;      // in doubles from the LDS base (off_begin: the lens constants in the lens kernel)
 };
 
@@ -87,6 +88,7 @@ __device__ __forceinline__ int find_cfg(const int64_t *s_begin, int n_cfg, int64
 // Molecule j of configuration C from its Philox stream, walked through the configuration's
 // elements `el` before its lens and through the lens entrance.  Returns true if it enters the lens;
 // otherwise it has finished with label `lab`.
+template <int KM>
 __device__ __forceinline__ bool batch_start(const BCfg &C, const ctraj_elem *el, uint64_t j, double g,
                                             int label_detected, MState &m, int &lab, WalkCounters &wc) {
   double z[6];
@@ -104,7 +106,7 @@ __device__ __forceinline__ bool batch_start(const BCfg &C, const ctraj_elem *el,
   m.ay = -g;
   m.az = 0.0;
   lab = label_detected;
-  const bool ok = walk_plain(el, 0, C.lens >= 0 ? C.lens : C.ne, g, m, lab, wc);
+  const bool ok = walk_plain<KM>(el, 0, C.lens >= 0 ? C.lens : C.ne, g, m, lab, wc);
   if (!ok || C.lens < 0) return false;
   const ctraj_elem &L = el[C.lens];
   drift_dt(m, time_to(m, L.z0), g);
@@ -204,15 +206,40 @@ __global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
             const int64_t gi = next + rank;
             const int c = find_cfg(s_begin, A.n_cfg, gi);
             const BCfg &C = opaque(A.cfgs)[c];
+            const uint64_t j = (uint64_t)(gi - C.begin);
+            double z[6];  // the arithmetic of batch_start, written out: inlining the helper here costs registers
+            gaussian_pair(C.seed, j, 0u, z[0], z[1]);
+            gaussian_pair(C.seed, j, 1u, z[2], z[3]);
+            gaussian_pair(C.seed, j, 2u, z[4], z[5]);
             MState &m = s.m;
+            m.x = C.mean[0] + C.sigma[0] * z[0];
+            m.y = C.mean[1] + C.sigma[1] * z[1];
+            m.z = C.mean[2] + C.sigma[2] * z[2];
+            m.vx = C.mean[3] + C.sigma[3] * z[3];
+            m.vy = C.mean[4] + C.sigma[4] * z[4];
+            m.vz = C.mean[5] + C.sigma[5] * z[5];
+            m.t = 0.0;
+            m.ax = 0.0;
+            m.ay = -g;
+            m.az = 0.0;
             b.c = c;
             b.e0 = C.e0;
             b.ne = C.ne;
             b.lens = C.lens;
             s.status = 0;
             const ctraj_elem *el = opaque(A.elems) + b.e0;
-            int lab;
-            if (batch_start(C, el, (uint64_t)(gi - C.begin), g, A.label_detected, m, lab, wc)) {
+            int lab = A.label_detected;
+            bool ok = walk_plain(el, 0, b.lens >= 0 ? b.lens : b.ne, g, m, lab, wc);
+            if (ok && b.lens >= 0) {
+              const ctraj_elem &L = el[b.lens];
+              drift_dt(m, time_to(m, L.z0), g);
+              ++wc.faces;
+              if (rho_of(m.x, m.y) > L.p[0]) {
+                ok = false;
+                lab = L.label_hit;
+              }
+            }
+            if (ok && b.lens >= 0) {
               const ctraj_elem &L = el[b.lens];
               const double Hs = L.p[2], Ll = L.p[3];
               b.ks = L.p[4];
@@ -270,6 +297,7 @@ __global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
 // Two-phase scan, phase 1: every molecule is drawn and walked to and through its configuration's
 // lens entrance (grid-stride, wave-uniform trip count); the molecules that enter a lens are queued
 // in the wavefront's region, the others are tallied.
+template <int KM>
 __global__ void __launch_bounds__(256) k_batch_pre(BArgs A, BPipe P) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
   __shared__ unsigned long long s_work[4];
@@ -296,7 +324,7 @@ __global__ void __launch_bounds__(256) k_batch_pre(BArgs A, BPipe P) {
     if (active) {
       c = find_cfg(s_begin, A.n_cfg, gi);
       const BCfg &C = opaque(A.cfgs)[c];
-      enq = batch_start(C, opaque(A.elems) + C.e0, (uint64_t)(gi - C.begin), g, A.label_detected, m, lab, wc);
+      enq = batch_start<KM>(C, opaque(A.elems) + C.e0, (uint64_t)(gi - C.begin), g, A.label_detected, m, lab, wc);
     }
     const uint64_t qm = __ballot(enq);
     if (enq) queue_put(P.q, gwave * P.region + qn + lanes_below(qm), m, c, 0);
@@ -315,8 +343,11 @@ __global__ void __launch_bounds__(256) k_batch_pre(BArgs A, BPipe P) {
 // Two-phase scan, phase 2: persistent wavefronts integrate the queued molecules through their
 // configuration's lens with lane refill (the configuration's lens constants staged in LDS, its field
 // constants and step in the lane's registers), walk the survivors through the elements after the
-// lens and tally.  The arithmetic per molecule is the one of k_batch_lanes.
-__global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT) {
+// lens and tally.  The arithmetic per molecule is the one of k_batch_lanes.  UH: every configuration's
+// lens has the same dt, whose half and sixth come formed by the host (Hh, H6; otherwise unused).  KM:
+// the element kinds after the lenses (KM_CIRC keeps the walk's code and registers out).
+template <bool UH, int KM>
+__global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT, double Hh, double H6) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
   __shared__ unsigned long long s_work[4];
   const Tables T = stage_tables(lds, KT);
@@ -344,7 +375,7 @@ __global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT) 
   m.ax = 0.0; m.ay = -g; m.az = 0.0;
   int c = 0, ph = LN_IDLE, label = 0, i = 0, nfull = 0;
   uint32_t status = 0;
-  double ks = 0.0, cu = 0.0, R2 = 0.0, Hs = 0.0, dzs = 0.0, r2 = 0.0;
+  double ks = 0.0, cu = 0.0, R2 = 0.0, Hs = A.Hs, dzs = 0.0, r2 = 0.0;
 
   for (;;) {
     const uint64_t lensm = __ballot(ph == LN_LENS);
@@ -356,7 +387,7 @@ __global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT) 
           m.az = 0.0;
           const BLensC &K = s_cfg[c];
           label = A.label_detected;
-          walk_plain(opaque(A.elems) + K.e0, K.lens + 1, K.ne, g, m, label, wc);
+          walk_plain<KM>(opaque(A.elems) + K.e0, K.lens + 1, K.ne, g, m, label, wc);
           ph = LN_FIN;
         }
         if (__ballot(ph == LN_FIN)) {
@@ -402,7 +433,7 @@ __global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT) 
             ks = K.ks;
             cu = K.cu;
             R2 = K.R2;
-            Hs = K.Hs;
+            if constexpr (!UH) Hs = K.Hs;
             m.ax = 0.0;
             m.ay = -g;
             m.az = 0.0;
@@ -427,7 +458,8 @@ __global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT) 
       const Force<CTRAJ_FORCE_WSPLINE_HEX> F{ks, cu, g};
       const bool last = (i == nfull);
       if (!last) {
-        rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, Hs, m.x, m.y, m.vx, m.vy, r2, status);
+        if constexpr (UH) rk4_step_hh<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, Hs, Hh, H6, m.x, m.y, m.vx, m.vy, r2, status);
+        else rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, Hs, m.x, m.y, m.vx, m.vy, r2, status);
         m.z = m.z + dzs;
         m.t = m.t + Hs;
       } else {  // the final partial step, formed here, once per molecule that reaches the lens end
@@ -477,6 +509,9 @@ struct ctraj_batch {
   void *d_pipe = nullptr, *d_pipe_cnt = nullptr;
   int pre_grid = 0, n_pre_waves = 0;
   int64_t region = 0;
+  bool uniform_dt = false;  // every lens has the same dt
+  double dt = 0.0;
+  bool circ_only = false;   // only circular apertures (and lenses) in every configuration
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // captured launches by (block, chunk, grid, refill, fused)
   std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> graphs;
@@ -533,24 +568,33 @@ static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refi
   P.n_pre_waves = b->n_pre_waves;
   // phase 1: LDS = configuration offsets + histogram
   const size_t lds_pre = (size_t)b->n_cfg * sizeof(int64_t) + hist_bytes;
-  if ((rc = lds_limit(c, (const void *)k_batch_pre, lds_pre)) != CTRAJ_OK) return rc;
+  const void *kp = b->circ_only ? (const void *)k_batch_pre<KM_CIRC> : (const void *)k_batch_pre<KM_ALL>;
+  if ((rc = lds_limit(c, kp, lds_pre)) != CTRAJ_OK) return rc;
   // phase 2: LDS = tables + lens constants + histogram
   const size_t cfg_dbl = (size_t)b->n_cfg * (sizeof(BLensC) / sizeof(double));
   A.off_begin = (int)tab;
   A.off_hist = (int)(tab + cfg_dbl);
   const size_t lds_lens = (tab + cfg_dbl) * sizeof(double) + hist_bytes;
-  if ((rc = lds_limit(c, (const void *)k_batch_lens, lds_lens)) != CTRAJ_OK) return rc;
-  if ((rc = block_ok(c, (const void *)k_batch_lens, block)) != CTRAJ_OK) return rc;
-  if (grid <= 0) grid = grid_for(c, (const void *)k_batch_lens, block, lds_lens, (int64_t)1 << 40, 0);
+  const bool circ = b->circ_only;
+  const void *kl = b->uniform_dt ? (circ ? (const void *)k_batch_lens<true, KM_CIRC> : (const void *)k_batch_lens<true, KM_ALL>)
+                                 : (circ ? (const void *)k_batch_lens<false, KM_CIRC> : (const void *)k_batch_lens<false, KM_ALL>);
+  A.Hs = b->dt;
+  if ((rc = lds_limit(c, kl, lds_lens)) != CTRAJ_OK) return rc;
+  if ((rc = block_ok(c, kl, block)) != CTRAJ_OK) return rc;
+  if (grid <= 0) grid = grid_for(c, kl, block, lds_lens, (int64_t)1 << 40, 0);
   // queue counters and tallies are reset by fill kernels (no memset nodes in the captured graph)
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, c->stream, (unsigned long long *)b->d_pipe_cnt,
                      (int64_t)8 * 32);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, b->d_counter, nz);
   hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<int64_t>((nt + 255) / 256, 1024)), dim3(256), 0, c->stream,
                      (unsigned long long *)b->d_tally, nt);
-  hipLaunchKernelGGL(k_batch_pre, dim3(b->pre_grid), dim3(256), lds_pre, c->stream, A, P);
+  using PreK = void (*)(BArgs, BPipe);
+  hipLaunchKernelGGL((PreK)kp, dim3(b->pre_grid), dim3(256), lds_pre, c->stream, A, P);
   HIPCHK(c, hipGetLastError());
-  hipLaunchKernelGGL(k_batch_lens, dim3(grid), dim3(block), lds_lens, c->stream, A, P, kt);
+  // the RK4 half and sixth steps, rounded as rk4_step forms them
+  const double Hh = 0.5 * b->dt, H6 = b->dt * (1.0 / 6.0);
+  using LensK = void (*)(BArgs, BPipe, KTab, double, double);
+  hipLaunchKernelGGL((LensK)kl, dim3(grid), dim3(block), lds_lens, c->stream, A, P, kt, Hh, H6);
   HIPCHK(c, hipGetLastError());
   return CTRAJ_OK;
 }
@@ -630,6 +674,11 @@ int32_t ctraj_batch_create(ctraj_ctx *c, const ctraj_elem *elems, int32_t n_elem
   }
   // two-phase path: lens constants per configuration and a lens-entry queue sized for the batch
   std::vector<BLensC> hl(n_cfgs);
+  b->uniform_dt = true;
+  bool have_dt = false;
+  b->circ_only = true;
+  for (int e = 0; e < n_elems_total; ++e)
+    if (elems[e].kind != CTRAJ_CIRC && elems[e].kind != CTRAJ_LENS) b->circ_only = false;
   for (int k = 0; k < n_cfgs; ++k) {
     BLensC &K = hl[k];
     K = BLensC{};
@@ -645,11 +694,18 @@ int32_t ctraj_batch_create(ctraj_ctx *c, const ctraj_elem *elems, int32_t n_elem
       K.Hs = L.p[2];
       K.Ll = L.p[3];
       K.label_inside = L.label_inside;
+      if (!have_dt) {
+        b->dt = K.Hs;
+        have_dt = true;
+      } else if (K.Hs != b->dt) {
+        b->uniform_dt = false;
+      }
     }
   }
   if (any_lens && total > 0 && total <= kPipeMax) {
     const size_t lds_pre = (size_t)n_cfgs * sizeof(int64_t) + nt * sizeof(uint32_t);
-    int gpre = grid_for(c, (const void *)k_batch_pre, 256, lds_pre, total, 0);
+    const void *kp = b->circ_only ? (const void *)k_batch_pre<KM_CIRC> : (const void *)k_batch_pre<KM_ALL>;
+    int gpre = grid_for(c, kp, 256, lds_pre, total, 0);
     gpre = (gpre + 7) & ~7;  // whole groups of eight workgroups: one queue-region group per XCD
     const int64_t threads = (int64_t)gpre * 256;
     b->pre_grid = gpre;
