@@ -724,8 +724,11 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
 // ---------------------------------------------------------------------------------------------
 // k_record: re-run selected molecules and store their rows (or z-grid samples)
 // ---------------------------------------------------------------------------------------------
-template <int FORCE, int INTEG, int MODE>
-__global__ void __launch_bounds__(256) k_record(KBeam B, KTab KT, ctraj_record RC) {
+// Workgroups of up to 512 threads for the circular-aperture-only instantiations (two wavefronts per
+// SIMD fit their registers, and one workgroup per CU is all that several lens-state tables leave
+// room for in LDS); 256 for the general ones, whose z-grid form needs more registers.
+template <int FORCE, int INTEG, int MODE, int KM>
+__global__ void __launch_bounds__(KM == KM_CIRC ? 512 : 256) k_record(KBeam B, KTab KT, ctraj_record RC) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
   const Tables T = stage_tables(lds, KT);
   __syncthreads();
@@ -749,15 +752,15 @@ __global__ void __launch_bounds__(256) k_record(KBeam B, KTab KT, ctraj_record R
     WalkCounters wc{0u, 0u};
     if constexpr (MODE == CTRAJ_REC_ROWS) {
       RowSink sk{RC.buf, RC.n, s, RC.cap, 0};
-      walk<FORCE, INTEG>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status, wc, sk);
+      walk<FORCE, INTEG, decltype(sk), KM>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status, wc, sk);
       sk.finish(m);
       if (RC.n_rows) RC.n_rows[s] = sk.count;
     } else {
-      ZGridSink sk;
+      ZGridSinkT<MODE == CTRAJ_REC_ZGRID_MOL> sk;
       sk.buf = RC.buf; sk.n = RC.n; sk.slot = s; sk.cap = RC.cap;
       sk.z_start = RC.z_start; sk.z_step = RC.z_step;
       sk.init();
-      walk<FORCE, INTEG>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status, wc, sk);
+      walk<FORCE, INTEG, decltype(sk), KM>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status, wc, sk);
       sk.finish(m);
       if (RC.n_rows) RC.n_rows[s] = sk.written;
     }
@@ -1081,11 +1084,16 @@ static LaneFn lane_kernel(int f, int i) {
   return nullptr;
 }
 
-static RecFn rec_kernel(int f, int i, int mode) {
-#define X(F, I)                                                             \
-  if (f == F && i == I) {                                                  \
-    return mode == CTRAJ_REC_ROWS ? k_record<F, I, CTRAJ_REC_ROWS> : k_record<F, I, CTRAJ_REC_ZGRID>; \
-  }
+template <int F, int I, int KM>
+static RecFn rec_kernel_km(int mode) {
+  if (mode == CTRAJ_REC_ROWS) return k_record<F, I, CTRAJ_REC_ROWS, KM>;
+  if (mode == CTRAJ_REC_ZGRID) return k_record<F, I, CTRAJ_REC_ZGRID, KM>;
+  return k_record<F, I, CTRAJ_REC_ZGRID_MOL, KM>;
+}
+
+static RecFn rec_kernel(int f, int i, int mode, int km) {
+#define X(F, I)                                                                                       \
+  if (f == F && i == I) return km == KM_CIRC ? rec_kernel_km<F, I, KM_CIRC>(mode) : rec_kernel_km<F, I, KM_ALL>(mode);
   CT_FI_LIST(X)
 #undef X
   return nullptr;
@@ -1450,16 +1458,19 @@ int32_t ctraj_record_trajectories(ctraj_ctx *c, const ctraj_record *RC) {
   if (RC->n == 0) return CTRAJ_OK;
   if (!RC->in.x || !RC->in.y || !RC->in.vx || !RC->in.vy || !RC->in.vz || !RC->buf)
     return fail(c, CTRAJ_EINVAL, "missing inputs or buffer");
-  if (RC->mode != CTRAJ_REC_ROWS && RC->mode != CTRAJ_REC_ZGRID) return fail(c, CTRAJ_EINVAL, "bad record mode");
-  if (RC->mode == CTRAJ_REC_ZGRID && !(RC->z_step > 0.0)) return fail(c, CTRAJ_EINVAL, "z_step must be > 0");
-  const int block = RC->block ? RC->block : 128;
-  if (block < 64 || block > 256 || block % 64) return fail(c, CTRAJ_EINVAL, "block must be 64..256, multiple of 64");
+  if (RC->mode != CTRAJ_REC_ROWS && RC->mode != CTRAJ_REC_ZGRID && RC->mode != CTRAJ_REC_ZGRID_MOL)
+    return fail(c, CTRAJ_EINVAL, "bad record mode");
+  if (RC->mode != CTRAJ_REC_ROWS && !(RC->z_step > 0.0)) return fail(c, CTRAJ_EINVAL, "z_step must be > 0");
   int rc0 = check_tables(c, c->h_elems.data(), c->n_elems, c->phys.n_states);
   if (rc0) return rc0;
-  RecFn k = rec_kernel(c->force, c->integ, RC->mode);
+  const int km = kinds_mask(c->h_elems, 0, c->n_elems);
+  RecFn k = rec_kernel(c->force, c->integ, RC->mode, km);
   KTab kt = ctx_ktab(c);
   if (c->force == CTRAJ_FORCE_RTABLE) kt.ws_states = 0;
   const size_t lds = ktab_lds_bytes(kt);
+  // one large workgroup per CU when the tables fill LDS (several lens states), else 128 threads
+  const int block = RC->block ? RC->block : (RC->mode == CTRAJ_REC_ZGRID_MOL ? 256 : 128);
+  if (block < 64 || block % 64) return fail(c, CTRAJ_EINVAL, "block must be a positive multiple of 64");
   int rc = lds_limit(c, (const void *)k, lds);
   if (rc) return rc;
   rc = block_ok(c, (const void *)k, block);

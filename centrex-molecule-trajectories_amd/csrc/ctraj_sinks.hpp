@@ -40,8 +40,12 @@ struct RowSink {
 // takes index -1, i.e. the last row (:61).  Streaming form: a sample is decided at the first row
 // with z >= z_k; if that row is past z_k the decision waits one more row, because a zero-length
 // face step (an L = 0 aperture) can land the next row back on z_k exactly.
-// buf[f][k][slot]: x, y, vx, vy, vz, t; every sample is written (NaN where the rule gives none).
-struct ZGridSink {
+// buf[f][k][slot] (MOL false) or buf[slot][k][f] (MOL true): x, y, vx, vy, vz, t; every sample is
+// written (NaN where the rule gives none).  In the molecule-major layout a lane writes one
+// contiguous stream (48 B per sample, three 16-byte stores), so partial cache lines are completed
+// by the same lane's next samples however far the lanes of a wavefront drift apart in z.
+template <bool MOL>
+struct ZGridSinkT {
   static constexpr bool kRows = true;
   double *buf;
   int64_t n, slot;
@@ -54,20 +58,31 @@ struct ZGridSink {
   bool have_prev;
   MState prev, pend_row;
   __device__ __forceinline__ double zk(int i) const { return z_start + (double)i * z_step; }
-  __device__ __forceinline__ void put(int f, int i, double v) { buf[((size_t)f * cap + i) * n + slot] = v; }
+  __device__ __forceinline__ void put6(int i, double x, double y, double vx, double vy, double vz, double t) {
+    if constexpr (MOL) {
+      double2 *q = reinterpret_cast<double2 *>(buf + ((size_t)slot * cap + i) * 6);
+      q[0] = make_double2(x, y);
+      q[1] = make_double2(vx, vy);
+      q[2] = make_double2(vz, t);
+    } else {
+      const size_t s0 = (size_t)i * n + slot, fs = (size_t)cap * n;
+      buf[s0] = x; buf[s0 + fs] = y; buf[s0 + 2 * fs] = vx;
+      buf[s0 + 3 * fs] = vy; buf[s0 + 4 * fs] = vz; buf[s0 + 5 * fs] = t;
+    }
+  }
+  __device__ __forceinline__ void put_nan(int i) {
+    const double q = nan64();
+    put6(i, q, q, q, q, q, q);
+  }
   __device__ __forceinline__ void exact(int i, const MState &m) {
-    put(0, i, m.x); put(1, i, m.y); put(2, i, m.vx); put(3, i, m.vy); put(4, i, m.vz); put(5, i, m.t);
+    put6(i, m.x, m.y, m.vx, m.vy, m.vz, m.t);
     ++written;
   }
   __device__ __forceinline__ void extrap(int i, const MState &p) {
     const double dt = (zk(i) - p.z) / p.vz;
     const double dt2 = dt * dt;
-    put(0, i, (p.x + p.vx * dt) + (p.ax * dt2) / 2.0);
-    put(1, i, (p.y + p.vy * dt) + (p.ay * dt2) / 2.0);
-    put(2, i, p.vx + p.ax * dt);
-    put(3, i, p.vy + p.ay * dt);
-    put(4, i, p.vz + p.az * dt);
-    put(5, i, p.t + dt);
+    put6(i, (p.x + p.vx * dt) + (p.ax * dt2) / 2.0, (p.y + p.vy * dt) + (p.ay * dt2) / 2.0, p.vx + p.ax * dt,
+         p.vy + p.ay * dt, p.vz + p.az * dt, p.t + dt);
     ++written;
   }
   __device__ __forceinline__ void init() {
@@ -101,20 +116,20 @@ struct ZGridSink {
     pend_lo = pend_hi = k;
     // post_processing.py:42: nothing beyond the last row's z
     for (int i = k - 1; i >= kdefer && zk(i) > last.z; --i) {
-      for (int f = 0; f < 6; ++f) put(f, i, nan64());
+      put_nan(i);
       --written;
     }
     for (int i = 0; i < kdefer; ++i) {
       if (last.z < zk(i)) {
-        for (int f = 0; f < 6; ++f) put(f, i, nan64());
+        put_nan(i);
         continue;
       }
       extrap(i, last);
     }
     // samples past the last row's z were never reached
-    for (int i = k; i < cap; ++i)
-      for (int f = 0; f < 6; ++f) put(f, i, nan64());
+    for (int i = k; i < cap; ++i) put_nan(i);
   }
 };
+using ZGridSink = ZGridSinkT<false>;
 
 }  // namespace ctraj

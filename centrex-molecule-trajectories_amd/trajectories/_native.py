@@ -27,7 +27,7 @@ INTEG_REF, INTEG_RK4 = 0, 1
 ST_OOT, ST_CAPPED = 1, 2
 WORK_FACES, WORK_STEPS, WORK_OOT, WORK_CAPPED, WORK_N = 0, 1, 2, 3, 8
 RUN_STATUS_OR, RUN_SIMPLE, RUN_LANES = 1, 2, 4
-REC_ROWS, REC_ZGRID = 1, 2
+REC_ROWS, REC_ZGRID, REC_ZGRID_MOL = 1, 2, 3
 H2D, D2H, D2D = 1, 2, 3
 
 dptr = C.c_void_p

@@ -173,7 +173,10 @@ class Engine:
             N.check("ctraj_record_trajectories", N.lib().ctraj_record_trajectories(ctx.handle, C.byref(Rc)),
                     ctx.handle)
         ctx.sync()
-        data = buf.to_host()[: nf * cap * n].reshape(nf, cap, n)
+        if mode == N.REC_ZGRID_MOL:
+            data = buf.to_host()[: nf * cap * n].reshape(n, cap, nf)
+        else:
+            data = buf.to_host()[: nf * cap * n].reshape(nf, cap, n)
         return data, nrows.to_host()[:n], lab.to_host()[:n]
 
     def record_rows(self, x, y, z, vx, vy, vz, t=None, a0=None, state=None, cap: int = 64):
@@ -193,6 +196,12 @@ class Engine:
         """Samples at z_k = z_start + k z_step by the post-processing rule: [6][n_samples][n]
         (x, y, vx, vy, vz, t; NaN where the molecule does not reach z_k), written count, labels."""
         return self._record(N.REC_ZGRID, x, y, z, vx, vy, vz, t, a0, state, n_samples, z_start, z_step)
+
+    def zgrid_by_molecule(self, x, y, z, vx, vy, vz, t=None, a0=None, state=None, *, z_start: float = 0.0,
+                          z_step: float = 1e-3, n_samples: int = 2501):
+        """The samples of ``zgrid`` laid out molecule-major, [n][n_samples][6] (each molecule's samples
+        contiguous on the device), written count, labels."""
+        return self._record(N.REC_ZGRID_MOL, x, y, z, vx, vy, vz, t, a0, state, n_samples, z_start, z_step)
 
     def draw_gaussian(self, n: int, mean, sigma, seed: int, offset: int = 0) -> list:
         """Device Gaussian source (Philox4x32-10 + Box-Muller): list of DeviceArrays, one per dimension."""

@@ -11,7 +11,8 @@ propagate (per-state, per-element tallies and the global indices of detected mol
 accumulate on the device; the host never waits between chunks).
 Pass 2: sort the detected indices (radix sort), regenerate those molecules' inputs from the same
 counter-based streams, and re-walk them in one record launch that writes their samples
-[6][n_samples][n_detected] (x, y, vx, vy, vz, t), molecule-minor so that writes are coalesced.
+[n_detected][n_samples][6] (x, y, vx, vy, vz, t): each molecule's samples contiguous, so that every
+lane writes one sequential stream (the lanes of a wavefront soon sample different z).
 """
 from __future__ import annotations
 
@@ -39,7 +40,7 @@ class StreamResult:
     n_samples: int
     z_start: float
     z_step: float
-    trajectories: DeviceArray | None  # [6][n_samples][n_detected] on the device, or None
+    trajectories: DeviceArray | None  # [n_detected][n_samples][6] on the device, or None
     samples_written: np.ndarray | None
     timings: dict = field(default_factory=dict)
     work: np.ndarray | None = None
@@ -58,13 +59,12 @@ class StreamResult:
         nd, ns = self.n_detected, self.n_samples
         if js.size and (js.min() < 0 or js.max() >= nd):
             raise IndexError("detected molecule out of range")
-        f = np.arange(6)[None, :, None]
-        k = np.arange(ns)[None, None, :]
-        idx = np.ascontiguousarray(((f * ns + k) * nd + js[:, None, None]).reshape(-1))
+        e = np.arange(ns * 6)[None, :]
+        idx = np.ascontiguousarray((js[:, None] * (ns * 6) + e).reshape(-1))
         d_idx = ctx.to_device(idx)
         out = ctx.alloc(max(idx.size, 1))
         gather(ctx, idx.size, d_idx, 0, 6 * ns * nd, [self.trajectories], [out])
-        return out.to_host()[: idx.size].reshape(js.size, 6, ns).transpose(0, 2, 1)
+        return out.to_host()[: idx.size].reshape(js.size, ns, 6)
 
 
 class StreamedRun:
@@ -137,7 +137,7 @@ class StreamedRun:
                 R.inp = _soa(dict(zip(FIELDS6, init)))
                 R.z_const, R.t_const = 0.0, 0.0
                 R.state = init_state.ptr
-                R.mode, R.cap = N.REC_ZGRID, self.n_samples
+                R.mode, R.cap = N.REC_ZGRID_MOL, self.n_samples
                 R.z_start, R.z_step = self.z_start, self.z_step
                 R.buf, R.n_rows, R.label = traj.ptr, written.ptr, None
                 bind(ctx, cb)

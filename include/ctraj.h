@@ -173,6 +173,7 @@ int32_t ctraj_propagate(ctraj_ctx *ctx, const ctraj_run *run);
 /* ---- saved trajectories (molecule.py:111-167 rows; post_processing.py:20-140 resampling) ---- */
 #define CTRAJ_REC_ROWS 1  /* every trajectory row; buf = [10][cap][n]: x,y,z,vx,vy,vz,ax,ay,az,t */
 #define CTRAJ_REC_ZGRID 2 /* samples at z_k = z_start + k*z_step; buf = [6][cap][n]: x,y,vx,vy,vz,t */
+#define CTRAJ_REC_ZGRID_MOL 3 /* the same samples molecule-major: buf = [n][cap][6], each molecule contiguous */
 typedef struct ctraj_record {
   int64_t n;              /* molecules to record */
   const int64_t *index;   /* indices into `in` (NULL -> 0..n-1) */
@@ -186,7 +187,8 @@ typedef struct ctraj_record {
   double *buf;            /* see mode; unwritten entries are NaN */
   int32_t *n_rows;        /* [n] rows produced (ROWS; may exceed cap) or samples written (ZGRID) */
   int32_t *label;         /* [n] final label (optional) */
-  int32_t block;
+  int32_t block;          /* threads per block, multiple of 64: up to 512 for beamlines of circular apertures and
+                           * lenses, else 256 (0 = 256 for ZGRID_MOL, else 128) */
   int32_t reserved;
 } ctraj_record;
 int32_t ctraj_record_trajectories(ctraj_ctx *ctx, const ctraj_record *rec);

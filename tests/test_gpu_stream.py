@@ -92,3 +92,20 @@ def test_streamed_run_equals_one_propagation_and_the_oracle(ctx):
         same = (got[q] == want) | (np.isnan(got[q]) & np.isnan(want))
         assert same.all(), (j, np.argwhere(~same)[:5])
     assert np.all(res.samples_written[js] > 2400)
+
+
+def test_molecule_major_zgrid_is_the_field_major_transposed(ctx):
+    """CTRAJ_REC_ZGRID_MOL writes the same samples as CTRAJ_REC_ZGRID, [n][k][f] instead of [f][k][n],
+    NaN entries included; a 4-state beamline with mixed states."""
+    n = 3000
+    cols = [a.to_host() for a in draw_gaussian(ctx, n, configs.SOURCE_MEAN, configs.SOURCE_SIGMA, 21)]
+    st = draw_uniform_u8(ctx, n, 4, 21).to_host()
+    eng = Engine(configs.config4(), ctx, n_states=4)
+    fm, w1, l1 = eng.zgrid(*cols, state=st, n_samples=2501)
+    mm, w2, l2 = eng.zgrid_by_molecule(*cols, state=st, n_samples=2501)
+    np.testing.assert_array_equal(w1, w2)
+    np.testing.assert_array_equal(l1, l2)
+    want = fm.transpose(2, 1, 0)
+    same = (mm == want) | (np.isnan(mm) & np.isnan(want))
+    assert same.all()
+    assert np.isfinite(mm).any() and np.isnan(mm).any()
