@@ -721,11 +721,15 @@ int32_t ctraj_batch_create(ctraj_ctx *c, const ctraj_elem *elems, int32_t n_elem
       b->d_pipe = nullptr;
     }
   }
+  // the device copy of the elements: circular apertures carry their squared-radius threshold
+  std::vector<ctraj_elem> dev_elems(elems, elems + n_elems_total);
+  for (auto &E : dev_elems)
+    if (E.kind == CTRAJ_CIRC) E.p[1] = circ_r2_threshold(E.p[0]);
   if (hipStreamSynchronize(c->stream) != hipSuccess ||
       (b->d_lensk && hipMemcpy(b->d_lensk, hl.data(), sizeof(BLensC) * n_cfgs, hipMemcpyHostToDevice) != hipSuccess) ||
       (b->d_ws && hipMemcpy(b->d_ws, c->d_ws, sizeof(double) * 4 * b->ws_nseg, hipMemcpyDeviceToDevice) != hipSuccess) ||
       hipMemcpy(b->d_cfgs, hc.data(), sizeof(BCfg) * n_cfgs, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->d_elems, elems, sizeof(ctraj_elem) * n_elems_total, hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(b->d_elems, dev_elems.data(), sizeof(ctraj_elem) * n_elems_total, hipMemcpyHostToDevice) != hipSuccess) {
     const hipError_t e = hipGetLastError();
     ctraj_batch_destroy(b);
     return hipfail(c, e, "batch upload");

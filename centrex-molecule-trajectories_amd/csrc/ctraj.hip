@@ -939,9 +939,23 @@ int check_tables(ctraj_ctx *ctx, const ctraj_elem *e, int n, int n_states) {
 
 // Device copy of an element table: an RTABLE lens carries its table's knot offset (table),
 // knot count (reserved) and inverse knot spacing (p[6]) instead of the table id.
+double circ_r2_threshold(double d) {
+  if (std::isnan(d) || std::isinf(d)) return d;  // NaN: never beyond; +inf: never; -inf: always
+  if (d < 0.0) return -1.0;                        // x^2 + y^2 >= 0 > -1: always beyond (NaN r2 never)
+  double t = d * d;
+  while (std::sqrt(t) > d) t = std::nextafter(t, -HUGE_VAL);
+  for (;;) {
+    const double nx = std::nextafter(t, HUGE_VAL);
+    if (std::isinf(nx) || std::sqrt(nx) > d) break;
+    t = nx;
+  }
+  return t;
+}
+
 void resolve_elems(const ctraj_ctx *ctx, const ctraj_elem *e, int n, ctraj_elem *out) {
   for (int i = 0; i < n; ++i) {
     out[i] = e[i];
+    if (e[i].kind == CTRAJ_CIRC) out[i].p[1] = circ_r2_threshold(e[i].p[0]);
     if (e[i].kind == CTRAJ_LENS && e[i].force == CTRAJ_FORCE_RTABLE) {
       const int id = e[i].table;
       out[i].table = ctx->rt_off[id];

@@ -80,6 +80,11 @@ __device__ __forceinline__ double time_to(const MState &m, double zf) { return (
 
 __device__ __forceinline__ double rho_of(double x, double y) { return __builtin_sqrt(x * x + y * y); }
 
+// rho_of(x, y) > d without the square root: the host stores thr2 = the largest double whose
+// correctly rounded square root is <= d (circ_r2_threshold), and sqrt is monotonic, so
+// sqrt(r2) > d exactly when r2 > thr2 (also for NaN, infinite and negative d).
+__device__ __forceinline__ bool beyond_r2(double x, double y, double thr2) { return x * x + y * y > thr2; }
+
 // ---------------------------------------------------------------------------------------------
 // Lens forces.  Each returns the acceleration (ax, ay) at transverse position (x, y); a_z is 0.
 // ---------------------------------------------------------------------------------------------
@@ -450,16 +455,17 @@ __device__ __forceinline__ void walk(const ctraj_elem *__restrict__ elems, int n
   for (int e = 0; e < n_elems; ++e) {
     const ctraj_elem E = elems[e];
     if (E.kind == CTRAJ_CIRC) {
-      // CircularAperture.propagate_through (apertures.py:92-115): test at z0 and z1; x0, y0 unused
-      const double half_d = E.p[0];
+      // CircularAperture.propagate_through (apertures.py:92-115): test at z0 and z1; x0, y0 unused.
+      // p[1] holds the squared-radius threshold of d/2 (beyond_r2)
+      const double thr2 = E.p[1];
       drift_dt(m, time_to(m, E.z0), g);
       ++wc.faces;
       if constexpr (Sink::kRows) sink.row(m);
-      if (rho_of(m.x, m.y) > half_d) { label = E.label_hit; return; }
+      if (beyond_r2(m.x, m.y, thr2)) { label = E.label_hit; return; }
       drift_dt(m, time_to(m, E.z1), g);
       ++wc.faces;
       if constexpr (Sink::kRows) sink.row(m);
-      if (rho_of(m.x, m.y) > half_d) { label = E.label_hit; return; }
+      if (beyond_r2(m.x, m.y, thr2)) { label = E.label_hit; return; }
     } else if ((KM & KM_RECT) && E.kind == CTRAJ_RECT) {
       // RectangularAperture.propagate_through (apertures.py:165-189): strict box test
       const double x1 = E.p[0], x2 = E.p[1], y1 = E.p[2], y2 = E.p[3];

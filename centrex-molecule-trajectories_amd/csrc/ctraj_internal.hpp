@@ -74,6 +74,9 @@ size_t ktab_lds_bytes(const KTab &k);
 int check_elems(ctraj_ctx *ctx, const ctraj_elem *e, int n, const ctraj_phys *p, int *force, int *integ);
 int check_tables(ctraj_ctx *ctx, const ctraj_elem *e, int n, int n_states);
 void resolve_elems(const ctraj_ctx *ctx, const ctraj_elem *e, int n, ctraj_elem *out);
+// The largest double whose correctly rounded square root is <= d: for r2 >= 0, sqrt(r2) > d exactly
+// when r2 > circ_r2_threshold(d).  Device element tables keep it in a circular aperture's p[1].
+double circ_r2_threshold(double d);
 int check_phys(ctraj_ctx *ctx, const ctraj_phys *p);
 KBeam make_kbeam(const ctraj_elem *d_elems, int n, const ctraj_phys &p);
 int lds_limit(ctraj_ctx *ctx, const void *kern, size_t bytes);

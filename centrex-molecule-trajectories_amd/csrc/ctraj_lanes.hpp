@@ -43,13 +43,13 @@ __device__ __forceinline__ bool walk_plain(const ctraj_elem *__restrict__ elems,
   for (int e = e0; e < e1; ++e) {
     const ctraj_elem E = elems[e];
     if (E.kind == CTRAJ_CIRC) {
-      const double half_d = E.p[0];
+      const double thr2 = E.p[1];  // the squared-radius threshold of d/2 (beyond_r2)
       drift_dt(m, time_to(m, E.z0), g);
       ++wc.faces;
-      if (rho_of(m.x, m.y) > half_d) { label = E.label_hit; return false; }
+      if (beyond_r2(m.x, m.y, thr2)) { label = E.label_hit; return false; }
       drift_dt(m, time_to(m, E.z1), g);
       ++wc.faces;
-      if (rho_of(m.x, m.y) > half_d) { label = E.label_hit; return false; }
+      if (beyond_r2(m.x, m.y, thr2)) { label = E.label_hit; return false; }
     } else if ((KM & KM_RECT) && E.kind == CTRAJ_RECT) {
       const double x1 = E.p[0], x2 = E.p[1], y1 = E.p[2], y2 = E.p[3];
       drift_dt(m, time_to(m, E.z0), g);

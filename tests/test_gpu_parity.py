@@ -266,3 +266,35 @@ def test_small_and_empty_runs():
             np.testing.assert_array_equal(dev["label"], orc["label"])
             for k in FIELDS:
                 np.testing.assert_array_equal(dev["out"][k], orc["out"][k])
+
+
+def test_aperture_edge_decisions_bitwise():
+    """Molecules that reach circular apertures at radii within a few ulps of d/2 (and at d/2 exactly,
+    along x, y and the diagonal): every stop/pass decision equals the oracle's sqrt(x^2 + y^2) > d/2,
+    with the device's squared-radius thresholds.  Radii chosen so that x^2 + y^2 rounds onto both
+    sides of the threshold."""
+    halves = [3e-3, 4e-3, 5e-3, 6e-3, 1e-3 / 3, 0.0123456789, 2.5e-3 * (1 + 2 ** -40)]
+    xs, ys, ds = [], [], []
+    for d in halves:
+        for k in range(-6, 7):
+            r = d
+            for _ in range(abs(k)):
+                r = np.nextafter(r, np.inf if k > 0 else -np.inf)
+            xs += [r, 0.0, r / np.sqrt(2.0), -r]
+            ys += [0.0, r, r / np.sqrt(2.0), 0.0]
+            ds += [d] * 4
+    xs, ys, ds = np.array(xs), np.array(ys), np.array(ds)
+    n_per = xs.size // len(halves)
+    for j, d in enumerate(halves):
+        sl = slice(j * n_per, (j + 1) * n_per)
+        bl = Beamline([CircularAperture("Ap", 0.5, 0.0, d=2 * d)], configs.G)
+        n = n_per
+        # start on the aperture plane with no transverse motion, so the tested radius is the start's
+        X = np.array([xs[sl], ys[sl], np.full(n, 0.5)])
+        V = np.array([np.zeros(n), np.zeros(n), np.full(n, 184.0)])
+        eng, dev, orc = run_both(bl, X, V)
+        np.testing.assert_array_equal(dev["label"], orc["label"])
+        beyond = np.sqrt(xs[sl] ** 2 + ys[sl] ** 2) > d
+        assert beyond.any() and (~beyond).any()
+        det = eng.cb.label_detected
+        np.testing.assert_array_equal(orc["label"] != det, beyond)
