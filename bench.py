@@ -26,7 +26,7 @@ for _p in (str(ROOT), str(ROOT / "centrex-molecule-trajectories_amd")):
 
 import numpy as np  # noqa: E402
 
-FLOP_PER_FACE = 29  # drift to a plane (25) + radius test (4); see DESIGN.md
+FLOP_PER_FACE = 29  # drift to a plane (25: time to the plane incl. the division, position, velocity) + radius test (4)
 FLOP_PER_RK4_STEP = 106  # classical RK4 step with 4 spline-force evaluations + bore test
 FLOP_ADJ_STEP = 204  # reverse of one RK4 step: 4 force+derivative evaluations, stage points, adjoint sums
 FLOP_SMOOTH_FACE = 23  # smoothed objective, per aperture face: drift, radius, sigmoid, product
@@ -260,7 +260,7 @@ def bench_stream(ctx, n: int) -> dict:
         "molecules": n, "chunks": (n + (1 << 22) - 1) >> 22,
         "pass1_device_ms": res.timings["pass1_device_ms"], "pass1_wall_s": res.timings["pass1_wall_s"],
         "pass1_molecules_per_s": n / (res.timings["pass1_device_ms"] * 1e-3),
-        "pass2_device_ms": p2, "detected": res.n_detected, "detected_per_state": det.tolist(),
+        "pass2_device_ms": p2, "pass2_alloc_s": res.timings["pass2_alloc_s"], "detected": res.n_detected, "detected_per_state": det.tolist(),
         "samples_per_molecule": res.n_samples, "trajectory_bytes": tb,
         "trajectory_write_GBps": tb / (p2 * 1e-3) / 1e9 if p2 > 0 else None,
         "total_wall_s": wall, "rk4_steps_pass1": steps,
@@ -362,14 +362,26 @@ def main():
     n_det = int(aoi_count.to_host()[0])
 
     traffic = None
-    tf = ROOT / "profiles" / "traffic_propagate_lens.json"
-    if args.config == "lens" and tf.exists() and n == 10_000_000:
+    lens = args.config == "lens"
+    tf = ROOT / "profiles" / ("traffic_propagate_lens.json" if lens else "traffic_propagate_ballistic.json")
+    if tf.exists() and n == (10_000_000 if lens else 100_000_000):
         try:
             traffic = json.loads(tf.read_text()).get("bytes_per_launch")
         except Exception:
             traffic = None
+    if lens:
+        roof = {"bound": "mfma", "engine": ENGINE, "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64, "traffic": traffic}
+    else:
+        # no lens: a few divisions per element face against 48 B of inputs per molecule, so HBM is the roof;
+        # algorithmic bytes = the inputs read once + each kept molecule's index and 7 fields written once
+        gbps = (48.0 * n + 64.0 * n_det) / (kavg * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": gbps, "peak": PEAK_HBM, "unit": "GB/s", "frac": gbps / PEAK_HBM,
+                "traffic": traffic, "algorithmic_bytes_per_launch": 48.0 * n + 64.0 * n_det,
+                "fp64_tflops": achieved, "fp64_frac": achieved / PEAK_FP64}
     line = {
-        "metric": "molecules/sec through ES-lens beamline (FP64), fwd and fwd+adjoint, vs FP64 roofline",
+        "metric": ("molecules/sec through ES-lens beamline (FP64), fwd and fwd+adjoint, vs FP64 roofline" if lens
+                   else "molecules/sec through ballistic beamline (FP64), vs HBM roofline"),
         "value": value,
         "unit": "molecules/s",
         "n_gpus": 1,
@@ -382,9 +394,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (benchmark source drawn with RandomState(%d) in the reference's order)" % seed,
         "config": {"workload": workload, "molecules": n, "outputs": "per-element tallies + compacted detected phase space"},
-        "roofline": {"bound": "mfma", "engine": ENGINE, "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": traffic,
-                     "kernel": ("k_propagate<WSPLINE_HEX,RK4>" if args.simple or args.config != "lens"
+        "roofline": {**roof,
+                     "kernel": ("k_propagate<circular apertures, no lens>" if not lens
+                                else "k_propagate<WSPLINE_HEX,RK4>" if args.simple
                                 else "k_propagate_lanes<WSPLINE_HEX,RK4>" if args.lanes
                                 else "k_propagate<PRE> + k_lens<WSPLINE_HEX,RK4> (one ctraj_propagate)"),
                      "kernel_ms_avg": kavg,

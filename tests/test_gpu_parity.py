@@ -298,3 +298,34 @@ def test_aperture_edge_decisions_bitwise():
         assert beyond.any() and (~beyond).any()
         det = eng.cb.label_detected
         np.testing.assert_array_equal(orc["label"] != det, beyond)
+
+
+def test_aoi_capacity_smaller_than_the_kept_molecules(ctx):
+    """Kept molecules beyond aoi_cap are counted but not stored; the stored ones are distinct kept
+    molecules with their own phase space.  A small capacity also overflows the per-XCD staging
+    parts, so both the parts and the direct path to the output are exercised; a second launch
+    appends after the first."""
+    bl = configs.config0()
+    X, V = configs.draw_source(200_000, seed=11)
+    eng = Engine(bl, ctx)
+    ref = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], aoi_labels=("Detected",))
+    det = eng.labels.index("Detected")
+    want = np.nonzero(ref["label"] == det)[0]
+    assert want.size > 600
+    inp = {k: ctx.to_device(np.asarray(a)) for k, a in zip(FIELDS[:6], (X[0], X[1], X[2], V[0], V[1], V[2]))}
+    for cap in (150, want.size - 1, want.size, 2 * want.size + 10):
+        count = ctx.alloc(1, np.int64).fill(0)
+        index = ctx.alloc(cap, np.int64).fill(-1)
+        out = {k: ctx.alloc(cap) for k in FIELDS}
+        for rep in range(2):  # the second launch appends
+            eng.launch(inp, aoi_mask=1 << det, aoi_cap=cap, aoi_count=count, aoi_index=index, aoi_out=out,
+                       n=X.shape[1])
+        ctx.sync()
+        assert int(count.to_host()[0]) == 2 * want.size
+        got = index.to_host()
+        stored = min(cap, 2 * want.size)
+        assert np.all(got[:stored] >= 0)
+        vals, cnt = np.unique(got[:stored], return_counts=True)
+        assert np.all(np.isin(vals, want)) and np.all(cnt <= 2)
+        for k in FIELDS:
+            np.testing.assert_array_equal(out[k].to_host()[:stored], ref["out"][k][got[:stored]])
