@@ -41,7 +41,7 @@ struct BCfg {
 
 // Lens constants of one configuration, staged in the two-phase lens kernel's LDS.
 struct BLensC {
-  double ks, cu, R2, z1, Hs, Ll;
+  double ks, cu, R2, z1, Hs, Ll, gk;  // gk = -g/ks (every two-phase lens folds ks into its step)
   int32_t label_inside, e0, ne, lens;  // element slice and the lens's index in it
 };
 
@@ -72,7 +72,7 @@ struct BPipe {
 
 struct BLane {
   int c, e0, ne, lens, label_inside;
-  double ks, cu, R2, z1;
+  double ks, cu, R2, z1, gk;
 };
 
 __device__ __forceinline__ int find_cfg(const int64_t *s_begin, int n_cfg, int64_t gi) {
@@ -244,6 +244,7 @@ __global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
               const double Hs = L.p[2], Ll = L.p[3];
               b.ks = L.p[4];
               b.cu = L.p[5];
+              b.gk = hex_folds(b.ks) ? -g / b.ks : 0.0;
               b.R2 = L.p[1];
               b.z1 = L.z1;
               b.label_inside = L.label_inside;
@@ -269,9 +270,9 @@ __global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
     steps_w += (unsigned long long)__popcll(__ballot(s.phase == PH_LENS));
     if (s.phase == PH_LENS) {
       MState &m = s.m;
-      const Force<CTRAJ_FORCE_WSPLINE_HEX> F{b.ks, b.cu, g};
+      const Force<CTRAJ_FORCE_WSPLINE_HEX> F{b.ks, b.cu, g, b.gk};
       const bool last = (s.i == s.nfull);
-      rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, s.h, m.x, m.y, m.vx, m.vy, s.r2, s.status);
+      lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, s.h, m.x, m.y, m.vx, m.vy, s.r2, s.status);
       m.z = last ? b.z1 : m.z + s.dzs;
       m.t = m.t + s.h;
       s.r2 = __builtin_fma(m.x, m.x, m.y * m.y);
@@ -375,7 +376,7 @@ __global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT, 
   m.ax = 0.0; m.ay = -g; m.az = 0.0;
   int c = 0, ph = LN_IDLE, label = 0, i = 0, nfull = 0;
   uint32_t status = 0;
-  double ks = 0.0, cu = 0.0, R2 = 0.0, Hs = A.Hs, dzs = 0.0, r2 = 0.0;
+  double ks = 0.0, cu = 0.0, gk = 0.0, R2 = 0.0, Hs = A.Hs, dzs = 0.0, r2 = 0.0;
 
   for (;;) {
     const uint64_t lensm = __ballot(ph == LN_LENS);
@@ -432,6 +433,7 @@ __global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT, 
             const BLensC &K = s_cfg[c];
             ks = K.ks;
             cu = K.cu;
+            gk = K.gk;
             R2 = K.R2;
             if constexpr (!UH) Hs = K.Hs;
             m.ax = 0.0;
@@ -455,17 +457,19 @@ __global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT, 
 
     steps_w += (unsigned long long)__popcll(__ballot(ph == LN_LENS));
     if (ph == LN_LENS) {
-      const Force<CTRAJ_FORCE_WSPLINE_HEX> F{ks, cu, g};
+      const Force<CTRAJ_FORCE_WSPLINE_HEX> F{ks, cu, g, gk};
       const bool last = (i == nfull);
-      if (!last) {
-        if constexpr (UH) rk4_step_hh<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, Hs, Hh, H6, m.x, m.y, m.vx, m.vy, r2, status);
-        else rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, Hs, m.x, m.y, m.vx, m.vy, r2, status);
+      if (!last) {  // every lens here folds (the host sends batches with one that does not to the fused kernel)
+        if constexpr (UH)
+          rk4_step_hex_k(wsb, T.umax, cu, gk, Hs, Hh, H6, ks * Hh, ks * Hs, ks * H6, m.x, m.y, m.vx, m.vy, r2);
+        else
+          lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, Hs, m.x, m.y, m.vx, m.vy, r2, status);
         m.z = m.z + dzs;
         m.t = m.t + Hs;
       } else {  // the final partial step, formed here, once per molecule that reaches the lens end
         const BLensC &K = s_cfg[c];
         const double hp = K.Ll / m.vz - (double)nfull * Hs;
-        rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, hp, m.x, m.y, m.vx, m.vy, r2, status);
+        lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, hp, m.x, m.y, m.vx, m.vy, r2, status);
         m.z = K.z1;
         m.t = m.t + hp;
       }
@@ -676,6 +680,7 @@ int32_t ctraj_batch_create(ctraj_ctx *c, const ctraj_elem *elems, int32_t n_elem
   std::vector<BLensC> hl(n_cfgs);
   b->uniform_dt = true;
   bool have_dt = false;
+  bool all_fold = true;
   b->circ_only = true;
   for (int e = 0; e < n_elems_total; ++e)
     if (elems[e].kind != CTRAJ_CIRC && elems[e].kind != CTRAJ_LENS) b->circ_only = false;
@@ -693,6 +698,8 @@ int32_t ctraj_batch_create(ctraj_ctx *c, const ctraj_elem *elems, int32_t n_elem
       K.z1 = L.z1;
       K.Hs = L.p[2];
       K.Ll = L.p[3];
+      K.gk = hex_folds(K.ks) ? -b->g / K.ks : 0.0;
+      all_fold = all_fold && hex_folds(K.ks);
       K.label_inside = L.label_inside;
       if (!have_dt) {
         b->dt = K.Hs;
@@ -702,7 +709,7 @@ int32_t ctraj_batch_create(ctraj_ctx *c, const ctraj_elem *elems, int32_t n_elem
       }
     }
   }
-  if (any_lens && total > 0 && total <= kPipeMax) {
+  if (any_lens && total > 0 && total <= kPipeMax && all_fold) {
     const size_t lds_pre = (size_t)n_cfgs * sizeof(int64_t) + nt * sizeof(uint32_t);
     const void *kp = b->circ_only ? (const void *)k_batch_pre<KM_CIRC> : (const void *)k_batch_pre<KM_ALL>;
     int gpre = grid_for(c, kp, 256, lds_pre, total, 0);

@@ -73,6 +73,17 @@ struct InMol {
 // is recorded (outputs, tallies, kept-molecule compaction) by the phase in which it finishes.
 enum : int { PHASE_FULL = 0, PHASE_PRE = 1 };
 
+// Lens step constants: the half and sixth of the step (as rk4_step forms them) and, for a folding
+// hexapole lens, ks times the half step, the step and the sixth, and gk = -g/ks (rk4_step_hex_k).
+struct StepK {
+  double hh, h6, khh, kh, kh6, gk;
+};
+
+__host__ __device__ inline StepK step_k(double h, double ks, double g) {
+  const double hh = 0.5 * h, h6 = h * (1.0 / 6.0);
+  return StepK{hh, h6, ks * hh, ks * h, ks * h6, hex_folds(ks) ? -g / ks : 0.0};
+}
+
 struct PipeArgs {
   longlong2 *q;                // lens-entry queue, 4 vectors per entry
   uint32_t *wcount;            // PRE: entries queued by each wavefront (a prefix of its region)
@@ -442,7 +453,7 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
       if constexpr (INTEG == CTRAJ_INTEG_RK4) {
         // branch-free step tail: every lane in the lens runs the same instructions
         const bool last = (s.i == s.nfull);
-        rk4_step<FORCE>(F, T, s.wsb, s.h, m.x, m.y, m.vx, m.vy, s.r2, s.status);
+        lens_step<FORCE>(F, T, s.wsb, s.h, m.x, m.y, m.vx, m.vy, s.r2, s.status);
         m.z = last ? z1 : m.z + s.dzs;
         m.t = m.t + s.h;
         s.r2 = __builtin_fma(m.x, m.x, m.y * m.y);
@@ -505,7 +516,7 @@ enum : int { LN_IDLE = 0, LN_LENS = 1, LN_EXIT = 2, LN_FIN = 3, LN_DONE = 4 };
 template <int FORCE, int INTEG, int KM>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
     FORCE == CTRAJ_FORCE_WSPLINE_HEX && INTEG == CTRAJ_INTEG_RK4 && KM == KM_CIRC ? 5 : 1)))
-k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refill_min, double Hh, double H6) {
+k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refill_min, StepK K) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
@@ -525,8 +536,8 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
   const ctraj_elem &L0 = B.elems[P.lens_e];
   const Force<FORCE> F = make_force<FORCE>(L0, B.g);
   const double R2 = L0.p[1], z1 = L0.z1;
-  // config-mode step size (uniform); Hh = 0.5 Hs and H6 = Hs (1/6) come formed by the host, so all
-  // three stay in scalar registers
+  // config-mode step size (uniform); its half and sixth, and for a folding hexapole the velocity
+  // constants ks Hs/2, ks Hs, ks Hs/6, come formed by the host in K, so all stay in scalar registers
   const double Hs = L0.p[2];
   const int label_inside = L0.label_inside;
   WalkCounters wc{0u, 0u};
@@ -677,11 +688,20 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
       if constexpr (INTEG == CTRAJ_INTEG_RK4) {
         const bool last = (i == nfull);
         if (!last) {  // full steps: the uniform step size
-          rk4_step_hh<FORCE>(F, T, wsb, Hs, Hh, H6, m.x, m.y, m.vx, m.vy, r2, status);
+          if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX)  // the host launches only folding lenses here
+            rk4_step_hex_k(wsb, T.umax, F.cu, K.gk, Hs, K.hh, K.h6, K.khh, K.kh, K.kh6, m.x, m.y, m.vx, m.vy, r2);
+          else
+            rk4_step_hh<FORCE>(F, T, wsb, Hs, K.hh, K.h6, m.x, m.y, m.vx, m.vy, r2, status);
           m.z = m.z + dzs;
           m.t = m.t + Hs;
         } else {  // the final partial step, once per molecule that reaches the lens end
-          rk4_step<FORCE>(F, T, wsb, hp, m.x, m.y, m.vx, m.vy, r2, status);
+          if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX) {
+            const double hh = 0.5 * hp, h6 = hp * kSixth;
+            rk4_step_hex_k(wsb, T.umax, F.cu, K.gk, hp, hh, h6, F.ks * hh, F.ks * hp, F.ks * h6, m.x, m.y, m.vx, m.vy,
+                           r2);
+          } else {
+            rk4_step<FORCE>(F, T, wsb, hp, m.x, m.y, m.vx, m.vy, r2, status);
+          }
           m.z = z1;
           m.t = m.t + hp;
         }
@@ -1048,7 +1068,7 @@ int ensure_buf(ctraj_ctx *ctx, ctraj_devbuf &b, size_t bytes) {
   X(CTRAJ_FORCE_WSPLINE_LIN, CTRAJ_INTEG_RK4)
 
 using PropFn = void (*)(KBeam, KTab, ctraj_run, int, PipeArgs);
-using LensFn = void (*)(KBeam, KTab, const ctraj_run *, PipeArgs, int, double, double);
+using LensFn = void (*)(KBeam, KTab, const ctraj_run *, PipeArgs, int, StepK);
 using LaneFn = void (*)(KBeam, KTab, const ctraj_run *, LaneArgs);
 using RecFn = void (*)(KBeam, KTab, ctraj_record);
 
@@ -1365,7 +1385,12 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     }
   // the lens-entry queue costs 64 B per molecule: single runs above 2^27 molecules (8.6 GB of
   // queue) use the single-kernel walker, which needs no queue
-  const bool pipeline = n_lens == 1 && !(R->flags & (CTRAJ_RUN_SIMPLE | CTRAJ_RUN_LANES)) && R->n <= ((int64_t)1 << 27);
+  // (and a hexapole lens whose force scale does not fold into the step constants, hex_folds, takes
+  // the single-kernel walker, which tests that per molecule)
+  const bool folds = c->force != CTRAJ_FORCE_WSPLINE_HEX || c->integ != CTRAJ_INTEG_RK4 ||
+                     (lens_e >= 0 && hex_folds(c->h_elems[lens_e].p[4]));
+  const bool pipeline = n_lens == 1 && !(R->flags & (CTRAJ_RUN_SIMPLE | CTRAJ_RUN_LANES)) && R->n <= ((int64_t)1 << 27) &&
+                        folds;
   const bool lanes = n_lens == 1 && !(R->flags & CTRAJ_RUN_SIMPLE) && !pipeline;
   // The lane-refill kernels keep their tables once per workgroup: with several lens-state tables
   // (more than ~40 KB) one large workgroup per CU holds as many wavefronts as its registers allow,
@@ -1418,11 +1443,9 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     }
     const int refill = R->refill > 0 ? std::min(R->refill, 64) : 12;
     const int grid = grid_for(c, (const void *)kl, block, lds_lens, (int64_t)1 << 40, 0);
-    // the RK4 half and sixth steps, rounded as rk4_step forms them
-    const double Hs = c->h_elems[lens_e].p[2];
-    const double Hh = 0.5 * Hs, H6 = Hs * (1.0 / 6.0);
+    const StepK K = step_k(c->h_elems[lens_e].p[2], c->h_elems[lens_e].p[4], c->phys.g);
     hipLaunchKernelGGL(kl, dim3(grid), dim3(block), lds_lens, c->stream, B, kt, (const ctraj_run *)c->d_run, P, refill,
-                       Hh, H6);
+                       K);
   } else if (lanes) {
     LaneFn k = lane_kernel(c->force, c->integ);
     if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");

@@ -233,6 +233,67 @@ static void rk4_step(const OForce *F, const OTab *T, int state, double h, double
   *vy = fma(h6, svy, *vy);
 }
 
+/* A hexapole lens folds its force scale into the step constants when |ks| lies in [2^-500, 2^500]:
+ * a = ks W'(cu r^2) (x, y) - g y_hat = ks s with s = W' (x, y) + (0, gk), gk = -g/ks, so the stages
+ * carry s and ks enters through ks hh, ks h, ks h6 (the device's rk4_step_hex_k, same operations). */
+static int hex_folds(double ks) {
+  const double a = fabs(ks);
+  return a >= 0x1p-500 && a <= 0x1p500;
+}
+
+static void slope_hex_k(const OTab *T, int state, double cu, double gk, double x, double y, double *sx, double *sy) {
+  const double w = wspline_eval(T, state, fma(x, x, y * y) * cu);
+  *sx = w * x;
+  *sy = fma(w, y, gk);
+}
+
+static void rk4_step_hex_k(const OForce *F, const OTab *T, int state, double h, double *x, double *y, double *vx,
+                           double *vy) {
+  const double hh = 0.5 * h, h6 = h * (1.0 / 6.0);
+  const double khh = F->ks * hh, kh = F->ks * h, kh6 = F->ks * h6;
+  const double gk = -F->g / F->ks, cu = F->cu;
+  double ax, ay;
+  slope_hex_k(T, state, cu, gk, *x, *y, &ax, &ay);
+  double xs = fma(hh, *vx, *x), ys = fma(hh, *vy, *y);
+  double ux = fma(khh, ax, *vx), uy = fma(khh, ay, *vy);
+  double svx = ax, svy = ay;
+  slope_hex_k(T, state, cu, gk, xs, ys, &ax, &ay);
+  double sx = fma(2.0, ux, *vx), sy = fma(2.0, uy, *vy);
+  svx = fma(2.0, ax, svx);
+  svy = fma(2.0, ay, svy);
+  xs = fma(hh, ux, *x);
+  ys = fma(hh, uy, *y);
+  ux = fma(khh, ax, *vx);
+  uy = fma(khh, ay, *vy);
+  slope_hex_k(T, state, cu, gk, xs, ys, &ax, &ay);
+  sx = fma(2.0, ux, sx);
+  sy = fma(2.0, uy, sy);
+  svx = fma(2.0, ax, svx);
+  svy = fma(2.0, ay, svy);
+  xs = fma(h, ux, *x);
+  ys = fma(h, uy, *y);
+  ux = fma(kh, ax, *vx);
+  uy = fma(kh, ay, *vy);
+  slope_hex_k(T, state, cu, gk, xs, ys, &ax, &ay);
+  sx = sx + ux;
+  sy = sy + uy;
+  svx = svx + ax;
+  svy = svy + ay;
+  *x = fma(h6, sx, *x);
+  *y = fma(h6, sy, *y);
+  *vx = fma(kh6, svx, *vx);
+  *vy = fma(kh6, svy, *vy);
+}
+
+/* one configuration-mode lens step: the folded hexapole step where the lens folds, else rk4_step */
+static void lens_step(const OForce *F, const OTab *T, int state, double h, double *x, double *y, double *vx,
+                      double *vy, uint32_t *st) {
+  if (F->kind == CTRAJ_FORCE_WSPLINE_HEX && hex_folds(F->ks))
+    rk4_step_hex_k(F, T, state, h, x, y, vx, vy);
+  else
+    rk4_step(F, T, state, h, x, y, vx, vy, st);
+}
+
 /* configuration-mode lens interior: floor(L/(vz dt)) steps of dt, then L/vz - n dt; bore test
  * x^2 + y^2 > (d/2)^2 after each step; rows store the force at the row (exit row: (0,-g,0)) */
 static int lens_rk4(const ctraj_elem *E, const OTab *T, int state, double g, int cap, Mol *m, uint32_t *st,
@@ -251,12 +312,12 @@ static int lens_rk4(const ctraj_elem *E, const OTab *T, int state, double g, int
   int alive = 1;
   for (int i = 0; i <= nfull; ++i) {
     if (i < nfull) {
-      rk4_step(&F, T, state, h, &x, &y, &vx, &vy, st);
+      lens_step(&F, T, state, h, &x, &y, &vx, &vy, st);
       z = z + dzs;
       t = t + h;
     } else {
       const double hp = L / m->vz - (double)nfull * h;
-      rk4_step(&F, T, state, hp, &x, &y, &vx, &vy, st);
+      lens_step(&F, T, state, hp, &x, &y, &vx, &vy, st);
       z = E->z1;
       t = t + hp;
     }
