@@ -329,3 +329,24 @@ def test_aoi_capacity_smaller_than_the_kept_molecules(ctx):
         assert np.all(np.isin(vals, want)) and np.all(cnt <= 2)
         for k in FIELDS:
             np.testing.assert_array_equal(out[k].to_host()[:stored], ref["out"][k][got[:stored]])
+
+
+def test_unfolded_lens_at_zero_voltage_bitwise():
+    """A lens at V0 = 0 has force scale ks = 0, which cannot be folded into the step constants
+    (gravity would enter as -g/ks): it takes the lane walker with the plain RK4 step on every path
+    (default, lanes, one molecule per thread), bit-identical to the oracle, and its molecules fall
+    under gravity alone."""
+    bl = configs.config1(V0=0.0)
+    X, V = configs.draw_source(30_000, seed=44)
+    eng = Engine(bl)
+    assert eng.cb.elems[1].p[4] == 0.0
+    orc = O.propagate(eng.cb, X[0], X[1], X[2], V[0], V[1], V[2])
+    for flags in (0, N.RUN_LANES, N.RUN_SIMPLE):
+        dev = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], want_accel=True, flags=flags)
+        assert assert_parity(dev, orc, f"V0=0 flags={flags}"), "unfolded lens step must be bit-exact"
+        np.testing.assert_array_equal(dev["tally"][0], np.bincount(orc["label"], minlength=len(eng.labels)))
+    det = orc["label"] == eng.cb.label_detected
+    assert det.any()
+    # no lens force: vx is unchanged and vy fell by g t over the flight
+    np.testing.assert_allclose(orc["out"]["vx"][det], V[0][det], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(orc["out"]["vy"][det], V[1][det] - configs.G * orc["out"]["t"][det], rtol=0, atol=1e-9)
