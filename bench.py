@@ -119,7 +119,7 @@ def timed_ms(ctx, fn, reps: int, warmup: int = 2) -> float:
 
 def bench_adjoint(ctx, n: int, reps: int = 5) -> dict:
     """Forward+adjoint of the smoothed detected flux (config-1 geometry, 10^6 molecules, theta =
-    (r1, V0, z0, r2, r3, mu_vz), sigmoid width 50 um, checkpoints every 64 steps) against the
+    (r1, V0, z0, r2, r3, mu_vz), sigmoid width 50 um, checkpoints every 8 steps) against the
     2 n_params objective-only evaluations that central differences need."""
     from trajectories import configs
     from trajectories.smooth import SmoothFlux
@@ -171,7 +171,7 @@ def bench_adjoint(ctx, n: int, reps: int = 5) -> dict:
     achieved = f_adj / (t_adj * 1e-3) / 1e12
     return {
         "workload": "smoothed detected flux over the ES-lens beamline, theta=(r1,V0,z0,r2,r3,mu_vz), sigmoid 50 um, "
-                    "RK4 dt=10 us, checkpoints every 64 steps, source RandomState(3)",
+                    "RK4 dt=10 us, checkpoints every 8 steps, source RandomState(3)",
         "molecules": n, "n_params": npar,
         "fwd_adjoint_ms": t_adj, "fwd_adjoint_molecules_per_s": n / (t_adj * 1e-3),
         "fwd_only_ms": t_fwd, "fwd_only_molecules_per_s": n / (t_fwd * 1e-3),

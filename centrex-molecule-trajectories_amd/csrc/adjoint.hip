@@ -46,7 +46,8 @@ struct SmoothArgs {
 
 // Inside a checkpoint segment the reverse sweep works in windows of kSub steps: the segment is
 // recomputed once to store the state every kSub steps (HBM rows, cache-resident), then each
-// window, last first, is recomputed into the wavefront's LDS slice and reversed.
+// window, last first, is recomputed into the wavefront's LDS slice and reversed.  With K <= kSub
+// (the default K = kSub) a segment is one window: every step is recomputed once.
 constexpr int kSub = 8;
 
 // sigmoid(w) and 1 - sigmoid(w) without cancellation.
@@ -367,9 +368,10 @@ __global__ void __launch_bounds__(512) k_smooth(KBeam B, KTab KT, SmoothArgs A) 
               double x, y, vx, vy;
               get4(W, A.off_ck + cslot + sg, lane, x, y, vx, vy);
               double r2 = __builtin_fma(x, x, y * y);
-              // the state every kSub steps of the segment
+              // the state every kSub steps of the segment (a one-window segment starts from the
+              // checkpoint itself, still in registers)
               for (int q = 0; q < nsub; ++q) {
-                put4(W, A.off_sub + q, lane, x, y, vx, vy);
+                if (nsub > 1) put4(W, A.off_sub + q, lane, x, y, vx, vy);
                 if (q + 1 < nsub) {
                   for (int s = s0 + q * kSub; s < s0 + (q + 1) * kSub; ++s) {
                     rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, s < nfull ? h : hp, x, y, vx, vy, r2, st);
@@ -380,7 +382,7 @@ __global__ void __launch_bounds__(512) k_smooth(KBeam B, KTab KT, SmoothArgs A) 
               // windows, last first: recompute into LDS, then reverse
               for (int q = nsub - 1; q >= 0; --q) {
                 const int t0 = s0 + q * kSub, t1 = min(t0 + kSub, s1);
-                get4(W, A.off_sub + q, lane, x, y, vx, vy);
+                if (nsub > 1) get4(W, A.off_sub + q, lane, x, y, vx, vy);
                 r2 = __builtin_fma(x, x, y * y);
                 for (int s = t0; s < t1 - 1; ++s) {
                   lds_double *r = rec + (s - t0) * 256;

@@ -37,7 +37,7 @@ class SmoothFlux:
     (default: what the slowest molecule of the evaluated parameter sets needs)."""
 
     def __init__(self, beamline, x, y, vx, vy, eps, *, sigma_vz: float, softness: float = 50e-6,
-                 z_src: float = 0.0, ckpt_every: int = 64, max_steps: int | None = None, ctx=None, block: int = 0):
+                 z_src: float = 0.0, ckpt_every: int = 8, max_steps: int | None = None, ctx=None, block: int = 0):
         self.engine = Engine(beamline, ctx)
         self.ctx = self.engine.ctx
         self.beamline = beamline
