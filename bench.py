@@ -240,13 +240,16 @@ def bench_batch(ctx, reps: int = 5, tune_reps: int = 2) -> dict:
 
 
 def bench_stream(ctx, n: int) -> dict:
-    """The streamed run: n molecules of the ES-lens beamline in 2^22-molecule chunks drawn on the
+    """The streamed run: n molecules of the ES-lens beamline in 2^25-molecule chunks drawn on the
     device, four lens states assigned uniformly, per-state tallies, and the 1-mm z-grid samples of
     every detected molecule written to one compacted device buffer."""
     from trajectories import configs
     from trajectories.stream import StreamedRun
 
-    run = StreamedRun(configs.config4(), n, chunk=1 << 22, seed=4, ctx=ctx)
+    run = StreamedRun(configs.config4(), n, chunk=1 << 25, seed=4, ctx=ctx)
+    # one untimed run first: kernel code objects load and the context's sort buffers grow on first use
+    warm = run.run(trajectories=True)
+    del warm
     t0 = time.perf_counter()
     res = run.run(trajectories=True)
     wall = time.perf_counter() - t0
@@ -256,8 +259,8 @@ def bench_stream(ctx, n: int) -> dict:
     steps = float(res.work[1])
     return {
         "workload": "ES-lens beamline, 4 Stark states (signs -/+, dipole x0.5, x0.8) assigned uniformly, "
-                    "2^22-molecule chunks drawn on the device (seed 4), detected trajectories every 1 mm in z",
-        "molecules": n, "chunks": (n + (1 << 22) - 1) >> 22,
+                    "2^25-molecule chunks drawn on the device (seed 4), detected trajectories every 1 mm in z",
+        "molecules": n, "chunks": (n + (1 << 25) - 1) >> 25,
         "pass1_device_ms": res.timings["pass1_device_ms"], "pass1_wall_s": res.timings["pass1_wall_s"],
         "pass1_molecules_per_s": n / (res.timings["pass1_device_ms"] * 1e-3),
         "pass2_device_ms": p2, "pass2_alloc_s": res.timings["pass2_alloc_s"], "detected": res.n_detected, "detected_per_state": det.tolist(),

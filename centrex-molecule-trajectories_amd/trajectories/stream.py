@@ -68,7 +68,7 @@ class StreamResult:
 
 
 class StreamedRun:
-    def __init__(self, beamline, n: int, *, chunk: int = 1 << 22, seed: int = 4, n_states: int = 4,
+    def __init__(self, beamline, n: int, *, chunk: int = 1 << 25, seed: int = 4, n_states: int = 4,
                  mean=None, sigma=None, z_start: float = 0.0, z_step: float = 1e-3, z_end: float | None = None,
                  ctx: Context | None = None):
         from . import configs
