@@ -106,7 +106,7 @@ __device__ __forceinline__ bool batch_start(const BCfg &C, const ctraj_elem *el,
   m.ay = -g;
   m.az = 0.0;
   lab = label_detected;
-  const bool ok = walk_plain<KM>(el, 0, C.lens >= 0 ? C.lens : C.ne, g, m, lab, wc);
+  const bool ok = walk_plain<KM, true>(el, 0, C.lens >= 0 ? C.lens : C.ne, g, m, lab, wc);
   if (!ok || C.lens < 0) return false;
   const ctraj_elem &L = el[C.lens];
   drift_dt(m, time_to(m, L.z0), g);

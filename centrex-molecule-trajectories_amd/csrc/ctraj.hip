@@ -29,7 +29,10 @@ using namespace ctraj;
 // k_propagate: one molecule per thread, grid-stride over the SoA store.  Each wavefront stages its
 // kept (aoi) molecules in its own LDS slice and flushes them to the dense output with one global
 // atomic per ~AOI_WAVE/2 molecules (no workgroup barrier in the loop).  The next molecule's inputs
-// are loaded before the current one is walked, so HBM latency overlaps the walk.
+// are loaded before the current one is walked, so HBM latency overlaps the walk.  A flush runs at
+// the start of the iteration after the one that filled the slice, before that iteration's input
+// loads: vector memory operations complete in order, so the wait for those loads at the end of the
+// iteration would otherwise also wait for the flush's stores to reach memory.
 // INTEG = CTRAJ_INTEG_NONE instantiates the walker without lens code (beamlines with no lens).
 // ---------------------------------------------------------------------------------------------
 constexpr int AOI_WAVE = 128;          // staged entries per wavefront
@@ -131,6 +134,7 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
     A.label = reinterpret_cast<int *>(base + AOI_WAVE * 64);
   }
   int aoi_n = 0;  // entries staged by this wavefront (wave-uniform)
+  bool aoi_full = false;  // the slice holds more than AOI_WAVE - 64 entries: flush before staging more
   __syncthreads();
 
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -152,17 +156,22 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
     const int st = q.st;
     const int64_t i = q.idx;
     uint32_t status = (uint32_t)q.status;
+    if (aoi_full) {  // wave-uniform
+      aoi_flush_wave(R, A, aoi_n, lane);
+      aoi_n = 0;
+      aoi_full = false;
+    }
     if (j + stride < n_items) load_mol(R, B, j + stride, q);  // prefetch the next molecule
     int label = 0;
     bool enq = false;
     if (active) {
       if constexpr (PHASE == PHASE_FULL) {
         NullSink sk;
-        walk<FORCE, INTEG, NullSink, KM>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status,
+        walk<FORCE, INTEG, NullSink, KM, true>(B.elems, B.n_elems, T, B.g, B.cap, st, B.label_detected, m, label, status,
                                          wc, sk);
       } else if constexpr (PHASE == PHASE_PRE) {
         label = B.label_detected;
-        if (walk_plain<KM>(B.elems, 0, P.lens_e, B.g, m, label, wc)) {
+        if (walk_plain<KM, true>(B.elems, 0, P.lens_e, B.g, m, label, wc)) {
           const ctraj_elem &L = B.elems[P.lens_e];
           drift_dt(m, time_to(m, L.z0), B.g);  // lens entrance (electrostatic_lens.py:59-64)
           ++wc.faces;
@@ -216,10 +225,7 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
           A.val[6 * AOI_WAVE + slot] = m.t;
         }
         aoi_n += __popcll(mask);
-        if (aoi_n > AOI_WAVE - 64) {
-          aoi_flush_wave(R, A, aoi_n, lane);
-          aoi_n = 0;
-        }
+        aoi_full = aoi_n > AOI_WAVE - 64;
       }
     }
   }

@@ -522,14 +522,49 @@ __device__ __forceinline__ bool lens_rk4(const ctraj_elem &E, const Tables &T, i
 // that covers the beamline, which keeps unused code (and its registers) out of the kernel.
 constexpr int KM_CIRC = 1, KM_RECT = 2, KM_PLATES = 4, KM_ALL = 7;
 
-template <int FORCE, int INTEG, class Sink, int KM = KM_ALL>
+// The element table is written by the host before a launch and never by a kernel, so a streaming
+// walker (one that prefetches its next molecule) reads it through the constant address space:
+// wave-uniform reads become scalar loads, which do not wait behind the wavefront's outstanding
+// vector loads and stores (vector memory operations complete in order, so a vector load of an
+// element would also wait for the prefetched next molecule).  The lane-refill kernels, which
+// prefetch nothing and are short of scalar registers, keep vector loads (SE = false).
+__device__ __forceinline__ ctraj_elem elem_at(const ctraj_elem *elems, int e) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const __attribute__((address_space(4))) ctraj_elem const_elem_t;
+  const const_elem_t &E = ((const_elem_t *)elems)[e];
+  ctraj_elem r;
+  r.kind = E.kind;
+  r.label_hit = E.label_hit;
+  r.label_inside = E.label_inside;
+  r.force = E.force;
+  r.integrator = E.integrator;
+  r.table = E.table;
+  r.n_steps = E.n_steps;
+  r.reserved = E.reserved;
+  r.z0 = E.z0;
+  r.z1 = E.z1;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.p[k] = E.p[k];
+  return r;
+#else
+  return elems[e];  // the host pass never runs device code
+#endif
+}
+
+template <bool SE>
+__device__ __forceinline__ ctraj_elem elem_get(const ctraj_elem *elems, int e) {
+  if constexpr (SE) return elem_at(elems, e);
+  else return elems[e];
+}
+
+template <int FORCE, int INTEG, class Sink, int KM = KM_ALL, bool SE = false>
 __device__ __forceinline__ void walk(const ctraj_elem *__restrict__ elems, int n_elems, const Tables &T, double g,
                                      int cap, int state, int label_detected, MState &m, int &label, uint32_t &status,
                                      WalkCounters &wc, Sink &sink) {
   if constexpr (Sink::kRows) sink.row(m);
   label = label_detected;
   for (int e = 0; e < n_elems; ++e) {
-    const ctraj_elem E = elems[e];
+    const ctraj_elem E = elem_get<SE>(elems, e);
     if (E.kind == CTRAJ_CIRC) {
       // CircularAperture.propagate_through (apertures.py:92-115): test at z0 and z1; x0, y0 unused.
       // p[1] holds the squared-radius threshold of d/2 (beyond_r2)

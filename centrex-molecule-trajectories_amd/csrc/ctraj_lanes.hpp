@@ -36,12 +36,12 @@ struct LaneMol {
 
 // Walk elements [e0, e1) (no lens among them) from the molecule's current row.  Returns false if
 // the molecule stopped (label set).
-template <int KM = KM_ALL>
+template <int KM = KM_ALL, bool SE = false>
 __device__ __forceinline__ bool walk_plain(const ctraj_elem *__restrict__ elems, int e0, int e1, double g, MState &m,
                                            int &label, WalkCounters &wc) {
   NullSink sk;
   for (int e = e0; e < e1; ++e) {
-    const ctraj_elem E = elems[e];
+    const ctraj_elem E = elem_get<SE>(elems, e);
     if (E.kind == CTRAJ_CIRC) {
       const double thr2 = E.p[1];  // the squared-radius threshold of d/2 (beyond_r2)
       drift_dt(m, time_to(m, E.z0), g);
