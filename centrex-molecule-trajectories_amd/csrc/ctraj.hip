@@ -258,6 +258,7 @@ struct LaneArgs {
   int lens_e;                   // index of the lens in the element table
   int refill_min;               // idle lanes that trigger a refill section
   int claim;                    // molecules claimed per atomic
+  double gk;                    // -g/ks of a hexapole lens that folds its force scale (FOLD)
 };
 
 // Record the molecules of lanes with s.fin set (convergent call).  A molecule stopped by the bore
@@ -320,7 +321,9 @@ __device__ __forceinline__ void record_finished(const KBeam &B, const ctraj_run 
 }
 
 
-template <int FORCE, int INTEG>
+// FOLD: a hexapole lens whose force scale folds into the step constants (rk4_step_hex_k); the host
+// launches FOLD = false for other lenses, which take rk4_step.
+template <int FORCE, int INTEG, bool FOLD = false>
 __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg,
                                                          LaneArgs LA) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
@@ -459,7 +462,13 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
       if constexpr (INTEG == CTRAJ_INTEG_RK4) {
         // branch-free step tail: every lane in the lens runs the same instructions
         const bool last = (s.i == s.nfull);
-        lens_step<FORCE>(F, T, s.wsb, s.h, m.x, m.y, m.vx, m.vy, s.r2, s.status);
+        if constexpr (FOLD) {
+          const double hh = 0.5 * s.h, h6 = s.h * kSixth;
+          rk4_step_hex_k(s.wsb, T.umax, F.cu, LA.gk, s.h, hh, h6, F.ks * hh, F.ks * s.h, F.ks * h6, m.x, m.y, m.vx,
+                         m.vy, s.r2);
+        } else {
+          rk4_step<FORCE>(F, T, s.wsb, s.h, m.x, m.y, m.vx, m.vy, s.r2, s.status);
+        }
         m.z = last ? z1 : m.z + s.dzs;
         m.t = m.t + s.h;
         s.r2 = __builtin_fma(m.x, m.x, m.y * m.y);
@@ -1116,7 +1125,9 @@ static LensFn lens_kernel(int f, int i, int km) {
   return nullptr;
 }
 
-static LaneFn lane_kernel(int f, int i) {
+static LaneFn lane_kernel(int f, int i, bool fold) {
+  if (fold && f == CTRAJ_FORCE_WSPLINE_HEX && i == CTRAJ_INTEG_RK4)
+    return k_propagate_lanes<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_RK4, true>;
 #define X(F, I) \
   if (f == F && i == I) return k_propagate_lanes<F, I>;
   CT_FI_LIST(X)
@@ -1453,7 +1464,8 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     hipLaunchKernelGGL(kl, dim3(grid), dim3(block), lds_lens, c->stream, B, kt, (const ctraj_run *)c->d_run, P, refill,
                        K);
   } else if (lanes) {
-    LaneFn k = lane_kernel(c->force, c->integ);
+    const double ks = c->h_elems[lens_e].p[4];
+    LaneFn k = lane_kernel(c->force, c->integ, hex_folds(ks));
     if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
     int rc = lds_limit(c, (const void *)k, lds);
     if (rc) return rc;
@@ -1473,6 +1485,7 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     LA.lens_e = lens_e;
     LA.refill_min = R->refill > 0 ? std::min(R->refill, 64) : 24;
     LA.claim = 128;
+    LA.gk = hex_folds(ks) ? -c->phys.g / ks : 0.0;
     const int grid = grid_for(c, (const void *)k, block, lds, R->n, R->grid);
     hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, (const ctraj_run *)c->d_run, LA);
   } else {
