@@ -76,7 +76,7 @@ class BatchScan:
         self._wspline = reg.wsplines[0] if reg.wsplines else None
         self.n_cfg = n_cfg
         self.total = int(counts.sum())
-        self.tune = N.BatchTune(block=0, chunk=128, grid=0, refill=16, use_graph=1, fused=0)
+        self.tune = N.BatchTune(block=0, chunk=128, grid=0, refill=0, use_graph=1, fused=0)
 
     def run(self, tune: N.BatchTune | None = None, graph: bool | None = None):
         """One launch over every configuration.  Returns (tally [n_cfg][n_labels], work, device ms)."""

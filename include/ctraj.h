@@ -133,11 +133,12 @@ int32_t ctraj_set_wspline(ctraj_ctx *ctx, int32_t n_states, int32_t n_seg, const
 /* ---- forward propagation: replaces run_simulation's per-molecule loop (trajectory_simulator.py:52-78)
  *      and Beamline.propagate_through (beamline.py:20-38).  A beamline with exactly one lens runs in
  *      two phases queued on the stream: a streaming walk to and through the lens entrance that queues
- *      the molecules entering the lens (a context-owned queue of 65 bytes per molecule), then a
+ *      the molecules entering the lens (a context-owned queue of 64 bytes per molecule), then a
  *      persistent lane-refill kernel that integrates the lens and walks the elements after it.  Runs
- *      of more than 2^27 molecules, or CTRAJ_RUN_LANES, use the single-kernel lane-refill walker;
- *      other beamlines, or CTRAJ_RUN_SIMPLE, run one molecule per thread.  All give bit-identical
- *      results. ---- */
+ *      of more than 2^27 molecules, CTRAJ_RUN_LANES, or a WSPLINE_HEX lens whose force scale p[4]
+ *      lies outside [2^-500, 2^500] in magnitude (V0 = 0, say: its RK4 step cannot fold ks into the
+ *      step constants) use the single-kernel lane-refill walker; other beamlines, or
+ *      CTRAJ_RUN_SIMPLE, run one molecule per thread.  All give bit-identical results. ---- */
 #define CTRAJ_RUN_STATUS_OR 1 /* OR status bits into run->status instead of overwriting */
 #define CTRAJ_RUN_SIMPLE 2    /* one molecule per thread start to finish (no lane refill) */
 #define CTRAJ_RUN_LANES 4     /* one lens: the single-kernel lane-refill walker instead of the two phases */
@@ -239,13 +240,14 @@ typedef struct ctraj_batch_cfg {
 /* A batch of at most 2^27 molecules with a lens runs in two phases, like ctraj_propagate: every
  * molecule is drawn and walked to its lens entrance, the molecules entering a lens are queued (64 B
  * each, batch-owned), then persistent wavefronts integrate the queue with lane refill.  Larger
- * batches, or fused = 1, run one lane-refill kernel that draws, walks and integrates. */
+ * batches, batches with a lens whose force scale does not fold (see ctraj_propagate), or fused = 1,
+ * run one lane-refill kernel that draws, walks and integrates. */
 typedef struct ctraj_batch_tune {
   int32_t block;     /* threads per block of the lens kernel: a multiple of 64 up to 1024 (0 = 1024); with
                       * fused: 64, 128, 192 or 256 (0 = 256) */
   int32_t chunk;     /* fused: molecules a wavefront claims at a time from the global queue (0 = 128) */
   int32_t grid;      /* persistent blocks (0 = as many as are resident) */
-  int32_t refill;    /* refill a wavefront when at least this many lanes are idle, 1..64 (0 = 16) */
+  int32_t refill;    /* refill a wavefront when at least this many lanes are idle, 1..64 (0 = 8, fused 16) */
   int32_t use_graph; /* 1: capture the launches (counter and tally resets, kernels) once as a hipGraph and replay */
   int32_t fused;     /* 1: the single-kernel walker instead of the two phases */
 } ctraj_batch_tune;
