@@ -107,7 +107,8 @@ __device__ __forceinline__ void load_mol(const ctraj_run &R, const KBeam &B, int
   q.ax = R.a0[0] ? R.a0[0][i] : 0.0;
   q.ay = R.a0[1] ? R.a0[1][i] : -B.g;
   q.az = R.a0[2] ? R.a0[2][i] : 0.0;
-  q.st = R.state ? min((int)R.state[i], B.n_states - 1) : 0;
+  q.st = R.state ? (int)R.state[i] : 0;  // clamped to the states present where it is used, not here:
+                                         // a use right after the load would wait for the whole prefetch
   q.status = 0;
   q.idx = i;
 }
@@ -153,7 +154,7 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
     MState m;
     m.x = q.x; m.y = q.y; m.z = q.z; m.vx = q.vx; m.vy = q.vy; m.vz = q.vz; m.t = q.t;
     m.ax = q.ax; m.ay = q.ay; m.az = q.az;
-    const int st = q.st;
+    const int st = min(q.st, B.n_states - 1);
     const int64_t i = q.idx;
     uint32_t status = (uint32_t)q.status;
     if (aoi_full) {  // wave-uniform

@@ -551,6 +551,15 @@ __device__ __forceinline__ ctraj_elem elem_at(const ctraj_elem *elems, int e) {
 #endif
 }
 
+// The label of each element visited, read in a streaming walker's loop and kept in a vector
+// register.  Without the empty asm the compiler sinks the read of the stopping element's label past
+// the loop, as a per-lane vector load of the element table, and the wait for that load (vector
+// memory operations complete in order) also waits for the prefetched next molecule.
+__device__ __forceinline__ int stop_label(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 template <bool SE>
 __device__ __forceinline__ ctraj_elem elem_get(const ctraj_elem *elems, int e) {
   if constexpr (SE) return elem_at(elems, e);
@@ -565,6 +574,7 @@ __device__ __forceinline__ void walk(const ctraj_elem *__restrict__ elems, int n
   label = label_detected;
   for (int e = 0; e < n_elems; ++e) {
     const ctraj_elem E = elem_get<SE>(elems, e);
+    const int lhit = SE ? stop_label(E.label_hit) : E.label_hit;
     if (E.kind == CTRAJ_CIRC) {
       // CircularAperture.propagate_through (apertures.py:92-115): test at z0 and z1; x0, y0 unused.
       // p[1] holds the squared-radius threshold of d/2 (beyond_r2)
@@ -572,29 +582,29 @@ __device__ __forceinline__ void walk(const ctraj_elem *__restrict__ elems, int n
       drift_dt(m, time_to(m, E.z0), g);
       ++wc.faces;
       if constexpr (Sink::kRows) sink.row(m);
-      if (beyond_r2(m.x, m.y, thr2)) { label = E.label_hit; return; }
+      if (beyond_r2(m.x, m.y, thr2)) { label = lhit; return; }
       drift_dt(m, time_to(m, E.z1), g);
       ++wc.faces;
       if constexpr (Sink::kRows) sink.row(m);
-      if (beyond_r2(m.x, m.y, thr2)) { label = E.label_hit; return; }
+      if (beyond_r2(m.x, m.y, thr2)) { label = lhit; return; }
     } else if ((KM & KM_RECT) && E.kind == CTRAJ_RECT) {
       // RectangularAperture.propagate_through (apertures.py:165-189): strict box test
       const double x1 = E.p[0], x2 = E.p[1], y1 = E.p[2], y2 = E.p[3];
       drift_dt(m, time_to(m, E.z0), g);
       ++wc.faces;
       if constexpr (Sink::kRows) sink.row(m);
-      if (!((x1 < m.x && m.x < x2) && (y1 < m.y && m.y < y2))) { label = E.label_hit; return; }
+      if (!((x1 < m.x && m.x < x2) && (y1 < m.y && m.y < y2))) { label = lhit; return; }
       drift_dt(m, time_to(m, E.z1), g);
       ++wc.faces;
       if constexpr (Sink::kRows) sink.row(m);
-      if (!((x1 < m.x && m.x < x2) && (y1 < m.y && m.y < y2))) { label = E.label_hit; return; }
+      if (!((x1 < m.x && m.x < x2) && (y1 < m.y && m.y < y2))) { label = lhit; return; }
     } else if ((KM & KM_PLATES) && E.kind == CTRAJ_PLATES) {
       // FieldPlates.propagate_through (apertures.py:227-270): x-bounded plates with wall-hit time
       const double x1 = E.p[0], x2 = E.p[1];
       drift_dt(m, time_to(m, E.z0), g);
       ++wc.faces;
       if constexpr (Sink::kRows) sink.row(m);
-      if (!(x1 < m.x && m.x < x2)) { label = E.label_hit; return; }
+      if (!(x1 < m.x && m.x < x2)) { label = lhit; return; }
       double dt = time_to(m, E.z1);
       double xe = m.x;
       if (dt != 0.0) xe = (m.x + m.vx * dt) + (m.ax * (dt * dt)) * 0.5;
@@ -606,13 +616,13 @@ __device__ __forceinline__ void walk(const ctraj_elem *__restrict__ elems, int n
       drift_dt(m, dt, g);
       ++wc.faces;
       if constexpr (Sink::kRows) sink.row(m);
-      if (!inside) { label = E.label_hit; return; }
+      if (!inside) { label = lhit; return; }
     } else if (INTEG != CTRAJ_INTEG_NONE && E.kind == CTRAJ_LENS) {
       // ElectrostaticLens.propagate_through (electrostatic_lens.py:48-77)
       drift_dt(m, time_to(m, E.z0), g);
       ++wc.faces;
       if constexpr (Sink::kRows) sink.row(m);
-      if (rho_of(m.x, m.y) > E.p[0]) { label = E.label_hit; return; }
+      if (rho_of(m.x, m.y) > E.p[0]) { label = lhit; return; }
       if constexpr (INTEG == CTRAJ_INTEG_NONE) {
       } else if constexpr (INTEG == CTRAJ_INTEG_REF) {
         if (!lens_ref<FORCE>(E, T, state, g, m, status, wc, sink)) { label = E.label_inside; return; }

@@ -42,27 +42,28 @@ __device__ __forceinline__ bool walk_plain(const ctraj_elem *__restrict__ elems,
   NullSink sk;
   for (int e = e0; e < e1; ++e) {
     const ctraj_elem E = elem_get<SE>(elems, e);
+    const int lhit = SE ? stop_label(E.label_hit) : E.label_hit;
     if (E.kind == CTRAJ_CIRC) {
       const double thr2 = E.p[1];  // the squared-radius threshold of d/2 (beyond_r2)
       drift_dt(m, time_to(m, E.z0), g);
       ++wc.faces;
-      if (beyond_r2(m.x, m.y, thr2)) { label = E.label_hit; return false; }
+      if (beyond_r2(m.x, m.y, thr2)) { label = lhit; return false; }
       drift_dt(m, time_to(m, E.z1), g);
       ++wc.faces;
-      if (beyond_r2(m.x, m.y, thr2)) { label = E.label_hit; return false; }
+      if (beyond_r2(m.x, m.y, thr2)) { label = lhit; return false; }
     } else if ((KM & KM_RECT) && E.kind == CTRAJ_RECT) {
       const double x1 = E.p[0], x2 = E.p[1], y1 = E.p[2], y2 = E.p[3];
       drift_dt(m, time_to(m, E.z0), g);
       ++wc.faces;
-      if (!((x1 < m.x && m.x < x2) && (y1 < m.y && m.y < y2))) { label = E.label_hit; return false; }
+      if (!((x1 < m.x && m.x < x2) && (y1 < m.y && m.y < y2))) { label = lhit; return false; }
       drift_dt(m, time_to(m, E.z1), g);
       ++wc.faces;
-      if (!((x1 < m.x && m.x < x2) && (y1 < m.y && m.y < y2))) { label = E.label_hit; return false; }
+      if (!((x1 < m.x && m.x < x2) && (y1 < m.y && m.y < y2))) { label = lhit; return false; }
     } else if ((KM & KM_PLATES) && E.kind == CTRAJ_PLATES) {
       const double x1 = E.p[0], x2 = E.p[1];
       drift_dt(m, time_to(m, E.z0), g);
       ++wc.faces;
-      if (!(x1 < m.x && m.x < x2)) { label = E.label_hit; return false; }
+      if (!(x1 < m.x && m.x < x2)) { label = lhit; return false; }
       double dt = time_to(m, E.z1);
       double xe = m.x;
       if (dt != 0.0) xe = (m.x + m.vx * dt) + (m.ax * (dt * dt)) * 0.5;
@@ -73,7 +74,7 @@ __device__ __forceinline__ bool walk_plain(const ctraj_elem *__restrict__ elems,
       }
       drift_dt(m, dt, g);
       ++wc.faces;
-      if (!inside) { label = E.label_hit; return false; }
+      if (!inside) { label = lhit; return false; }
     }
   }
   (void)sk;
