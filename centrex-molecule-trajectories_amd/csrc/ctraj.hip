@@ -113,7 +113,21 @@ __device__ __forceinline__ void load_mol(const ctraj_run &R, const KBeam &B, int
   q.idx = i;
 }
 
-template <int FORCE, int INTEG, int PHASE, int KM, bool OUT>
+// Inputs of a run that passes positions and velocities only (no time, stored acceleration or state
+// arrays): every load is issued unconditionally.
+__device__ __forceinline__ void load_min(const ctraj_run &R, int64_t i, InMol &q) {
+  q.x = R.in.x[i];
+  q.y = R.in.y[i];
+  q.z = R.in.z[i];
+  q.vx = R.in.vx[i];
+  q.vy = R.in.vy[i];
+  q.vz = R.in.vz[i];
+  q.idx = i;
+}
+
+// MIN: the run passes positions and velocities only; the inputs are then prefetched two iterations
+// ahead (and the constant time, acceleration and state are not carried per molecule).
+template <int FORCE, int INTEG, int PHASE, int KM, bool OUT, bool MIN = false>
 __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R, int aoi_lds_off, PipeArgs P) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
@@ -144,25 +158,35 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
   const int64_t n_items = R.n;
   const int64_t gwave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   int64_t qn = 0;  // PRE: entries this wavefront has queued
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  InMol q;
-  if (j < n_items) load_mol(R, B, j, q);
-  // wave-uniform trip count: every lane of a wavefront runs the same iterations
-  for (int64_t wbase = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < n_items;
-       wbase += stride, j += stride) {
+  const int64_t j0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t w0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  // one grid-stride iteration: molecule j (inputs in q, loaded earlier), then q is refilled
+  auto body = [&](InMol &q, const int64_t j, const int64_t wbase) {
     const bool active = j < n_items;
     MState m;
-    m.x = q.x; m.y = q.y; m.z = q.z; m.vx = q.vx; m.vy = q.vy; m.vz = q.vz; m.t = q.t;
-    m.ax = q.ax; m.ay = q.ay; m.az = q.az;
-    const int st = min(q.st, B.n_states - 1);
+    m.x = q.x; m.y = q.y; m.z = q.z; m.vx = q.vx; m.vy = q.vy; m.vz = q.vz;
+    int st = 0;
+    uint32_t status = 0;
+    if constexpr (MIN) {  // the values load_mol gives absent arrays
+      m.t = R.t_const;
+      m.ax = 0.0; m.ay = -B.g; m.az = 0.0;
+    } else {
+      m.t = q.t;
+      m.ax = q.ax; m.ay = q.ay; m.az = q.az;
+      st = min(q.st, B.n_states - 1);
+      status = (uint32_t)q.status;
+    }
     const int64_t i = q.idx;
-    uint32_t status = (uint32_t)q.status;
     if (aoi_full) {  // wave-uniform
       aoi_flush_wave(R, A, aoi_n, lane);
       aoi_n = 0;
       aoi_full = false;
     }
-    if (j + stride < n_items) load_mol(R, B, j + stride, q);  // prefetch the next molecule
+    if constexpr (MIN) {  // the molecule two iterations ahead; unconditional (the last index repeats at the end)
+      load_min(R, j + 2 * stride < n_items ? j + 2 * stride : n_items - 1, q);
+    } else if (j + stride < n_items) {
+      load_mol(R, B, j + stride, q);  // prefetch the next molecule
+    }
     int label = 0;
     bool enq = false;
     if (active) {
@@ -229,6 +253,23 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
         aoi_full = aoi_n > AOI_WAVE - 64;
       }
     }
+    };
+  // wave-uniform trip count: every lane of a wavefront runs the same iterations
+  if constexpr (MIN) {
+    // two input buffers, each consumed two iterations after its loads were issued; the loads are
+    // unconditional, so the compiler can count them and wait for the older buffer only
+    InMol qa, qb;
+    load_min(R, j0 < n_items ? j0 : n_items - 1, qa);
+    load_min(R, j0 + stride < n_items ? j0 + stride : n_items - 1, qb);
+    for (int64_t wbase = w0, j = j0; wbase < n_items; wbase += 2 * stride, j += 2 * stride) {
+      body(qa, j, wbase);
+      if (wbase + stride >= n_items) break;
+      body(qb, j + stride, wbase + stride);
+    }
+  } else {
+    InMol q;
+    if (j0 < n_items) load_mol(R, B, j0, q);
+    for (int64_t wbase = w0, j = j0; wbase < n_items; wbase += stride, j += stride) body(q, j, wbase);
   }
   if (R.aoi_mask && aoi_n > 0) aoi_flush_wave(R, A, aoi_n, lane);
   if constexpr (PHASE == PHASE_PRE) {
@@ -1102,15 +1143,22 @@ static bool wants_outputs(const ctraj_run *R) {
 }
 
 template <int PHASE>
-static PropFn walker_kernel(int km, bool out) {
+// min_in: the run passes positions and velocities only (k_propagate's MIN form, instantiated for the
+// circular-aperture walk without per-molecule outputs)
+static PropFn walker_kernel(int km, bool out, bool min_in) {
   constexpr int F = CTRAJ_FORCE_WSPLINE_HEX, I = CTRAJ_INTEG_NONE;
+  if (km == KM_CIRC && !out && min_in) return k_propagate<F, I, PHASE, KM_CIRC, false, true>;
   if (km == KM_CIRC)
     return out ? k_propagate<F, I, PHASE, KM_CIRC, true> : k_propagate<F, I, PHASE, KM_CIRC, false>;
   return out ? k_propagate<F, I, PHASE, KM_ALL, true> : k_propagate<F, I, PHASE, KM_ALL, false>;
 }
 
-static PropFn prop_kernel(int f, int i, bool has_lens, int km, bool out) {
-  if (!has_lens) return walker_kernel<PHASE_FULL>(km, out);
+static bool min_inputs(const ctraj_run *R) {
+  return R->in.z && !R->in.t && !R->a0[0] && !R->a0[1] && !R->a0[2] && !R->state;
+}
+
+static PropFn prop_kernel(int f, int i, bool has_lens, int km, bool out, bool min_in) {
+  if (!has_lens) return walker_kernel<PHASE_FULL>(km, out, min_in);
 #define X(F, I) \
   if (f == F && i == I) return k_propagate<F, I, PHASE_FULL, KM_ALL, true>;
   CT_FI_LIST(X)
@@ -1426,7 +1474,7 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     // enter the lens into its own region (no atomics)
     const int blk = 256;
     const size_t lds_walk = R->aoi_mask ? (size_t)(blk / 64) * AOI_WAVE * AOI_ENTRY_BYTES : 0;
-    PropFn kpre = walker_kernel<PHASE_PRE>(kinds_mask(c->h_elems, 0, lens_e), wants_outputs(R));
+    PropFn kpre = walker_kernel<PHASE_PRE>(kinds_mask(c->h_elems, 0, lens_e), wants_outputs(R), min_inputs(R));
     int rc = lds_limit(c, (const void *)kpre, lds_walk);
     if (rc) return rc;
     const int gpre = (grid_for(c, (const void *)kpre, blk, lds_walk, R->n, R->grid) + 7) & ~7;  // whole XCD groups
@@ -1490,7 +1538,8 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     const int grid = grid_for(c, (const void *)k, block, lds, R->n, R->grid);
     hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, (const ctraj_run *)c->d_run, LA);
   } else {
-    PropFn k = prop_kernel(c->force, c->integ, n_lens > 0, kinds_mask(c->h_elems, 0, c->n_elems), wants_outputs(R));
+    PropFn k = prop_kernel(c->force, c->integ, n_lens > 0, kinds_mask(c->h_elems, 0, c->n_elems), wants_outputs(R),
+                             min_inputs(R));
     if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
     if (n_lens == 0) kt.ws_states = kt.rt_total = 0;
     size_t lds_tab = n_lens > 0 ? lds : 0;

@@ -350,3 +350,47 @@ def test_unfolded_lens_at_zero_voltage_bitwise():
     # no lens force: vx is unchanged and vy fell by g t over the flight
     np.testing.assert_allclose(orc["out"]["vx"][det], V[0][det], rtol=0, atol=1e-12)
     np.testing.assert_allclose(orc["out"]["vy"][det], V[1][det] - configs.G * orc["out"]["t"][det], rtol=0, atol=1e-9)
+
+
+def launch_min(eng, X, V, n):
+    """Positions and velocities in HBM; tallies, work counters and kept (detected) molecules only."""
+    ctx = eng.ctx
+    inp = {k: ctx.to_device(a) for k, a in zip(("x", "y", "z", "vx", "vy", "vz"), (X[0], X[1], X[2], V[0], V[1], V[2]))}
+    tally = ctx.alloc(eng.cb.n_labels, np.int64).fill(0)
+    work = ctx.alloc(N.WORK_N, np.uint64).fill(0)
+    count = ctx.alloc(1, np.int64).fill(0)
+    index = ctx.alloc(max(n, 1), np.int64)
+    out = {k: ctx.alloc(max(n, 1)) for k in FIELDS}
+    eng.launch(inp, tally=tally, work=work, aoi_mask=1 << eng.cb.label_detected, aoi_cap=n, aoi_count=count,
+               aoi_index=index, aoi_out=out, n=n)
+    ctx.sync()
+    cnt = int(count.to_host()[0])
+    return {"tally": tally.to_host().reshape(1, -1), "work": work.to_host(), "aoi_index": index.to_host()[:cnt],
+            "aoi_out": {k: v.to_host()[:cnt] for k, v in out.items()}}
+
+
+@pytest.mark.parametrize("n", [1, 777, 300_001, 2_000_003])
+def test_positions_and_velocities_only_runs_equal_the_general_walker(n):
+    """A run that passes only positions and velocities and asks for no per-molecule outputs takes
+    the walkers that prefetch two molecules ahead (the ballistic walk, and the lens beamline's first
+    phase); its tallies, work counters and kept molecules equal the general walker's (a run with
+    per-molecule outputs) and the oracle's, at sizes that end the two-deep loop on either buffer."""
+    for bl, seed in ((configs.config0(), 51), (configs.config1(), 52)):
+        X, V = configs.draw_source(-(-n // 100) * 100, seed=seed)  # the source draws in hundreds
+        X, V = np.ascontiguousarray(X[:, :n]), np.ascontiguousarray(V[:, :n])
+        eng = Engine(bl)
+        fast = launch_min(eng, X, V, n)
+        gen = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], aoi_labels=("Detected",))
+        np.testing.assert_array_equal(fast["tally"], gen["tally"])
+        np.testing.assert_array_equal(fast["work"], gen["work"])
+        o, og = np.argsort(fast["aoi_index"]), np.argsort(gen["aoi_index"])
+        np.testing.assert_array_equal(fast["aoi_index"][o], gen["aoi_index"][og])
+        for k in FIELDS:
+            np.testing.assert_array_equal(fast["aoi_out"][k][o], gen["aoi_out"][k][og])
+        if n <= 300_001:
+            orc = O.propagate(eng.cb, X[0], X[1], X[2], V[0], V[1], V[2])
+            np.testing.assert_array_equal(fast["tally"][0], np.bincount(orc["label"], minlength=len(eng.labels)))
+            det = np.nonzero(orc["label"] == eng.cb.label_detected)[0]
+            np.testing.assert_array_equal(fast["aoi_index"][o], det)
+            for k in FIELDS:
+                np.testing.assert_array_equal(fast["aoi_out"][k][o], orc["out"][k][det])
