@@ -158,6 +158,16 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
   const int64_t n_items = R.n;
   const int64_t gwave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   int64_t qn = 0;  // PRE: entries this wavefront has queued
+  // PRE: the lens entrance, read once with scalar loads (a vector load of it in the loop would wait
+  // for the prefetched molecules and the queue stores in flight)
+  double lens_z0 = 0.0, lens_r = 0.0;
+  int lens_hit = 0;
+  if constexpr (PHASE == PHASE_PRE) {
+    const ctraj_elem L = elem_at(B.elems, P.lens_e);
+    lens_z0 = L.z0;
+    lens_r = L.p[0];
+    lens_hit = L.label_hit;
+  }
   const int64_t j0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t w0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
   // one grid-stride iteration: molecule j (inputs in q, loaded earlier), then q is refilled
@@ -197,10 +207,9 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
       } else if constexpr (PHASE == PHASE_PRE) {
         label = B.label_detected;
         if (walk_plain<KM, true>(B.elems, 0, P.lens_e, B.g, m, label, wc)) {
-          const ctraj_elem &L = B.elems[P.lens_e];
-          drift_dt(m, time_to(m, L.z0), B.g);  // lens entrance (electrostatic_lens.py:59-64)
+          drift_dt(m, time_to(m, lens_z0), B.g);  // lens entrance (electrostatic_lens.py:59-64)
           ++wc.faces;
-          if (rho_of(m.x, m.y) > L.p[0]) label = L.label_hit;
+          if (rho_of(m.x, m.y) > lens_r) label = lens_hit;
           else enq = true;
         }
       }
