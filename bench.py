@@ -26,8 +26,13 @@ for _p in (str(ROOT), str(ROOT / "centrex-molecule-trajectories_amd")):
 
 import numpy as np  # noqa: E402
 
-FLOP_PER_FACE = 29  # drift to a plane (25: time to the plane incl. the division, position, velocity) + radius test (4)
-FLOP_PER_RK4_STEP = 106  # classical RK4 step with 4 spline-force evaluations + bore test
+# FLOP counting rule: add/sub/mul/div/sqrt = 1, FMA = 2, counted from the kernel code; comparisons, min/max
+# clamps, conversions and address arithmetic are not counted.
+FLOP_PER_FACE = 28  # time to the plane (sub, div: 2) + drift (dt^2, 3 x position 5, 3 x velocity 2, time: 23)
+#                     + squared-radius test (x*x + y*y: 3)
+FLOP_PER_RK4_STEP = 102  # folded hexapole RK4 step: 4 slope evaluations (r^2*cu 1, fract 1, spline 2 FMA, slope
+#                          mul + FMA: 9 each, plus r^2 = 3 at stages 2-4), stage points 24, slope sums 28, z/t 2,
+#                          bore test r^2 3
 FLOP_ADJ_STEP = 204  # reverse of one RK4 step: 4 force+derivative evaluations, stage points, adjoint sums
 FLOP_SMOOTH_FACE = 23  # smoothed objective, per aperture face: drift, radius, sigmoid, product
 FLOP_SMOOTH_FACE_REV = 34  # its reverse: radius, sigmoid, factor derivative, drift adjoint
@@ -332,8 +337,10 @@ def main():
     tally.fill(0)
     work.fill(0)
     ctx.sync()
+    ctx.phase_times()  # forget anything timed before
     K = args.steps
-    kernel_ms = []
+    kernel_ms, walk_ms, lens_ms, post_ms = [], [], [], []
+    tflags = flags | N.RUN_PHASE_TIMES  # the library brackets each kernel of the launch with events on its stream
     wall0 = time.perf_counter()
     ctx.event_record(0)
     for k in range(K):
@@ -341,23 +348,34 @@ def main():
         s = 2 + 2 * (k % 31)
         ctx.event_record(s)
         eng.launch(inp, tally=tally, work=work, aoi_mask=1 << det, aoi_cap=cap, aoi_count=aoi_count,
-                   aoi_index=aoi_index, aoi_out=aoi_out, n=n, block=args.block, flags=flags, refill=args.refill)
+                   aoi_index=aoi_index, aoi_out=aoi_out, n=n, block=args.block, flags=tflags, refill=args.refill)
         ctx.event_record(s + 1)
         if k % 31 == 30:
             kernel_ms += [ctx.event_elapsed_ms(2 + 2 * j, 3 + 2 * j) for j in range(31)]
+            w, ln, po = ctx.phase_times()
+            walk_ms += list(w)
+            lens_ms += list(ln)
+            post_ms += list(po)
     ctx.event_record(1)
     ctx.sync()
     wall = time.perf_counter() - wall0
     rem = K % 31
     kernel_ms += [ctx.event_elapsed_ms(2 + 2 * j, 3 + 2 * j) for j in range(rem)]
+    w, ln, po = ctx.phase_times()
+    walk_ms += list(w)
+    lens_ms += list(ln)
+    post_ms += list(po)
+    assert len(lens_ms) == K, (len(lens_ms), K)
     region_ms = ctx.event_elapsed_ms(0, 1)
     tal = tally.to_host()
     wk = work.to_host()
     assert int(tal.sum()) == n * K, (int(tal.sum()), n * K)
     faces = int(wk[N.WORK_FACES]) / K
+    faces_lens = int(wk[N.WORK_FACES_POST]) / K
     steps_ = int(wk[N.WORK_STEPS]) / K
     flop = FLOP_PER_FACE * faces + FLOP_PER_RK4_STEP * steps_
     kavg = float(np.mean(kernel_ms))
+    walk_avg, lens_avg = float(np.mean(walk_ms)), float(np.mean(lens_ms))
     achieved = flop / (kavg * 1e-3) / 1e12
     ms_per_step = wall * 1e3 / K
     value = n / (wall / K)
@@ -366,13 +384,29 @@ def main():
 
     traffic = None
     lens = args.config == "lens"
+    two_phase = lens and not (args.simple or args.lanes)
     tf = ROOT / "profiles" / ("traffic_propagate_lens.json" if lens else "traffic_propagate_ballistic.json")
     if tf.exists() and n == (10_000_000 if lens else 100_000_000):
         try:
-            traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+            tj = json.loads(tf.read_text())
+            traffic = tj.get("k_lens_bytes_per_launch" if two_phase else "bytes_per_launch")
         except Exception:
             traffic = None
-    if lens:
+    walk_phase = None
+    if two_phase:
+        # the dominant kernel: k_lens (the lens and the elements after it), timed by the library's events
+        flop_lens = FLOP_PER_RK4_STEP * steps_ + FLOP_PER_FACE * faces_lens
+        ach_lens = flop_lens / (lens_avg * 1e-3) / 1e12
+        roof = {"bound": "mfma", "engine": ENGINE, "achieved": ach_lens, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                "frac": ach_lens / PEAK_FP64, "traffic": traffic}
+        # walk phase: 48 B of inputs per molecule read, 64 B queued per molecule entering the lens
+        n_enter = n - counts.get("Aperture 1", 0) - counts.get("Lens entrance", 0)
+        wbytes = 48.0 * n + 64.0 * n_enter
+        walk_phase = {"kernel": "k_propagate<PRE> (walk to and through the lens entrance, queue the lens molecules)",
+                      "ms_avg": walk_avg, "bytes_per_launch": wbytes, "molecules_queued": n_enter,
+                      "GBps": wbytes / (walk_avg * 1e-3) / 1e9, "hbm_frac": wbytes / (walk_avg * 1e-3) / 1e9 / PEAK_HBM,
+                      "faces": faces - faces_lens}
+    elif lens:
         roof = {"bound": "mfma", "engine": ENGINE, "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64, "traffic": traffic}
     else:
@@ -401,14 +435,23 @@ def main():
                      "kernel": ("k_propagate<circular apertures, no lens>" if not lens
                                 else "k_propagate<WSPLINE_HEX,RK4>" if args.simple
                                 else "k_propagate_lanes<WSPLINE_HEX,RK4>" if args.lanes
-                                else "k_propagate<PRE> + k_lens<WSPLINE_HEX,RK4> (one ctraj_propagate)"),
-                     "kernel_ms_avg": kavg,
-                     "flop_per_launch": flop, "faces_per_launch": faces, "rk4_steps_per_launch": steps_},
+                                else "k_lens<WSPLINE_HEX,RK4,circular apertures>"),
+                     "kernel_ms_avg": lens_avg if two_phase else kavg,
+                     "flop_per_launch": (FLOP_PER_RK4_STEP * steps_ + FLOP_PER_FACE * faces_lens) if two_phase else flop,
+                     "faces_per_launch": faces_lens if two_phase else faces, "rk4_steps_per_launch": steps_,
+                     "flop_rule": "add/sub/mul/div/sqrt 1, FMA 2; %d per RK4 step, %d per element face"
+                                  % (FLOP_PER_RK4_STEP, FLOP_PER_FACE),
+                     "timing": "HIP events recorded by ctraj_propagate around each kernel on its stream, "
+                               "mean over the %d timed launches" % K},
+        "propagation": {"ms_avg": kavg, "flop_per_launch": flop, "tflops": achieved, "fp64_frac": achieved / PEAK_FP64,
+                        "faces": faces, "rk4_steps": steps_},
         "region_ms_per_step": region_ms / K,
         "counts": counts,
         "detected": n_det,
         "host_draw_s": t_draw,
     }
+    if walk_phase:
+        line["walk_phase"] = walk_phase
     if args.config == "lens" and not args.no_adjoint:
         line["fwd_adjoint"] = bench_adjoint(ctx, args.adjoint_n)
     if args.config == "lens" and not args.no_batch:

@@ -805,6 +805,9 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
       if (s_first[b] != (~0ull >> 1)) atomicMin((unsigned long long *)&R.first_index[b], s_first[b]);
   if (R.work && threadIdx.x < 4 && s_work[threadIdx.x])
     atomicAdd((unsigned long long *)&R.work[threadIdx.x], s_work[threadIdx.x]);
+  // the faces of this phase (the elements after the lens) also go to their own counter
+  if (R.work && threadIdx.x == 4 && s_work[0])
+    atomicAdd((unsigned long long *)&R.work[CTRAJ_WORK_FACES_POST], s_work[0]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1264,8 +1267,11 @@ int32_t ctraj_ctx_destroy(ctraj_ctx *c) {
   if (c->pipe.p) (void)hipFree(c->pipe.p);
   if (c->pipe_cnt.p) (void)hipFree(c->pipe_cnt.p);
   if (c->sort_alt.p) (void)hipFree(c->sort_alt.p);
-  for (int i = 0; i < 64; ++i)
+  for (int i = 0; i < 64; ++i) {
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    for (int k = 0; k < 4; ++k)
+      if (c->pt_ev[i][k]) (void)hipEventDestroy(c->pt_ev[i][k]);
+  }
   (void)hipStreamDestroy(c->stream);
   delete c;
   return CTRAJ_OK;
@@ -1476,6 +1482,14 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
   if (block < 64 || block > max_block || block % 64)
     return fail(c, CTRAJ_EINVAL, "block must be a multiple of 64 up to " + std::to_string(max_block));
   HIPCHK(c, hipSetDevice(c->device));
+  // CTRAJ_RUN_PHASE_TIMES: three events of the context's ring bracket the kernels of this launch
+  hipEvent_t *pev = nullptr;
+  if (R->flags & CTRAJ_RUN_PHASE_TIMES) {
+    const int slot = (int)(c->pt_next % 64);
+    for (int k = 0; k < 4; ++k)
+      if (!c->pt_ev[slot][k]) HIPCHK(c, hipEventCreate(&c->pt_ev[slot][k]));
+    pev = c->pt_ev[slot];
+  }
   if (pipeline) {
     LensFn kl = lens_kernel(c->force, c->integ, kinds_mask(c->h_elems, lens_e + 1, c->n_elems));
     if (!kl) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
@@ -1504,9 +1518,6 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     P.n_pre_waves = nw;
     P.lens_e = lens_e;
     if ((rc = ctraj_fill_i64(c, (int64_t *)P.claim, 0, 8 * 32)) != CTRAJ_OK) return rc;
-    hipLaunchKernelGGL(kpre, dim3(gpre), dim3(blk), lds_walk, c->stream, B, kt, *R, 0, P);
-    HIPCHK(c, hipGetLastError());
-    // phase 2: the lens and the elements after it
     const size_t lds_lens = lds;
     if ((rc = lds_limit(c, (const void *)kl, lds_lens)) != CTRAJ_OK) return rc;
     if ((rc = block_ok(c, (const void *)kl, block)) != CTRAJ_OK) return rc;
@@ -1516,6 +1527,11 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
       c->last_run = *R;
       c->have_last_run = true;
     }
+    if (pev) HIPCHK(c, hipEventRecord(pev[0], c->stream));
+    hipLaunchKernelGGL(kpre, dim3(gpre), dim3(blk), lds_walk, c->stream, B, kt, *R, 0, P);
+    HIPCHK(c, hipGetLastError());
+    if (pev) HIPCHK(c, hipEventRecord(pev[1], c->stream));
+    // phase 2: the lens and the elements after it
     const int refill = R->refill > 0 ? std::min(R->refill, 64) : 12;
     const int grid = grid_for(c, (const void *)kl, block, lds_lens, (int64_t)1 << 40, 0);
     const StepK K = step_k(c->h_elems[lens_e].p[2], c->h_elems[lens_e].p[4], c->phys.g);
@@ -1545,6 +1561,10 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     LA.claim = 128;
     LA.gk = hex_folds(ks) ? -c->phys.g / ks : 0.0;
     const int grid = grid_for(c, (const void *)k, block, lds, R->n, R->grid);
+    if (pev) {
+      HIPCHK(c, hipEventRecord(pev[0], c->stream));
+      HIPCHK(c, hipEventRecord(pev[1], c->stream));
+    }
     hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, (const ctraj_run *)c->d_run, LA);
   } else {
     PropFn k = prop_kernel(c->force, c->integ, n_lens > 0, kinds_mask(c->h_elems, 0, c->n_elems), wants_outputs(R),
@@ -1560,9 +1580,36 @@ int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
     rc = block_ok(c, (const void *)k, block);
     if (rc) return rc;
     const int grid = grid_for(c, (const void *)k, block, lds_all, R->n, R->grid);
+    if (pev) {
+      HIPCHK(c, hipEventRecord(pev[0], c->stream));
+      HIPCHK(c, hipEventRecord(pev[1], c->stream));
+    }
     hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds_all, c->stream, B, kt, *R, aoi_off, PipeArgs{});
   }
   HIPCHK(c, hipGetLastError());
+  if (pev) {
+    HIPCHK(c, hipEventRecord(pev[2], c->stream));
+    HIPCHK(c, hipEventRecord(pev[3], c->stream));
+    if (++c->pt_next - c->pt_read > 64) c->pt_read = c->pt_next - 64;
+  }
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_phase_times(ctraj_ctx *c, int32_t max, float *walk_ms, float *lens_ms, float *post_ms, int32_t *count) {
+  if (!c || !count || max < 0 || (max > 0 && (!walk_ms || !lens_ms || !post_ms)))
+    return fail(c, CTRAJ_EINVAL, "null argument");
+  *count = 0;
+  const int64_t first = std::max(c->pt_read, c->pt_next - (int64_t)max);
+  int32_t k = 0;
+  for (int64_t j = first; j < c->pt_next; ++j, ++k) {
+    hipEvent_t *e = c->pt_ev[j % 64];
+    HIPCHK(c, hipEventSynchronize(e[3]));
+    HIPCHK(c, hipEventElapsedTime(&walk_ms[k], e[0], e[1]));
+    HIPCHK(c, hipEventElapsedTime(&lens_ms[k], e[1], e[2]));
+    HIPCHK(c, hipEventElapsedTime(&post_ms[k], e[2], e[3]));
+  }
+  c->pt_read = c->pt_next;
+  *count = k;
   return CTRAJ_OK;
 }
 

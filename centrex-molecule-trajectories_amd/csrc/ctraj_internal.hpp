@@ -41,6 +41,10 @@ struct ctraj_ctx {
   ctraj_run last_run{};
   bool have_last_run = false;
   hipEvent_t ev[64];
+  // CTRAJ_RUN_PHASE_TIMES: per launch, events before the first kernel, between the kernels and after the
+  // last one, in a ring of 64 launches; launches [pt_read, pt_next) are not yet read
+  hipEvent_t pt_ev[64][4] = {};
+  int64_t pt_next = 0, pt_read = 0;
   // smooth objective: per-wave checkpoint/recompute store and per-item partial sums
   ctraj_devbuf sm_ws, sm_part;
   // radix sort scratch (ctraj_sort_i64)

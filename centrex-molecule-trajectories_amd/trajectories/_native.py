@@ -25,8 +25,8 @@ FORCE_RTABLE, FORCE_WSPLINE_HEX, FORCE_WSPLINE_LIN = 0, 1, 2
 INTEG_REF, INTEG_RK4 = 0, 1
 
 ST_OOT, ST_CAPPED = 1, 2
-WORK_FACES, WORK_STEPS, WORK_OOT, WORK_CAPPED, WORK_N = 0, 1, 2, 3, 8
-RUN_STATUS_OR, RUN_SIMPLE, RUN_LANES = 1, 2, 4
+WORK_FACES, WORK_STEPS, WORK_OOT, WORK_CAPPED, WORK_FACES_POST, WORK_N = 0, 1, 2, 3, 4, 8
+RUN_STATUS_OR, RUN_SIMPLE, RUN_LANES, RUN_PHASE_TIMES = 1, 2, 4, 8
 REC_ROWS, REC_ZGRID, REC_ZGRID_MOL = 1, 2, 3
 H2D, D2H, D2D = 1, 2, 3
 
@@ -113,6 +113,18 @@ class Record(C.Structure):
     ]
 
 
+class Sample(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("row_off", dptr),
+        ("n_rows", C.c_int64),
+        ("rows", dptr),
+        ("z", C.c_double),
+        ("out", dptr * 6),
+        ("valid", dptr),
+    ]
+
+
 class BatchCfg(C.Structure):
     _fields_ = [
         ("elem_begin", C.c_int32),
@@ -180,7 +192,10 @@ SIGNATURES = [
     ("ctraj_set_rtable", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]),
     ("ctraj_set_wspline", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
     ("ctraj_propagate", C.c_int32, [C.c_void_p, C.POINTER(Run)]),
+    ("ctraj_phase_times", C.c_int32,
+     [C.c_void_p, C.c_int32] + [C.POINTER(C.c_float)] * 3 + [C.POINTER(C.c_int32)]),
     ("ctraj_record_trajectories", C.c_int32, [C.c_void_p, C.POINTER(Record)]),
+    ("ctraj_sample_rows", C.c_int32, [C.c_void_p, C.POINTER(Sample)]),
     ("ctraj_draw_gaussian", C.c_int32,
      [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("ctraj_batch_create", C.c_int32,

@@ -11,7 +11,7 @@ CSRC = PKG.parent / "csrc"
 INCLUDE = PKG.parent.parent / "include"
 OUT = PKG / "libctraj.so"
 
-SOURCES = ["ctraj.hip", "batch.hip", "adjoint.hip", "sort.hip"]
+SOURCES = ["ctraj.hip", "batch.hip", "adjoint.hip", "sort.hip", "resample.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off",  # every fused multiply-add in the kernels is written out explicitly

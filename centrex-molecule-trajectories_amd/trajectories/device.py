@@ -114,6 +114,17 @@ class Context:
                 self.handle)
         return float(ms.value)
 
+    def phase_times(self, max_n: int = 64):
+        """Kernel times (ms) of the launches made with RUN_PHASE_TIMES since the last call, oldest first:
+        (walk to the lens, lens, walk after the lens) arrays; a single-kernel run has all its time in the
+        lens array."""
+        bufs = [(C.c_float * max_n)() for _ in range(3)]
+        cnt = C.c_int32()
+        N.check("ctraj_phase_times", N.lib().ctraj_phase_times(self.handle, int(max_n), *bufs, C.byref(cnt)),
+                self.handle)
+        k = cnt.value
+        return tuple(np.array(b[:k], dtype=np.float64) for b in bufs)
+
     def close(self):
         if getattr(self, "handle", None):
             N.lib().ctraj_ctx_destroy(self.handle)

@@ -15,8 +15,11 @@ from dataclasses import dataclass
 from pathlib import Path
 from typing import List
 
+import warnings
+
 import numpy as np
 
+from . import _native as N
 from .beamline import Beamline
 from .distributions import CeNTREXPositionDistribution, CeNTREXVelocityDistribution, Distribution
 from .engine import Engine
@@ -80,15 +83,36 @@ class MoleculeList(Sequence):
 
     BLOCK = 2048
 
-    def __init__(self, engine: Engine, init: np.ndarray, labels: np.ndarray, cap: int):
+    def __init__(self, engine: Engine, init: np.ndarray, labels: np.ndarray, cap: int, states=None):
         self._engine = engine
         self._init = init  # [7][n]: x, y, z, vx, vy, vz, t
         self._labels = labels
+        self._states = None if states is None else np.ascontiguousarray(states, dtype=np.uint8)
         self._cap = cap
         self._cache = {}
 
     def __len__(self):
         return self._init.shape[1]
+
+    @property
+    def aperture_hits(self) -> list:
+        """The stopping label of every kept molecule, in order (without materialising them)."""
+        names = self._engine.labels
+        return [names[int(k)] for k in self._labels]
+
+    def sample_at(self, z: float, select=None):
+        """State of the kept molecules ``select`` (all if None) at ``z`` by the post-processing rule
+        (post_processing.py:20-140): one device launch that re-walks them with the z-grid recorder.
+        Returns ([6][m] x, y, vx, vy, vz, t; NaN where a molecule ends before z, valid [m])."""
+        idx = np.arange(len(self)) if select is None else np.asarray(select, dtype=np.int64)
+        if idx.size == 0:
+            return np.zeros((6, 0)), np.zeros(0, bool)
+        I = self._init[:, idx]
+        st = None if self._states is None else self._states[idx]
+        data, written, lab = self._engine.zgrid(*I, state=st, z_start=float(z), z_step=1.0, n_samples=1)
+        if not np.array_equal(lab, self._labels[idx]):
+            raise RuntimeError("re-walked kept molecules stopped at other labels than in the run")
+        return np.ascontiguousarray(data[:, 0, :]), written.astype(bool)
 
     def _block(self, b):
         if b not in self._cache:
@@ -96,7 +120,10 @@ class MoleculeList(Sequence):
                 self._cache.clear()
             s = slice(b * self.BLOCK, min((b + 1) * self.BLOCK, len(self)))
             I = self._init[:, s]
-            rows, nrows, lab = self._engine.record_rows(*I, cap=self._cap)
+            st = None if self._states is None else self._states[s]
+            rows, nrows, lab = self._engine.record_rows(*I, state=st, cap=self._cap)
+            if not np.array_equal(lab, self._labels[s]):
+                raise RuntimeError("re-recorded kept molecules stopped at other labels than in the run")
             self._cap = max(self._cap, int(nrows.max()) if nrows.size else 0)
             names = self._engine.labels
             mols = []
@@ -165,6 +192,7 @@ class RunOutput:
     molecules: MoleculeList
     work: np.ndarray
     n_out_of_table: int
+    n_capped: int = 0
 
 
 def simulate_arrays(beamline: Beamline, X: np.ndarray, V: np.ndarray, apertures_of_interest=(), *, ctx=None,
@@ -178,7 +206,7 @@ def simulate_arrays(beamline: Beamline, X: np.ndarray, V: np.ndarray, apertures_
     first = np.full(nl, np.iinfo(np.int64).max, np.int64)
     work = np.zeros(8, np.uint64)
     aoi_idx, aoi_lab, aoi_fin = [], [], []
-    n_oot = 0
+    n_oot = n_capped = 0
     for s in range(0, max(n, 1), chunk):
         e = min(n, s + chunk)
         if e <= s:
@@ -189,7 +217,8 @@ def simulate_arrays(beamline: Beamline, X: np.ndarray, V: np.ndarray, apertures_
         tally += r["tally"]
         first = np.minimum(first, r["first_index"])
         work += r["work"]
-        n_oot += int(np.count_nonzero(r["status"] & 1))
+        n_oot += int(np.count_nonzero(r["status"] & N.ST_OOT))
+        n_capped += int(np.count_nonzero(r["status"] & N.ST_CAPPED))
         if "aoi_index" in r:
             o = np.argsort(r["aoi_index"], kind="stable")
             aoi_idx.append(r["aoi_index"][o])
@@ -198,6 +227,9 @@ def simulate_arrays(beamline: Beamline, X: np.ndarray, V: np.ndarray, apertures_
     if n_oot and raise_out_of_table:
         raise ValueError(f"{n_oot} molecules evaluated the lens force outside its table "
                          "(the reference's interp1d raises ValueError here)")
+    if n_capped:
+        warnings.warn(f"{n_capped} molecules needed more than max_lens_steps full RK4 steps in a lens; their "
+                      "step count was cut there and one partial step covered the rest of the lens", RuntimeWarning)
     idx = np.concatenate(aoi_idx) if aoi_idx else np.zeros(0, np.int64)
     lab = np.concatenate(aoi_lab) if aoi_lab else np.zeros(0, np.int32)
     fin = {k: np.concatenate([d[k] for d in aoi_fin]) for k in aoi_fin[0]} if aoi_fin else {}
@@ -210,8 +242,9 @@ def simulate_arrays(beamline: Beamline, X: np.ndarray, V: np.ndarray, apertures_
     vz = V[2, idx] if idx.size else np.array([1.0])
     vzmin = float(vz[vz > 0].min()) if np.any(vz > 0) else 1.0
     cap = 10 + sum(el.rows_needed(vzmin) if hasattr(el, "rows_needed") else el.N_steps() for el in beamline.elements)
-    mols = MoleculeList(eng, init, lab, cap)
-    return RunOutput(n, list(eng.cb.labels), tally, first, counter, idx, lab, fin, mols, work, n_oot)
+    kept_states = None if states is None else np.asarray(states, dtype=np.uint8)[idx]
+    mols = MoleculeList(eng, init, lab, cap, states=kept_states)
+    return RunOutput(n, list(eng.cb.labels), tally, first, counter, idx, lab, fin, mols, work, n_oot, n_capped)
 
 
 @dataclass

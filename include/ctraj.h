@@ -93,6 +93,8 @@ typedef struct ctraj_soa {
 #define CTRAJ_WORK_STEPS 1  /* lens integration steps taken */
 #define CTRAJ_WORK_OOT 2    /* molecules with CTRAJ_ST_OOT */
 #define CTRAJ_WORK_CAPPED 3 /* molecules with CTRAJ_ST_CAPPED */
+#define CTRAJ_WORK_FACES_POST 4 /* of CTRAJ_WORK_FACES, the faces of the elements after the lens when a run is
+                                 * split into kernels (0 for a single-kernel run) */
 #define CTRAJ_WORK_N 8
 
 typedef struct ctraj_ctx ctraj_ctx;
@@ -142,6 +144,7 @@ int32_t ctraj_set_wspline(ctraj_ctx *ctx, int32_t n_states, int32_t n_seg, const
 #define CTRAJ_RUN_STATUS_OR 1 /* OR status bits into run->status instead of overwriting */
 #define CTRAJ_RUN_SIMPLE 2    /* one molecule per thread start to finish (no lane refill) */
 #define CTRAJ_RUN_LANES 4     /* one lens: the single-kernel lane-refill walker instead of the two phases */
+#define CTRAJ_RUN_PHASE_TIMES 8 /* time the launch's kernels with HIP events on the stream (ctraj_phase_times) */
 typedef struct ctraj_run {
   int64_t n;             /* molecules */
   int64_t index_base;    /* global index of molecule 0 (for first_index / aoi_index in chunked runs) */
@@ -170,6 +173,14 @@ typedef struct ctraj_run {
   int32_t refill;        /* lane refill: idle lanes per wavefront that trigger a refill (0 = 12 two-phase, 24 single kernel) */
 } ctraj_run;
 int32_t ctraj_propagate(ctraj_ctx *ctx, const ctraj_run *run);
+/* Device times (ms) of the kernels of the launches made with CTRAJ_RUN_PHASE_TIMES since the last call, oldest
+ * first: walk_ms[i] is the walk to and through the lens entrance of a run split into kernels, lens_ms[i] its lens
+ * kernel and post_ms[i] its walk of the elements after the lens (walk and post are 0 for a single-kernel run, whose
+ * whole kernel is lens_ms[i]).  No counterpart in the reference (it has no device).  Waits for those launches;
+ * writes at most max entries (the most recent ones; at most 64 are kept) to the host arrays, sets *count, and
+ * forgets every pending measurement. */
+int32_t ctraj_phase_times(ctraj_ctx *ctx, int32_t max, float *walk_ms, float *lens_ms, float *post_ms,
+                          int32_t *count);
 
 /* ---- saved trajectories (molecule.py:111-167 rows; post_processing.py:20-140 resampling) ---- */
 #define CTRAJ_REC_ROWS 1  /* every trajectory row; buf = [10][cap][n]: x,y,z,vx,vy,vz,ax,ay,az,t */
@@ -193,6 +204,25 @@ typedef struct ctraj_record {
   int32_t reserved;
 } ctraj_record;
 int32_t ctraj_record_trajectories(ctraj_ctx *ctx, const ctraj_record *rec);
+
+/* ---- stored trajectories at one z: find_radial_pos_dist / find_vel_dist (post_processing.py:20-140) ----
+ * Molecule j owns rows [row_off[j], row_off[j+1]) of `rows`, each row 10 doubles x, y, z, vx, vy, vz, ax, ay, az,
+ * t, in the order the reference appends them (molecule.py:133-144) and with its NaN rows dropped
+ * (molecule.py:160-167).  out[f][j] for f = x, y, vx, vy, vz, t (a NULL out[f] is not written) is the molecule's
+ * state at z by the reference's rule: valid[j] = 0 and NaN outputs when its last row lies before z (:42-43);
+ * otherwise the first row with that z exactly (:52-56), else the row before the first row with !(z_row < z)
+ * advanced to z by take_timestep with its stored acceleration (:61-72, :9-17), the last row when that is the
+ * first row (Python's index -1).  row_off is checked on the host (one copy) before the launch. */
+typedef struct ctraj_sample {
+  int64_t n;              /* molecules */
+  const int64_t *row_off; /* device [n + 1], non-decreasing, row_off[0] >= 0, row_off[n] <= n_rows */
+  int64_t n_rows;         /* rows in `rows` */
+  const double *rows;     /* device [n_rows][10] */
+  double z;
+  double *out[6];         /* device [n] each */
+  uint8_t *valid;         /* device [n] (optional) */
+} ctraj_sample;
+int32_t ctraj_sample_rows(ctraj_ctx *ctx, const ctraj_sample *s);
 
 /* ---- device source sampling: Distribution.draw (distributions.py:32-162) as Philox4x32-10 + Box-Muller ---- */
 /* out[d][i] = mean[d] + sigma[d] * N(0,1) drawn from counter (i + offset, pair(d)) with key seed; host mean/sigma,

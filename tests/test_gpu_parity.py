@@ -233,6 +233,7 @@ def test_lane_refill_kernel_matches_simple_kernel():
         keep = ("Detected", "Inside lens")
         base = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], state=st, flags=N.RUN_SIMPLE, want_accel=True,
                              aoi_labels=keep)
+        post_faces = set()
         for flags, refill in ((0, 1), (0, 24), (0, 64), (N.RUN_LANES, 1), (N.RUN_LANES, 16), (N.RUN_LANES, 64)):
             res = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], state=st, refill=refill, want_accel=True,
                                 aoi_labels=keep, flags=flags)
@@ -244,13 +245,18 @@ def test_lane_refill_kernel_matches_simple_kernel():
                 np.testing.assert_array_equal(res["out_a"][q], base["out_a"][q])
             np.testing.assert_array_equal(res["tally"], base["tally"])
             np.testing.assert_array_equal(res["first_index"], base["first_index"])
-            np.testing.assert_array_equal(res["work"], base["work"])
+            np.testing.assert_array_equal(res["work"][:4], base["work"][:4])
+            if flags == 0:  # the multi-kernel path counts the faces after the lens apart, the same at every threshold
+                post_faces.add(int(res["work"][N.WORK_FACES_POST]))
+            else:
+                assert int(res["work"][N.WORK_FACES_POST]) == 0
             o = np.argsort(res["aoi_index"])
             ob = np.argsort(base["aoi_index"])
             np.testing.assert_array_equal(res["aoi_index"][o], base["aoi_index"][ob])
             np.testing.assert_array_equal(res["aoi_label"][o], base["aoi_label"][ob])
             for k in FIELDS:
                 np.testing.assert_array_equal(res["aoi_out"][k][o], base["aoi_out"][k][ob])
+        assert len(post_faces) == 1 and 0 < post_faces.pop() < int(base["work"][N.WORK_FACES])
 
 
 def test_small_and_empty_runs():

@@ -267,6 +267,48 @@ def test_zgrid_resampling_of_a_ballistic_molecule():
     np.testing.assert_allclose(s[5], t, rtol=1e-12, atol=1e-18)
 
 
+def _row(z, x=0.0, y=0.0, vx=0.0, vy=0.0, vz=100.0, ax=0.0, ay=-9.81, t=0.0):
+    return [x, y, z, vx, vy, vz, ax, 0.0 if ay is None else ay, 0.0, t]
+
+
+def test_resampling_rule_special_cases():
+    """post_processing.py:40-75 on hand-built rows whose answers are known without the rule's code:
+    the exact-z row is returned as stored even when it is off the parabola; of two rows at that z the
+    first wins; the extrapolated row is the one before the FIRST row with z >= z_q (not the last row
+    below z_q, which differs when z goes back); a z_q before the first row uses the last row (Python's
+    index -1); a z_q beyond the last row gives nothing."""
+    rows = np.array([
+        _row(0.0, x=1e-3, vx=2.0, t=0.0),
+        _row(0.4, x=5e-3, vx=2.0, t=0.004),
+        _row(0.4, x=9e-3, vx=7.0, t=0.005),   # a second row at z = 0.4 (an L = 0 face): not chosen
+        _row(0.2, x=-3e-3, vx=-1.0, t=0.006),  # z goes back
+        _row(1.0, x=4e-3, y=-2e-3, vx=0.5, vy=-0.25, vz=200.0, t=0.01),
+    ])
+    n = rows.shape[0]
+
+    def at(zq):
+        return O.zgrid_from_rows(rows, n, zq, 1.0, 1)[:, 0]
+
+    # exact hit: the first row at 0.4, as stored
+    np.testing.assert_array_equal(at(0.4), [5e-3, 0.0, 2.0, 0.0, 100.0, 0.004])
+    # z_q = 0.3: the first row with z >= 0.3 is row 1, so row 0 is advanced by dt = 0.3/100
+    dt = 0.3 / 100.0
+    np.testing.assert_allclose(at(0.3), [1e-3 + 2.0 * dt, -9.81 * dt * dt / 2, 2.0, -9.81 * dt, 100.0, dt],
+                               rtol=1e-13, atol=1e-18)
+    # z_q = 0.7: rows 0..3 are below, row 4 is the first >= 0.7, so row 3 (z = 0.2) is advanced
+    dt = (0.7 - 0.2) / 100.0
+    np.testing.assert_allclose(at(0.7), [-3e-3 - 1.0 * dt, -9.81 * dt * dt / 2, -1.0, -9.81 * dt, 100.0,
+                                         0.006 + dt], rtol=1e-13, atol=1e-18)
+    # z_q before the first row: index -1, the last row advanced backwards
+    dt = (-0.5 - 1.0) / 200.0
+    np.testing.assert_allclose(at(-0.5), [4e-3 + 0.5 * dt, -2e-3 - 0.25 * dt - 9.81 * dt * dt / 2, 0.5,
+                                          -0.25 - 9.81 * dt, 200.0, 0.01 + dt], rtol=1e-13, atol=1e-18)
+    # beyond the last row: skipped
+    assert np.all(np.isnan(at(1.0 + 1e-9)))
+    # exactly at the last row's z: returned as stored
+    np.testing.assert_array_equal(at(1.0), [4e-3, -2e-3, 0.5, -0.25, 200.0, 0.01])
+
+
 def test_philox_known_answers():
     """Published Philox4x32-10 known-answer vectors (Random123 kat_vectors)."""
     kats = [
