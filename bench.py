@@ -392,20 +392,27 @@ def main():
             traffic = tj.get("k_lens_bytes_per_launch" if two_phase else "bytes_per_launch")
         except Exception:
             traffic = None
-    walk_phase = None
+    walk_phase = post_phase = None
     if two_phase:
-        # the dominant kernel: k_lens (the lens and the elements after it), timed by the library's events
-        flop_lens = FLOP_PER_RK4_STEP * steps_ + FLOP_PER_FACE * faces_lens
+        # the dominant kernel: k_lens (the lens integration only), timed by the library's events
+        flop_lens = FLOP_PER_RK4_STEP * steps_
         ach_lens = flop_lens / (lens_avg * 1e-3) / 1e12
         roof = {"bound": "mfma", "engine": ENGINE, "achieved": ach_lens, "peak": PEAK_FP64, "unit": "TFLOP/s",
                 "frac": ach_lens / PEAK_FP64, "traffic": traffic}
-        # walk phase: 48 B of inputs per molecule read, 64 B queued per molecule entering the lens
+        # the other two kernels stream HBM: PRE reads 48 B of inputs per molecule and queues 64 B per
+        # molecule entering the lens; the lens kernel rewrites each entry in place; POST reads the
+        # entries back and writes each kept molecule (8 B index + 7 fields)
         n_enter = n - counts.get("Aperture 1", 0) - counts.get("Lens entrance", 0)
         wbytes = 48.0 * n + 64.0 * n_enter
         walk_phase = {"kernel": "k_propagate<PRE> (walk to and through the lens entrance, queue the lens molecules)",
                       "ms_avg": walk_avg, "bytes_per_launch": wbytes, "molecules_queued": n_enter,
                       "GBps": wbytes / (walk_avg * 1e-3) / 1e9, "hbm_frac": wbytes / (walk_avg * 1e-3) / 1e9 / PEAK_HBM,
                       "faces": faces - faces_lens}
+        pbytes = 64.0 * n_enter + 64.0 * n_det
+        post_phase = {"kernel": "k_propagate<POST> (elements after the lens, tallies and kept molecules of the lens "
+                                "molecules)", "ms_avg": float(np.mean(post_ms)), "bytes_per_launch": pbytes,
+                      "GBps": pbytes / (float(np.mean(post_ms)) * 1e-3) / 1e9,
+                      "hbm_frac": pbytes / (float(np.mean(post_ms)) * 1e-3) / 1e9 / PEAK_HBM, "faces": faces_lens}
     elif lens:
         roof = {"bound": "mfma", "engine": ENGINE, "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64, "traffic": traffic}
@@ -435,10 +442,10 @@ def main():
                      "kernel": ("k_propagate<circular apertures, no lens>" if not lens
                                 else "k_propagate<WSPLINE_HEX,RK4>" if args.simple
                                 else "k_propagate_lanes<WSPLINE_HEX,RK4>" if args.lanes
-                                else "k_lens<WSPLINE_HEX,RK4,circular apertures>"),
+                                else "k_lens<WSPLINE_HEX,RK4>"),
                      "kernel_ms_avg": lens_avg if two_phase else kavg,
-                     "flop_per_launch": (FLOP_PER_RK4_STEP * steps_ + FLOP_PER_FACE * faces_lens) if two_phase else flop,
-                     "faces_per_launch": faces_lens if two_phase else faces, "rk4_steps_per_launch": steps_,
+                     "flop_per_launch": (FLOP_PER_RK4_STEP * steps_) if two_phase else flop,
+                     "faces_per_launch": 0 if two_phase else faces, "rk4_steps_per_launch": steps_,
                      "flop_rule": "add/sub/mul/div/sqrt 1, FMA 2; %d per RK4 step, %d per element face"
                                   % (FLOP_PER_RK4_STEP, FLOP_PER_FACE),
                      "timing": "HIP events recorded by ctraj_propagate around each kernel on its stream, "
@@ -452,6 +459,7 @@ def main():
     }
     if walk_phase:
         line["walk_phase"] = walk_phase
+        line["post_phase"] = post_phase
     if args.config == "lens" and not args.no_adjoint:
         line["fwd_adjoint"] = bench_adjoint(ctx, args.adjoint_n)
     if args.config == "lens" and not args.no_batch:

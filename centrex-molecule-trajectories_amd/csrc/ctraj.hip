@@ -67,14 +67,17 @@ struct InMol {
   int st;
   int status;
   int64_t idx;
+  int outcome;  // POST: LENS_* of the queue entry (LENS_QUEUED: an empty slot)
 };
 
-// ---- two-phase propagation of beamlines with one lens ----
+// ---- propagation of beamlines with one lens in three kernels ----
 // PRE walks every molecule up to and through the lens entrance and queues the molecules that enter
 // the lens, each wavefront into its own region of the queue; k_lens integrates the queue with lane
-// refill and walks the molecules that leave the lens through the remaining elements.  A molecule
-// is recorded (outputs, tallies, kept-molecule compaction) by the phase in which it finishes.
-enum : int { PHASE_FULL = 0, PHASE_PRE = 1 };
+// refill and writes each molecule's state where the lens left it back into its queue entry, with
+// the outcome (stopped by the bore, or left the lens at z1); POST walks the molecules that left the
+// lens through the remaining elements and records every lens molecule.  A molecule is recorded
+// (outputs, tallies, kept-molecule compaction) by PRE if it stops before the lens, else by POST.
+enum : int { PHASE_FULL = 0, PHASE_PRE = 1, PHASE_POST = 2 };
 
 // Lens step constants: the half and sixth of the step (as rk4_step forms them) and, for a folding
 // hexapole lens, ks times the half step, the step and the sixth, and gk = -g/ks (rk4_step_hex_k).
@@ -96,6 +99,9 @@ struct PipeArgs {
   int lens_e;
 };
 
+// Outcome of a lens molecule, written by k_lens into its queue entry's key (queue_put_done).
+enum : int { LENS_QUEUED = 0, LENS_STOPPED = 1, LENS_LEFT = 2 };
+
 __device__ __forceinline__ void load_mol(const ctraj_run &R, const KBeam &B, int64_t i, InMol &q) {
   q.x = R.in.x[i];
   q.y = R.in.y[i];
@@ -111,6 +117,23 @@ __device__ __forceinline__ void load_mol(const ctraj_run &R, const KBeam &B, int
                                          // a use right after the load would wait for the whole prefetch
   q.status = 0;
   q.idx = i;
+}
+
+// POST: queue slot e (entry k of PRE wavefront w's region); slots past the region's count are empty.
+__device__ __forceinline__ void load_slot(const PipeArgs &P, int64_t e, InMol &q) {
+  const int64_t w = e / P.region;
+  const bool filled = e - w * P.region < (int64_t)P.wcount[w];
+  q.outcome = LENS_QUEUED;
+  if (filled) {
+    MState m;
+    int64_t key;
+    queue_get_done(P.q, e, m, key);
+    q.x = m.x; q.y = m.y; q.z = m.z; q.vx = m.vx; q.vy = m.vy; q.vz = m.vz; q.t = m.t;
+    q.idx = key & QKEY_INDEX;
+    q.status = (int)((key >> 40) & 0xff);
+    q.outcome = (int)((key >> 48) & 0xff);
+    q.st = (int)((uint64_t)key >> 56);
+  }
 }
 
 // Inputs of a run that passes positions and velocities only (no time, stored acceleration or state
@@ -132,13 +155,16 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
   extern __shared__ __attribute__((aligned(16))) double lds[];
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
-  __shared__ unsigned long long s_work[4];
+  __shared__ unsigned long long s_work[5];
   const int nbins = B.n_states * B.n_labels;
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0;
   for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x) s_first[b] = ~0ull >> 1;
-  if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
+  if (threadIdx.x < 5) s_work[threadIdx.x] = 0;
   Tables T{};
   if constexpr (INTEG != CTRAJ_INTEG_NONE && PHASE == PHASE_FULL) T = stage_tables(lds, KT);
+  // POST stages the lens tables only when stored accelerations are written (the host passes empty
+  // tables otherwise): a molecule stopped by the bore stores the force where it stopped
+  if constexpr (PHASE == PHASE_POST && OUT) T = stage_tables(lds, KT);
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
   AoiStage A;
@@ -154,8 +180,8 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
 
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   WalkCounters wc{0u, 0u};
-  uint32_t n_oot = 0, n_cap = 0;
-  const int64_t n_items = R.n;
+  uint32_t n_oot = 0, n_cap = 0, faces_post = 0;
+  const int64_t n_items = PHASE == PHASE_POST ? (int64_t)P.n_pre_waves * P.region : R.n;
   const int64_t gwave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   int64_t qn = 0;  // PRE: entries this wavefront has queued
   // PRE: the lens entrance, read once with scalar loads (a vector load of it in the loop would wait
@@ -168,11 +194,13 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
     lens_r = L.p[0];
     lens_hit = L.label_hit;
   }
+  int lens_inside = 0;
+  if constexpr (PHASE == PHASE_POST) lens_inside = elem_at(B.elems, P.lens_e).label_inside;
   const int64_t j0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t w0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
   // one grid-stride iteration: molecule j (inputs in q, loaded earlier), then q is refilled
   auto body = [&](InMol &q, const int64_t j, const int64_t wbase) {
-    const bool active = j < n_items;
+    const bool active = j < n_items && (PHASE != PHASE_POST || q.outcome != LENS_QUEUED);
     MState m;
     m.x = q.x; m.y = q.y; m.z = q.z; m.vx = q.vx; m.vy = q.vy; m.vz = q.vz;
     int st = 0;
@@ -192,8 +220,12 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
       aoi_n = 0;
       aoi_full = false;
     }
+    const int outcome = q.outcome;
     if constexpr (MIN) {  // the molecule two iterations ahead; unconditional (the last index repeats at the end)
       load_min(R, j + 2 * stride < n_items ? j + 2 * stride : n_items - 1, q);
+    } else if (PHASE == PHASE_POST) {
+      if (j + stride < n_items) load_slot(P, j + stride, q);
+      else q.outcome = LENS_QUEUED;
     } else if (j + stride < n_items) {
       load_mol(R, B, j + stride, q);  // prefetch the next molecule
     }
@@ -211,6 +243,27 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
           ++wc.faces;
           if (rho_of(m.x, m.y) > lens_r) label = lens_hit;
           else enq = true;
+        }
+      } else if constexpr (PHASE == PHASE_POST) {
+        if (outcome == LENS_LEFT) {
+          // the row at z1 stores (0, -g, 0) (configuration mode); the reference lens's exit hop was made
+          // by k_lens with its stored a = l1 and ended with the same row
+          m.ax = 0.0; m.ay = -B.g; m.az = 0.0;
+          label = B.label_detected;
+          const uint32_t f0 = wc.faces;
+          walk_plain<KM, true>(B.elems, P.lens_e + 1, B.n_elems, B.g, m, label, wc);
+          faces_post += wc.faces - f0;
+        } else {
+          label = lens_inside;
+          if constexpr (OUT && INTEG == CTRAJ_INTEG_RK4) {
+            // stopped by the bore: the row stores the force at the stopping point
+            if (R.out_a[0] || R.out_a[1] || R.out_a[2]) {
+              const Force<FORCE> F = make_force<FORCE>(elem_at(B.elems, P.lens_e), B.g);
+              uint32_t st2 = 0;
+              F(T, ws_of(T, st), m.x, m.y, m.ax, m.ay, st2);
+              m.az = 0.0;
+            }
+          }
         }
       }
     }
@@ -277,7 +330,12 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
     }
   } else {
     InMol q;
-    if (j0 < n_items) load_mol(R, B, j0, q);
+    q.outcome = LENS_QUEUED;
+    if constexpr (PHASE == PHASE_POST) {
+      if (j0 < n_items) load_slot(P, j0, q);
+    } else {
+      if (j0 < n_items) load_mol(R, B, j0, q);
+    }
     for (int64_t wbase = w0, j = j0; wbase < n_items; wbase += stride, j += stride) body(q, j, wbase);
   }
   if (R.aoi_mask && aoi_n > 0) aoi_flush_wave(R, A, aoi_n, lane);
@@ -289,6 +347,7 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
     if (wc.steps) atomicAdd(&s_work[1], (unsigned long long)wc.steps);
     if (n_oot) atomicAdd(&s_work[2], (unsigned long long)n_oot);
     if (n_cap) atomicAdd(&s_work[3], (unsigned long long)n_cap);
+    if (faces_post) atomicAdd(&s_work[4], (unsigned long long)faces_post);
   }
   __syncthreads();
   if (R.tally)
@@ -297,7 +356,7 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
   if (R.first_index)
     for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x)
       if (s_first[b] != (~0ull >> 1)) atomicMin((unsigned long long *)&R.first_index[b], s_first[b]);
-  if (R.work && threadIdx.x < 4 && s_work[threadIdx.x])
+  if (R.work && threadIdx.x < 5 && s_work[threadIdx.x])
     atomicAdd((unsigned long long *)&R.work[threadIdx.x], s_work[threadIdx.x]);
 }
 
@@ -562,35 +621,38 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
 }
 
 // ---------------------------------------------------------------------------------------------
-// k_lens: the lens and what follows it.  Persistent wavefronts integrate the lens-entry queue
-// with lane refill: every iteration advances all lanes inside the lens by one step; when at least
-// `refill_min` lanes are idle (or none is in the lens), a convergent section walks the molecules
-// that left the lens through the remaining elements, records every finished molecule (outputs,
-// tallies, kept-molecule compaction) and gives idle lanes the next queued molecules.  Wavefronts
-// claim the queue in 64-entry chunks, region by region, from eight counters: counter g serves the
-// regions of the PRE workgroups dispatched to XCD g (workgroup b runs on XCD b mod 8), and a
-// wavefront starts with its own XCD's counter and moves to the next one when it runs dry, which
-// spreads the claim atomics over eight addresses.  Chunk c of group g is entries
-// [64 (c % cpr), +64) of the group's region c / cpr (cpr = region / 64), of which the first wcount
-// are filled, so a claim needs no search and a chunk past a region's count is simply claimed over.
-// The per-molecule arithmetic is the one of walk().
+// k_lens: integration of the lens-entry queue.  Persistent wavefronts keep their lanes busy by
+// lane refill: every iteration advances the lanes that take full steps by one step; when at least
+// `refill_min` lanes do not, or none does, a convergent section finishes the molecules that stopped
+// taking full steps -- those that reached the end of the lens take the final partial step
+// (configuration mode) or the exit hop with the stored a = l1 (reference mode) -- writes each
+// finished molecule's state back into its queue entry with its outcome (queue_put_done), and gives
+// the empty lanes the next queued molecules.  A lane's rem is its full steps left (> 0: it takes
+// steps; 0: it reached the end of the lens), REM_STOPPED once the bore stopped it, REM_EMPTY when it
+// holds no molecule, so one compare per iteration gives both the step's lane mask and the refill
+// count, and a step updates rem with one compare, one add and one select.  Finished molecules are walked
+// through the remaining elements and recorded by POST, so the refill section carries no element
+// walk and no recording.  Wavefronts claim the queue in 64-entry chunks, region by region, from
+// eight counters: counter g serves the regions of the PRE workgroups dispatched to XCD g
+// (workgroup b runs on XCD b mod 8), and a wavefront starts with its own XCD's counter and moves to
+// the next one when it runs dry, which spreads the claim atomics over eight addresses.  Chunk c of
+// group g is entries [64 (c % cpr), +64) of the group's region c / cpr (cpr = region / 64), of
+// which the first wcount are filled, so a claim needs no search and a chunk past a region's count
+// is simply claimed over.  The per-molecule arithmetic is the one of walk().
 // ---------------------------------------------------------------------------------------------
-enum : int { LN_IDLE = 0, LN_LENS = 1, LN_EXIT = 2, LN_FIN = 3, LN_DONE = 4 };
 
-// Five wavefronts per SIMD (at most 96 VGPRs) for the hexapole RK4 circular-aperture-only
-// instantiation, which fits them without spilling; the others need more registers than that.
-template <int FORCE, int INTEG, int KM>
+// Six wavefronts per SIMD (at most 80 VGPRs) for the hexapole RK4 instantiation, which fits them
+// without spilling (with one lens state, 256-thread workgroups and 24 KB of tables, LDS holds six
+// workgroups per CU); the others need more registers than that.
+constexpr int REM_EMPTY = -1, REM_STOPPED = -2;
+
+template <int FORCE, int INTEG>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
-    FORCE == CTRAJ_FORCE_WSPLINE_HEX && INTEG == CTRAJ_INTEG_RK4 && KM == KM_CIRC ? 5 : 1)))
-k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refill_min, StepK K) {
+    FORCE == CTRAJ_FORCE_WSPLINE_HEX && INTEG == CTRAJ_INTEG_RK4 ? 6 : 1)))
+k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long *work) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
-  __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
-  __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
-  __shared__ unsigned long long s_work[4];
-  const int nbins = B.n_states * B.n_labels;
-  for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0;
-  for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x) s_first[b] = ~0ull >> 1;
-  if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
+  __shared__ unsigned long long s_work[2];
+  if (threadIdx.x < 2) s_work[threadIdx.x] = 0;
   const Tables T = stage_tables(lds, KT);
   __syncthreads();
   const uint32_t cpr = (uint32_t)(P.region >> 6);
@@ -601,107 +663,72 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
   const int lane = lane_id();
   const ctraj_elem &L0 = B.elems[P.lens_e];
   const Force<FORCE> F = make_force<FORCE>(L0, B.g);
-  const double R2 = L0.p[1], z1 = L0.z1;
-  // config-mode step size (uniform); its half and sixth, and for a folding hexapole the velocity
-  // constants ks Hs/2, ks Hs, ks Hs/6, come formed by the host in K, so all stay in scalar registers
-  const double Hs = L0.p[2];
-  const int label_inside = L0.label_inside;
-  WalkCounters wc{0u, 0u};
-  uint32_t n_oot = 0, n_cap = 0;
-  unsigned long long steps_w = 0;
+  // the lens: z1, bore (r^2 for RK4, r for the reference), step (dt for RK4, dz for the reference),
+  // length; for RK4 the uniform step's half and sixth and, for a folding hexapole, the velocity
+  // constants ks Hs/2, ks Hs, ks Hs/6 come formed by the host in K, so all stay in scalar registers
+  const double R2 = L0.p[1], half_d = L0.p[0], z1 = L0.z1, Hs = L0.p[2], Ll = L0.p[3];
+  const int n_ref = L0.n_steps;
+  uint32_t steps_w = 0, faces_w = 0;  // wave-uniform counts
   int64_t next = 0, end = 0;
   bool exhausted = false;
 
+  // lane state: the molecule (z, t advance with the steps; vz is constant in the lens), its queue
+  // entry and key, rem (see above), its full-step count and state table
   MState m;
+  m.x = m.y = m.z = m.vx = m.vy = m.vz = m.t = 0.0;
   m.ax = 0.0; m.ay = -B.g; m.az = 0.0;
-  double r2 = 0.0, h = 0.0, hp = 0.0, dzs = 0.0;
-  int i = 0, nfull = 0, st = 0, ph = LN_IDLE, label = 0;
+  double r2 = 0.0, dzs = 0.0, h = 0.0;
+  int rem = REM_EMPTY, nfull = 0;
   uint32_t status = 0;
-  int64_t idx = 0;
+  int64_t slot = 0, key = 0;
   WsRef wsb = ws_of(T, 0);
 
   for (;;) {
-    const uint64_t lensm = __ballot(ph == LN_LENS);
-    if (lensm == 0 || 64 - __popcll(lensm) >= refill_min) {
+    bool act = rem > 0;
+    uint64_t am = __builtin_amdgcn_ballot_w64(act);
+    if (64 - __popcll(am) >= refill_min) {
       for (;;) {
-        // molecules that left the lens walk the remaining elements
-        if (ph == LN_EXIT) {
-          if constexpr (INTEG == CTRAJ_INTEG_REF) {
-            drift_dt(m, time_to(m, z1), B.g);  // exit hop with the stored a = l1 (electrostatic_lens.py:70-75)
-            ++wc.faces;
+        // finish the molecules that hold a lane but take no more full steps: stopped by the bore, or
+        // at the end of the lens, which takes one more step (configuration mode) or the exit hop
+        // (reference mode)
+        const bool fin = rem == 0 || rem == REM_STOPPED;
+        const bool dead = rem == REM_STOPPED;
+        const uint64_t ends = __builtin_amdgcn_ballot_w64(rem == 0);
+        if constexpr (INTEG == CTRAJ_INTEG_RK4) steps_w += (uint32_t)__popcll(ends);
+        else faces_w += (uint32_t)__popcll(ends);
+        if (fin) {
+          int outcome = LENS_STOPPED;
+          if constexpr (INTEG == CTRAJ_INTEG_RK4) {
+            if (!dead) {  // the end of the lens: one partial step of L/vz - n dt, so the lens ends at z1
+              const double hp = Ll / m.vz - (double)nfull * Hs;
+              if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX) {  // the host launches only folding lenses here
+                const double hh = 0.5 * hp, h6 = hp * kSixth;
+                rk4_step_hex_k(wsb, T.umax, F.cu, K.gk, hp, hh, h6, F.ks * hh, F.ks * hp, F.ks * h6, m.x, m.y, m.vx,
+                               m.vy, r2);
+              } else {
+                rk4_step<FORCE>(F, T, wsb, hp, m.x, m.y, m.vx, m.vy, r2, status);
+              }
+              m.z = z1;
+              m.t = m.t + hp;
+              r2 = __builtin_fma(m.x, m.x, m.y * m.y);
+              outcome = r2 > R2 ? LENS_STOPPED : LENS_LEFT;
+            }
           } else {
-            m.ax = 0.0;  // the exit row at z1 stores (0, -g, 0)
-            m.ay = -B.g;
-            m.az = 0.0;
-          }
-          label = B.label_detected;
-          walk_plain<KM>(opaque(B.elems), P.lens_e + 1, B.n_elems, B.g, m, label, wc);
-          ph = LN_FIN;
-        }
-        // record finished molecules
-        if (__ballot(ph == LN_FIN)) {
-          const ctraj_run &R = *opaque(Rg);
-          if (ph == LN_FIN) {
-            if constexpr (INTEG == CTRAJ_INTEG_RK4) {
-              // a molecule stopped by the bore stores the force at its last position; only the
-              // acceleration outputs see it, so it is evaluated only when they are requested
-              if ((R.out_a[0] || R.out_a[1]) && label == label_inside) {
-                uint32_t st2 = 0;
-                F(T, wsb, m.x, m.y, m.ax, m.ay, st2);
-                m.az = 0.0;
-              }
-            }
-            if (R.out.x) R.out.x[idx] = m.x;
-            if (R.out.y) R.out.y[idx] = m.y;
-            if (R.out.z) R.out.z[idx] = m.z;
-            if (R.out.vx) R.out.vx[idx] = m.vx;
-            if (R.out.vy) R.out.vy[idx] = m.vy;
-            if (R.out.vz) R.out.vz[idx] = m.vz;
-            if (R.out.t) R.out.t[idx] = m.t;
-            if (R.out_a[0]) R.out_a[0][idx] = m.ax;
-            if (R.out_a[1]) R.out_a[1][idx] = m.ay;
-            if (R.out_a[2]) R.out_a[2][idx] = m.az;
-            if (R.label) R.label[idx] = label;
-            if (R.status)
-              R.status[idx] = (R.flags & CTRAJ_RUN_STATUS_OR) ? (uint8_t)(R.status[idx] | status) : (uint8_t)status;
-            n_oot += (status & CTRAJ_ST_OOT) ? 1u : 0u;
-            n_cap += (status & CTRAJ_ST_CAPPED) ? 1u : 0u;
-            wave_hist(s_hist, R.first_index ? s_first : nullptr, st * B.n_labels + label, label, R.index_base + idx);
-          }
-          if (R.aoi_mask) {
-            const bool keep = ph == LN_FIN && ((R.aoi_mask >> label) & 1ull);
-            const uint64_t mask = __ballot(keep);
-            if (mask) {
-              unsigned long long base = 0;
-              if (lane == 0) base = atomicAdd((unsigned long long *)R.aoi_count, (unsigned long long)__popcll(mask));
-              base = __shfl(base, 0);
-              if (keep) {
-                const int64_t slot = (int64_t)base + lanes_below(mask);
-                if (slot < R.aoi_cap) {
-                  R.aoi_index[slot] = R.index_base + idx;
-                  if (R.aoi_label) R.aoi_label[slot] = label;
-                  if (R.aoi_out.x) R.aoi_out.x[slot] = m.x;
-                  if (R.aoi_out.y) R.aoi_out.y[slot] = m.y;
-                  if (R.aoi_out.z) R.aoi_out.z[slot] = m.z;
-                  if (R.aoi_out.vx) R.aoi_out.vx[slot] = m.vx;
-                  if (R.aoi_out.vy) R.aoi_out.vy[slot] = m.vy;
-                  if (R.aoi_out.vz) R.aoi_out.vz[slot] = m.vz;
-                  if (R.aoi_out.t) R.aoi_out.t[slot] = m.t;
-                }
-              }
+            if (!dead) {  // exit hop with the stored a = l1 of the last step (electrostatic_lens.py:74-77)
+              drift_dt(m, time_to(m, z1), B.g);
+              outcome = LENS_LEFT;
             }
           }
-          if (ph == LN_FIN) ph = LN_IDLE;
+          queue_put_done(P.q, slot, m.x, m.y, m.z, m.vx, m.vy, m.vz, m.t,
+                         key | ((int64_t)status << 40) | ((int64_t)outcome << 48));
+          rem = REM_EMPTY;
         }
-        const uint64_t need = __ballot(ph == LN_IDLE);
-        if (!need || exhausted) {
-          if (ph == LN_IDLE && exhausted) ph = LN_DONE;
-          break;
-        }
+        const uint64_t need = __builtin_amdgcn_ballot_w64(rem == REM_EMPTY);
+        if (!need || exhausted) break;
         if (next >= end) {
           unsigned long long c = 0;
           if (lane == 0) c = atomicAdd(P.claim + 32 * grp, 1ull);
-          c = __shfl(c, 0);
+          c = uniform_u64(c);  // every lane is active here, lane 0 first
           if (c >= n_chunks) {
             if (++tried == 8) exhausted = true;
             grp = (grp + 1) & 7;
@@ -716,98 +743,60 @@ k_lens(KBeam B, KTab KT, const ctraj_run *__restrict__ Rg, PipeArgs P, int refil
           continue;
         }
         const int64_t avail = end - next;
-        if (ph == LN_IDLE) {
+        if (rem == REM_EMPTY) {
           const int rank = lanes_below(need);
           if (rank < avail) {
-            queue_get(P.q, next + rank, m, idx, st);
-            m.ax = 0.0;
+            slot = next + rank;
+            queue_get_done(P.q, slot, m, key);
+            m.ax = 0.0;  // the lens entrance row stores (0, -g, 0)
             m.ay = -B.g;
             m.az = 0.0;
-            wsb = ws_of(T, st);
+            wsb = ws_of(T, (int)((uint64_t)key >> 56));
             status = 0;
-            i = 0;
-            const ctraj_elem &L = *opaque(B.elems + P.lens_e);
             if constexpr (INTEG == CTRAJ_INTEG_RK4) {
-              const double Ll = L.p[3];
               nfull = rk4_full_steps(Ll, m.vz, Hs, B.cap, status);
               dzs = m.vz * Hs;
-              hp = Ll / m.vz - (double)nfull * Hs;
               r2 = __builtin_fma(m.x, m.x, m.y * m.y);
-              ph = LN_LENS;
             } else {
-              nfull = L.n_steps;
-              h = L.p[2] / m.vz;
-              ph = nfull > 0 ? LN_LENS : LN_EXIT;
+              nfull = n_ref;
+              h = Hs / m.vz;  // dt = dz / v_z (electrostatic_lens.py:88)
             }
+            rem = nfull;
           }
         }
         next += avail < (int64_t)__popcll(need) ? avail : (int64_t)__popcll(need);
       }
-      if (!__ballot(ph == LN_LENS)) {
-        if (!__ballot(ph == LN_EXIT || ph == LN_FIN)) break;
-        continue;
-      }
+      act = rem > 0;
+      am = __builtin_amdgcn_ballot_w64(act);
+      if (!am) break;  // the queue is exhausted and every molecule has been finished
     }
 
-    steps_w += (unsigned long long)__popcll(__ballot(ph == LN_LENS));
-    if (ph == LN_LENS) {
+    steps_w += (uint32_t)__popcll(am);
+    if (act) {
       if constexpr (INTEG == CTRAJ_INTEG_RK4) {
-        const bool last = (i == nfull);
-        if (!last) {  // full steps: the uniform step size
-          if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX)  // the host launches only folding lenses here
-            rk4_step_hex_k(wsb, T.umax, F.cu, K.gk, Hs, K.hh, K.h6, K.khh, K.kh, K.kh6, m.x, m.y, m.vx, m.vy, r2);
-          else
-            rk4_step_hh<FORCE>(F, T, wsb, Hs, K.hh, K.h6, m.x, m.y, m.vx, m.vy, r2, status);
-          m.z = m.z + dzs;
-          m.t = m.t + Hs;
-        } else {  // the final partial step, once per molecule that reaches the lens end
-          if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX) {
-            const double hh = 0.5 * hp, h6 = hp * kSixth;
-            rk4_step_hex_k(wsb, T.umax, F.cu, K.gk, hp, hh, h6, F.ks * hh, F.ks * hp, F.ks * h6, m.x, m.y, m.vx, m.vy,
-                           r2);
-          } else {
-            rk4_step<FORCE>(F, T, wsb, hp, m.x, m.y, m.vx, m.vy, r2, status);
-          }
-          m.z = z1;
-          m.t = m.t + hp;
-        }
+        if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX)  // the host launches only folding lenses here
+          rk4_step_hex_k(wsb, T.umax, F.cu, K.gk, Hs, K.hh, K.h6, K.khh, K.kh, K.kh6, m.x, m.y, m.vx, m.vy, r2);
+        else
+          rk4_step_hh<FORCE>(F, T, wsb, Hs, K.hh, K.h6, m.x, m.y, m.vx, m.vy, r2, status);
+        m.z = m.z + dzs;
+        m.t = m.t + Hs;
         r2 = __builtin_fma(m.x, m.x, m.y * m.y);
-        ++i;
-        const bool dead = r2 > R2;
-        label = dead ? label_inside : label;
-        ph = dead ? (int)LN_FIN : (last ? (int)LN_EXIT : (int)LN_LENS);
+        rem = r2 > R2 ? REM_STOPPED : rem - 1;
       } else {
         ref_step<FORCE>(F, T, wsb, h, m, status);  // the row stores a = l1
-        ++i;
-        if (rho_of(m.x, m.y) > L0.p[0]) {
-          label = label_inside;
-          ph = LN_FIN;
-        } else if (i == nfull) {
-          ph = LN_EXIT;
-        }
+        rem = rho_of(m.x, m.y) > half_d ? REM_STOPPED : rem - 1;
       }
     }
   }
 
-  const ctraj_run &R = *Rg;
-  if (R.work) {
-    if (wc.faces) atomicAdd(&s_work[0], (unsigned long long)wc.faces);
-    if (lane == 0 && steps_w) atomicAdd(&s_work[1], steps_w);
-    if (n_oot) atomicAdd(&s_work[2], (unsigned long long)n_oot);
-    if (n_cap) atomicAdd(&s_work[3], (unsigned long long)n_cap);
+  if (work) {
+    if (lane == 0) {
+      if (steps_w) atomicAdd(&s_work[1], (unsigned long long)steps_w);
+      if (faces_w) atomicAdd(&s_work[0], (unsigned long long)faces_w);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 && s_work[threadIdx.x]) atomicAdd(&work[threadIdx.x], s_work[threadIdx.x]);
   }
-  __syncthreads();
-  if (R.tally)
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
-      if (s_hist[b]) atomicAdd((unsigned long long *)&R.tally[b], (unsigned long long)s_hist[b]);
-  if (R.first_index)
-    for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x)
-      if (s_first[b] != (~0ull >> 1)) atomicMin((unsigned long long *)&R.first_index[b], s_first[b]);
-  if (R.work && threadIdx.x < 4 && s_work[threadIdx.x])
-    atomicAdd((unsigned long long *)&R.work[threadIdx.x], s_work[threadIdx.x]);
-  // the faces of this phase (the elements after the lens) also go to their own counter
-  if (R.work && threadIdx.x == 4 && s_work[0])
-    atomicAdd((unsigned long long *)&R.work[CTRAJ_WORK_FACES_POST], s_work[0]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1137,7 +1126,7 @@ int ensure_buf(ctraj_ctx *ctx, ctraj_devbuf &b, size_t bytes) {
   X(CTRAJ_FORCE_WSPLINE_LIN, CTRAJ_INTEG_RK4)
 
 using PropFn = void (*)(KBeam, KTab, ctraj_run, int, PipeArgs);
-using LensFn = void (*)(KBeam, KTab, const ctraj_run *, PipeArgs, int, StepK);
+using LensFn = void (*)(KBeam, KTab, PipeArgs, int, StepK, unsigned long long *);
 using LaneFn = void (*)(KBeam, KTab, const ctraj_run *, LaneArgs);
 using RecFn = void (*)(KBeam, KTab, ctraj_record);
 
@@ -1178,12 +1167,26 @@ static PropFn prop_kernel(int f, int i, bool has_lens, int km, bool out, bool mi
   return nullptr;
 }
 
-static LensFn lens_kernel(int f, int i, int km) {
+static LensFn lens_kernel(int f, int i) {
 #define X(F, I) \
-  if (f == F && i == I) return km == KM_CIRC ? k_lens<F, I, KM_CIRC> : k_lens<F, I, KM_ALL>;
+  if (f == F && i == I) return k_lens<F, I>;
   CT_FI_LIST(X)
 #undef X
   return nullptr;
+}
+
+// POST: the walk after the lens and the recording of every lens molecule.  Only a run that asks for
+// stored accelerations needs the lens force (configuration mode: the force where the bore stopped a
+// molecule); the other instantiations do not depend on the lens model.
+static PropFn post_kernel(int f, int km, bool out, bool accel) {
+  constexpr int H = CTRAJ_FORCE_WSPLINE_HEX, I = CTRAJ_INTEG_RK4;
+  if (accel) {
+    if (f == CTRAJ_FORCE_RTABLE) return k_propagate<CTRAJ_FORCE_RTABLE, I, PHASE_POST, KM_ALL, true>;
+    if (f == CTRAJ_FORCE_WSPLINE_LIN) return k_propagate<CTRAJ_FORCE_WSPLINE_LIN, I, PHASE_POST, KM_ALL, true>;
+    return k_propagate<H, I, PHASE_POST, KM_ALL, true>;
+  }
+  if (km == KM_CIRC) return out ? k_propagate<H, I, PHASE_POST, KM_CIRC, true> : k_propagate<H, I, PHASE_POST, KM_CIRC, false>;
+  return out ? k_propagate<H, I, PHASE_POST, KM_ALL, true> : k_propagate<H, I, PHASE_POST, KM_ALL, false>;
 }
 
 static LaneFn lane_kernel(int f, int i, bool fold) {
@@ -1209,6 +1212,239 @@ static RecFn rec_kernel(int f, int i, int mode, int km) {
   CT_FI_LIST(X)
 #undef X
   return nullptr;
+}
+
+// Scratch layout of ctraj_propagate_noalloc (byte offsets into the device's scratch array).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr size_t SCR_CLAIM = 0;            // [8][32] claim counters
+constexpr size_t SCR_COUNTER = 2048;       // lane walker's molecule counter
+constexpr size_t SCR_RUN = 4096;           // device copy of the run (lane walker)
+constexpr size_t SCR_WCOUNT = 8192;        // per-PRE-wavefront queue counts
+constexpr size_t SCR_WCOUNT_MAX = 16384;   // PRE wavefronts at most
+constexpr size_t SCR_QUEUE = SCR_WCOUNT + 4 * SCR_WCOUNT_MAX;
+static_assert(sizeof(ctraj_run) <= SCR_WCOUNT - SCR_RUN, "run copy must fit its scratch slot");
+
+// Molecules per slice of a run split into kernels: the lens-entry queue holds one slice.
+constexpr int64_t SLICE_MAX = (int64_t)1 << 26;
+
+// The run restricted to molecules [s0, s0 + len): per-molecule arrays offset by s0, global indices
+// (first_index, kept molecules) by index_base + s0; the accumulated outputs are shared.
+static ctraj_run slice_of(const ctraj_run &R, int64_t s0, int64_t len) {
+  ctraj_run S = R;
+  auto off = [s0](auto *p) { return p ? p + s0 : p; };
+  S.n = len;
+  S.index_base = R.index_base + s0;
+  S.in.x = off(R.in.x); S.in.y = off(R.in.y); S.in.z = off(R.in.z); S.in.vx = off(R.in.vx);
+  S.in.vy = off(R.in.vy); S.in.vz = off(R.in.vz); S.in.t = off(R.in.t);
+  for (int k = 0; k < 3; ++k) {
+    S.a0[k] = off(R.a0[k]);
+    S.out_a[k] = off(R.out_a[k]);
+  }
+  S.state = off(R.state);
+  S.out.x = off(R.out.x); S.out.y = off(R.out.y); S.out.z = off(R.out.z); S.out.vx = off(R.out.vx);
+  S.out.vy = off(R.out.vy); S.out.vz = off(R.out.vz); S.out.t = off(R.out.t);
+  S.label = off(R.label);
+  S.status = off(R.status);
+  return S;
+}
+
+__global__ void k_put_run(ctraj_run R, ctraj_run *dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = R;
+}
+
+// ctraj_propagate and ctraj_propagate_noalloc.  scratch == nullptr: the context's buffers (grown as
+// needed); otherwise every device buffer the launches need lies in `scratch` and nothing is allocated.
+static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *scratch) {
+  if (!c || !R) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (!c->have_beamline) return fail(c, CTRAJ_ESTATE, "no beamline set");
+  if (R->n < 0) return fail(c, CTRAJ_EINVAL, "n < 0");
+  if (R->n == 0) return CTRAJ_OK;
+  if (!R->in.x || !R->in.y || !R->in.vx || !R->in.vy || !R->in.vz) return fail(c, CTRAJ_EINVAL, "missing inputs");
+  if (R->aoi_mask && (!R->aoi_count || !R->aoi_index || R->aoi_cap < 0))
+    return fail(c, CTRAJ_EINVAL, "aoi compaction needs aoi_count, aoi_index and aoi_cap");
+  if (c->phys.n_labels < 64 && (R->aoi_mask >> c->phys.n_labels))
+    return fail(c, CTRAJ_EINVAL, "aoi_mask names labels beyond n_labels");
+  if (scratch && (R->flags & CTRAJ_RUN_PHASE_TIMES))
+    return fail(c, CTRAJ_EINVAL, "CTRAJ_RUN_PHASE_TIMES creates events: not available without allocation");
+  int rc0 = check_tables(c, c->h_elems.data(), c->n_elems, c->phys.n_states);
+  if (rc0) return rc0;
+  KTab kt = ctx_ktab(c);
+  if (c->force == CTRAJ_FORCE_RTABLE) kt.ws_states = 0;
+  const size_t lds = ktab_lds_bytes(kt);
+  KBeam B = make_kbeam(c->d_elems, c->n_elems, c->phys);
+  int n_lens = 0, lens_e = -1;
+  for (int e = 0; e < c->n_elems; ++e)
+    if (c->h_elems[e].kind == CTRAJ_LENS) {
+      ++n_lens;
+      lens_e = e;
+    }
+  // A hexapole lens whose force scale does not fold into the step constants (hex_folds) takes the
+  // single-kernel walker, which tests that per molecule; so does a reference-mode lens when stored
+  // accelerations are asked for (the row of a molecule stopped by the bore stores the l1 of its last
+  // step, which the lens kernel does not pass on).
+  const bool accel = R->out_a[0] || R->out_a[1] || R->out_a[2];
+  const bool folds = c->force != CTRAJ_FORCE_WSPLINE_HEX || c->integ != CTRAJ_INTEG_RK4 ||
+                     (lens_e >= 0 && hex_folds(c->h_elems[lens_e].p[4]));
+  const bool pipeline = n_lens == 1 && !(R->flags & (CTRAJ_RUN_SIMPLE | CTRAJ_RUN_LANES)) && folds &&
+                        !(accel && c->integ == CTRAJ_INTEG_REF);
+  const bool lanes = n_lens == 1 && !(R->flags & CTRAJ_RUN_SIMPLE) && !pipeline;
+  // The lane-refill kernels keep their tables once per workgroup: with several lens-state tables
+  // (more than ~40 KB) one large workgroup per CU holds as many wavefronts as its registers allow,
+  // where 256-thread workgroups would be limited by LDS to one per CU.
+  // (one 1024-thread lens workgroup per CU holds 16 wavefronts beside several state tables)
+  const int block = R->block ? R->block : (pipeline && lds > 40960 ? 1024 : (lanes && lds > 40960 ? 768 : 256));
+  const int max_block = pipeline ? 1024 : (lanes ? 768 : 256);
+  if (block < 64 || block > max_block || block % 64)
+    return fail(c, CTRAJ_EINVAL, "block must be a multiple of 64 up to " + std::to_string(max_block));
+  HIPCHK(c, hipSetDevice(c->device));
+  // CTRAJ_RUN_PHASE_TIMES: four events of the context's ring bracket the kernels of this launch
+  hipEvent_t *pev = nullptr;
+  if (R->flags & CTRAJ_RUN_PHASE_TIMES) {
+    const int slot = (int)(c->pt_next % 64);
+    for (int k = 0; k < 4; ++k)
+      if (!c->pt_ev[slot][k]) HIPCHK(c, hipEventCreate(&c->pt_ev[slot][k]));
+    pev = c->pt_ev[slot];
+  }
+  if (pipeline) {
+    LensFn kl = lens_kernel(c->force, c->integ);
+    if (!kl) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
+    const bool out = wants_outputs(R);
+    // PRE: walk to the lens and through its entrance; each wavefront queues the molecules that enter
+    // the lens into its own region (no atomics)
+    const int blk = 256;
+    const size_t lds_aoi = R->aoi_mask ? (size_t)(blk / 64) * AOI_WAVE * AOI_ENTRY_BYTES : 0;
+    PropFn kpre = walker_kernel<PHASE_PRE>(kinds_mask(c->h_elems, 0, lens_e), out, min_inputs(R));
+    // POST: the elements after the lens, and the recording of the lens molecules; it needs the lens
+    // tables only for stored accelerations
+    PropFn kpost = post_kernel(c->force, kinds_mask(c->h_elems, lens_e + 1, c->n_elems), out, accel);
+    KTab kt_post = kt;
+    if (!accel) kt_post.ws_states = kt_post.rt_total = 0;
+    const size_t lds_post_tab = ktab_lds_bytes(kt_post);
+    const size_t lds_post = lds_post_tab + lds_aoi;
+    int rc = lds_limit(c, (const void *)kpre, lds_aoi);
+    if (rc) return rc;
+    if ((rc = lds_limit(c, (const void *)kpost, lds_post)) != CTRAJ_OK) return rc;
+    if ((rc = lds_limit(c, (const void *)kl, lds)) != CTRAJ_OK) return rc;
+    if ((rc = block_ok(c, (const void *)kl, block)) != CTRAJ_OK) return rc;
+    // a slice's queue holds threads * ceil(len / threads) entries (region = a multiple of 64 per
+    // PRE wavefront), so the PRE grid and the slice length follow from the queue's capacity
+    const int64_t n_first = std::min<int64_t>(R->n, SLICE_MAX);
+    int gpre = (grid_for(c, (const void *)kpre, blk, lds_aoi, n_first, R->grid) + 7) & ~7;  // whole XCD groups
+    int64_t slice = n_first;
+    PipeArgs P{};
+    if (scratch) {
+      const size_t q_entries = (NOALLOC_SCRATCH_BYTES - SCR_QUEUE) / 64;
+      const int g_cap = (int)std::min<size_t>(q_entries / blk, SCR_WCOUNT_MAX / (blk / 64)) & ~7;
+      gpre = std::min(gpre, g_cap);
+      const int64_t threads = (int64_t)gpre * blk;
+      slice = std::min<int64_t>(R->n, (int64_t)(q_entries / threads) * threads);
+      P.q = (longlong2 *)(scratch + SCR_QUEUE);
+      P.claim = (unsigned long long *)(scratch + SCR_CLAIM);
+      P.wcount = (uint32_t *)(scratch + SCR_WCOUNT);
+    } else {
+      const int64_t threads = (int64_t)gpre * blk;
+      const size_t cap = (size_t)threads * (size_t)((slice + threads - 1) / threads);
+      if ((rc = ensure_buf(c, c->pipe, cap * 64 + 256)) != CTRAJ_OK) return rc;
+      if ((rc = ensure_buf(c, c->pipe_cnt, 8 * 256 + (size_t)gpre * (blk / 64) * sizeof(uint32_t))) != CTRAJ_OK)
+        return rc;
+      P.q = (longlong2 *)c->pipe.p;
+      P.claim = (unsigned long long *)c->pipe_cnt.p;
+      P.wcount = (uint32_t *)((char *)c->pipe_cnt.p + 8 * 256);
+    }
+    const int64_t threads = (int64_t)gpre * blk;
+    P.n_pre_waves = gpre * (blk / 64);
+    P.lens_e = lens_e;
+    const int refill = R->refill > 0 ? std::min(R->refill, 64) : 12;
+    const int grid = grid_for(c, (const void *)kl, block, lds, (int64_t)1 << 40, 0);
+    const StepK K = step_k(c->h_elems[lens_e].p[2], c->h_elems[lens_e].p[4], c->phys.g);
+    const bool one_slice = slice >= R->n;
+    for (int64_t s0 = 0; s0 < R->n; s0 += slice) {
+      const int64_t len = std::min(slice, R->n - s0);
+      const ctraj_run S = slice_of(*R, s0, len);
+      P.region = (len + threads - 1) / threads * 64;
+      if ((rc = ctraj_fill_i64(c, (int64_t *)P.claim, 0, 8 * 32)) != CTRAJ_OK) return rc;
+      if (pev && s0 == 0) HIPCHK(c, hipEventRecord(pev[0], c->stream));
+      hipLaunchKernelGGL(kpre, dim3(gpre), dim3(blk), lds_aoi, c->stream, B, kt, S, 0, P);
+      HIPCHK(c, hipGetLastError());
+      if (pev && one_slice) HIPCHK(c, hipEventRecord(pev[1], c->stream));
+      hipLaunchKernelGGL(kl, dim3(grid), dim3(block), lds, c->stream, B, kt, P, refill, K,
+                         (unsigned long long *)R->work);
+      HIPCHK(c, hipGetLastError());
+      if (pev && one_slice) HIPCHK(c, hipEventRecord(pev[2], c->stream));
+      const int gpost = grid_for(c, (const void *)kpost, blk, lds_post, (int64_t)P.n_pre_waves * P.region, 0);
+      hipLaunchKernelGGL(kpost, dim3(gpost), dim3(blk), lds_post, c->stream, B, kt_post, S,
+                         (int)(lds_post_tab / sizeof(double)), P);
+      HIPCHK(c, hipGetLastError());
+    }
+    if (pev) {
+      HIPCHK(c, hipEventRecord(one_slice ? pev[3] : pev[1], c->stream));
+      c->pt_split[c->pt_next % 64] = one_slice;
+    }
+  } else if (lanes) {
+    const double ks = c->h_elems[lens_e].p[4];
+    LaneFn k = lane_kernel(c->force, c->integ, hex_folds(ks));
+    if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
+    int rc = lds_limit(c, (const void *)k, lds);
+    if (rc) return rc;
+    rc = block_ok(c, (const void *)k, block);
+    if (rc) return rc;
+    void *counter = nullptr;
+    ctraj_run *d_run = nullptr;
+    if (scratch) {
+      counter = scratch + SCR_COUNTER;
+      d_run = (ctraj_run *)(scratch + SCR_RUN);
+      hipLaunchKernelGGL(k_put_run, dim3(1), dim3(64), 0, c->stream, *R, d_run);
+      HIPCHK(c, hipGetLastError());
+    } else {
+      if (!c->d_counter) HIPCHK(c, hipMalloc((void **)&c->d_counter, 256));
+      counter = c->d_counter;
+      if (!c->d_run) HIPCHK(c, hipMalloc(&c->d_run, sizeof(ctraj_run)));
+      if (!c->have_last_run || std::memcmp(&c->last_run, R, sizeof(ctraj_run)) != 0) {
+        HIPCHK(c, hipMemcpyAsync(c->d_run, R, sizeof(ctraj_run), hipMemcpyHostToDevice, c->stream));
+        c->last_run = *R;
+        c->have_last_run = true;
+      }
+      d_run = (ctraj_run *)c->d_run;
+    }
+    rc = ctraj_fill_i64(c, (int64_t *)counter, 0, 1);
+    if (rc) return rc;
+    LaneArgs LA;
+    LA.counter = (unsigned long long *)counter;
+    LA.lens_e = lens_e;
+    LA.refill_min = R->refill > 0 ? std::min(R->refill, 64) : 24;
+    LA.claim = 128;
+    LA.gk = hex_folds(ks) ? -c->phys.g / ks : 0.0;
+    const int grid = grid_for(c, (const void *)k, block, lds, R->n, R->grid);
+    if (pev) HIPCHK(c, hipEventRecord(pev[0], c->stream));
+    hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, (const ctraj_run *)d_run, LA);
+    if (pev) {
+      HIPCHK(c, hipEventRecord(pev[1], c->stream));
+      c->pt_split[c->pt_next % 64] = false;
+    }
+  } else {
+    PropFn k = prop_kernel(c->force, c->integ, n_lens > 0, kinds_mask(c->h_elems, 0, c->n_elems), wants_outputs(R),
+                           min_inputs(R));
+    if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
+    if (n_lens == 0) kt.ws_states = kt.rt_total = 0;
+    size_t lds_tab = n_lens > 0 ? lds : 0;
+    // kept-molecule staging area after the tables: AOI_WAVE entries per wavefront
+    const int aoi_off = (int)(lds_tab / sizeof(double));
+    const size_t lds_all = lds_tab + (R->aoi_mask ? (size_t)(block / 64) * AOI_WAVE * AOI_ENTRY_BYTES : 0);
+    int rc = lds_limit(c, (const void *)k, lds_all);
+    if (rc) return rc;
+    rc = block_ok(c, (const void *)k, block);
+    if (rc) return rc;
+    const int grid = grid_for(c, (const void *)k, block, lds_all, R->n, R->grid);
+    if (pev) HIPCHK(c, hipEventRecord(pev[0], c->stream));
+    hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds_all, c->stream, B, kt, *R, aoi_off, PipeArgs{});
+    if (pev) {
+      HIPCHK(c, hipEventRecord(pev[1], c->stream));
+      c->pt_split[c->pt_next % 64] = false;
+    }
+  }
+  HIPCHK(c, hipGetLastError());
+  if (pev && ++c->pt_next - c->pt_read > 64) c->pt_read = c->pt_next - 64;
+  return CTRAJ_OK;
 }
 
 extern "C" {
@@ -1442,158 +1678,7 @@ int32_t ctraj_set_wspline(ctraj_ctx *c, int32_t n_states, int32_t n_seg, const d
   return CTRAJ_OK;
 }
 
-int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) {
-  if (!c || !R) return fail(c, CTRAJ_EINVAL, "null argument");
-  if (!c->have_beamline) return fail(c, CTRAJ_ESTATE, "no beamline set");
-  if (R->n < 0) return fail(c, CTRAJ_EINVAL, "n < 0");
-  if (R->n == 0) return CTRAJ_OK;
-  if (!R->in.x || !R->in.y || !R->in.vx || !R->in.vy || !R->in.vz) return fail(c, CTRAJ_EINVAL, "missing inputs");
-  if (R->aoi_mask && (!R->aoi_count || !R->aoi_index || R->aoi_cap < 0))
-    return fail(c, CTRAJ_EINVAL, "aoi compaction needs aoi_count, aoi_index and aoi_cap");
-  if (c->phys.n_labels < 64 && (R->aoi_mask >> c->phys.n_labels))
-    return fail(c, CTRAJ_EINVAL, "aoi_mask names labels beyond n_labels");
-  int rc0 = check_tables(c, c->h_elems.data(), c->n_elems, c->phys.n_states);
-  if (rc0) return rc0;
-  KTab kt = ctx_ktab(c);
-  if (c->force == CTRAJ_FORCE_RTABLE) kt.ws_states = 0;
-  const size_t lds = ktab_lds_bytes(kt);
-  KBeam B = make_kbeam(c->d_elems, c->n_elems, c->phys);
-  int n_lens = 0, lens_e = -1;
-  for (int e = 0; e < c->n_elems; ++e)
-    if (c->h_elems[e].kind == CTRAJ_LENS) {
-      ++n_lens;
-      lens_e = e;
-    }
-  // the lens-entry queue costs 64 B per molecule: single runs above 2^27 molecules (8.6 GB of
-  // queue) use the single-kernel walker, which needs no queue
-  // (and a hexapole lens whose force scale does not fold into the step constants, hex_folds, takes
-  // the single-kernel walker, which tests that per molecule)
-  const bool folds = c->force != CTRAJ_FORCE_WSPLINE_HEX || c->integ != CTRAJ_INTEG_RK4 ||
-                     (lens_e >= 0 && hex_folds(c->h_elems[lens_e].p[4]));
-  const bool pipeline = n_lens == 1 && !(R->flags & (CTRAJ_RUN_SIMPLE | CTRAJ_RUN_LANES)) && R->n <= ((int64_t)1 << 27) &&
-                        folds;
-  const bool lanes = n_lens == 1 && !(R->flags & CTRAJ_RUN_SIMPLE) && !pipeline;
-  // The lane-refill kernels keep their tables once per workgroup: with several lens-state tables
-  // (more than ~40 KB) one large workgroup per CU holds as many wavefronts as its registers allow,
-  // where 256-thread workgroups would be limited by LDS to one per CU.
-  // (one 1024-thread two-phase lens workgroup per CU holds 16 wavefronts beside several state tables)
-  const int block = R->block ? R->block : (pipeline && lds > 40960 ? 1024 : (lanes && lds > 40960 ? 768 : 256));
-  const int max_block = pipeline ? 1024 : (lanes ? 768 : 256);
-  if (block < 64 || block > max_block || block % 64)
-    return fail(c, CTRAJ_EINVAL, "block must be a multiple of 64 up to " + std::to_string(max_block));
-  HIPCHK(c, hipSetDevice(c->device));
-  // CTRAJ_RUN_PHASE_TIMES: three events of the context's ring bracket the kernels of this launch
-  hipEvent_t *pev = nullptr;
-  if (R->flags & CTRAJ_RUN_PHASE_TIMES) {
-    const int slot = (int)(c->pt_next % 64);
-    for (int k = 0; k < 4; ++k)
-      if (!c->pt_ev[slot][k]) HIPCHK(c, hipEventCreate(&c->pt_ev[slot][k]));
-    pev = c->pt_ev[slot];
-  }
-  if (pipeline) {
-    LensFn kl = lens_kernel(c->force, c->integ, kinds_mask(c->h_elems, lens_e + 1, c->n_elems));
-    if (!kl) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
-    // phase 1: walk to the lens and through its entrance; each wavefront queues the molecules that
-    // enter the lens into its own region (no atomics)
-    const int blk = 256;
-    const size_t lds_walk = R->aoi_mask ? (size_t)(blk / 64) * AOI_WAVE * AOI_ENTRY_BYTES : 0;
-    PropFn kpre = walker_kernel<PHASE_PRE>(kinds_mask(c->h_elems, 0, lens_e), wants_outputs(R), min_inputs(R));
-    int rc = lds_limit(c, (const void *)kpre, lds_walk);
-    if (rc) return rc;
-    const int gpre = (grid_for(c, (const void *)kpre, blk, lds_walk, R->n, R->grid) + 7) & ~7;  // whole XCD groups
-    const int nw = gpre * (blk / 64);
-    const int64_t threads = (int64_t)gpre * blk;
-    const int64_t region = (R->n + threads - 1) / threads * 64;
-    const size_t cap = (size_t)nw * (size_t)region;
-    const size_t bytes = cap * 64 + 256;
-    if ((rc = ensure_buf(c, c->pipe, bytes)) != CTRAJ_OK) return rc;
-    if ((rc = ensure_buf(c, c->pipe_cnt, 8 * 256 + (size_t)nw * sizeof(uint32_t))) != CTRAJ_OK) return rc;
-    PipeArgs P{};
-    char *b = (char *)c->pipe.p;
-    P.q = (longlong2 *)b;
-    char *cb = (char *)c->pipe_cnt.p;
-    P.claim = (unsigned long long *)cb;
-    P.wcount = (uint32_t *)(cb + 8 * 256);
-    P.region = region;
-    P.n_pre_waves = nw;
-    P.lens_e = lens_e;
-    if ((rc = ctraj_fill_i64(c, (int64_t *)P.claim, 0, 8 * 32)) != CTRAJ_OK) return rc;
-    const size_t lds_lens = lds;
-    if ((rc = lds_limit(c, (const void *)kl, lds_lens)) != CTRAJ_OK) return rc;
-    if ((rc = block_ok(c, (const void *)kl, block)) != CTRAJ_OK) return rc;
-    if (!c->d_run) HIPCHK(c, hipMalloc(&c->d_run, sizeof(ctraj_run)));
-    if (!c->have_last_run || std::memcmp(&c->last_run, R, sizeof(ctraj_run)) != 0) {
-      HIPCHK(c, hipMemcpyAsync(c->d_run, R, sizeof(ctraj_run), hipMemcpyHostToDevice, c->stream));
-      c->last_run = *R;
-      c->have_last_run = true;
-    }
-    if (pev) HIPCHK(c, hipEventRecord(pev[0], c->stream));
-    hipLaunchKernelGGL(kpre, dim3(gpre), dim3(blk), lds_walk, c->stream, B, kt, *R, 0, P);
-    HIPCHK(c, hipGetLastError());
-    if (pev) HIPCHK(c, hipEventRecord(pev[1], c->stream));
-    // phase 2: the lens and the elements after it
-    const int refill = R->refill > 0 ? std::min(R->refill, 64) : 12;
-    const int grid = grid_for(c, (const void *)kl, block, lds_lens, (int64_t)1 << 40, 0);
-    const StepK K = step_k(c->h_elems[lens_e].p[2], c->h_elems[lens_e].p[4], c->phys.g);
-    hipLaunchKernelGGL(kl, dim3(grid), dim3(block), lds_lens, c->stream, B, kt, (const ctraj_run *)c->d_run, P, refill,
-                       K);
-  } else if (lanes) {
-    const double ks = c->h_elems[lens_e].p[4];
-    LaneFn k = lane_kernel(c->force, c->integ, hex_folds(ks));
-    if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
-    int rc = lds_limit(c, (const void *)k, lds);
-    if (rc) return rc;
-    rc = block_ok(c, (const void *)k, block);
-    if (rc) return rc;
-    if (!c->d_counter) HIPCHK(c, hipMalloc((void **)&c->d_counter, 256));
-    rc = ctraj_fill_i64(c, (int64_t *)c->d_counter, 0, 1);
-    if (rc) return rc;
-    if (!c->d_run) HIPCHK(c, hipMalloc(&c->d_run, sizeof(ctraj_run)));
-    if (!c->have_last_run || std::memcmp(&c->last_run, R, sizeof(ctraj_run)) != 0) {
-      HIPCHK(c, hipMemcpyAsync(c->d_run, R, sizeof(ctraj_run), hipMemcpyHostToDevice, c->stream));
-      c->last_run = *R;
-      c->have_last_run = true;
-    }
-    LaneArgs LA;
-    LA.counter = (unsigned long long *)c->d_counter;
-    LA.lens_e = lens_e;
-    LA.refill_min = R->refill > 0 ? std::min(R->refill, 64) : 24;
-    LA.claim = 128;
-    LA.gk = hex_folds(ks) ? -c->phys.g / ks : 0.0;
-    const int grid = grid_for(c, (const void *)k, block, lds, R->n, R->grid);
-    if (pev) {
-      HIPCHK(c, hipEventRecord(pev[0], c->stream));
-      HIPCHK(c, hipEventRecord(pev[1], c->stream));
-    }
-    hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, c->stream, B, kt, (const ctraj_run *)c->d_run, LA);
-  } else {
-    PropFn k = prop_kernel(c->force, c->integ, n_lens > 0, kinds_mask(c->h_elems, 0, c->n_elems), wants_outputs(R),
-                             min_inputs(R));
-    if (!k) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
-    if (n_lens == 0) kt.ws_states = kt.rt_total = 0;
-    size_t lds_tab = n_lens > 0 ? lds : 0;
-    // kept-molecule staging area after the tables: AOI_WAVE entries per wavefront
-    const int aoi_off = (int)(lds_tab / sizeof(double));
-    const size_t lds_all = lds_tab + (R->aoi_mask ? (size_t)(block / 64) * AOI_WAVE * AOI_ENTRY_BYTES : 0);
-    int rc = lds_limit(c, (const void *)k, lds_all);
-    if (rc) return rc;
-    rc = block_ok(c, (const void *)k, block);
-    if (rc) return rc;
-    const int grid = grid_for(c, (const void *)k, block, lds_all, R->n, R->grid);
-    if (pev) {
-      HIPCHK(c, hipEventRecord(pev[0], c->stream));
-      HIPCHK(c, hipEventRecord(pev[1], c->stream));
-    }
-    hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds_all, c->stream, B, kt, *R, aoi_off, PipeArgs{});
-  }
-  HIPCHK(c, hipGetLastError());
-  if (pev) {
-    HIPCHK(c, hipEventRecord(pev[2], c->stream));
-    HIPCHK(c, hipEventRecord(pev[3], c->stream));
-    if (++c->pt_next - c->pt_read > 64) c->pt_read = c->pt_next - 64;
-  }
-  return CTRAJ_OK;
-}
+int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) { return propagate_impl(c, R, nullptr); }
 
 int32_t ctraj_phase_times(ctraj_ctx *c, int32_t max, float *walk_ms, float *lens_ms, float *post_ms, int32_t *count) {
   if (!c || !count || max < 0 || (max > 0 && (!walk_ms || !lens_ms || !post_ms)))
@@ -1603,10 +1688,16 @@ int32_t ctraj_phase_times(ctraj_ctx *c, int32_t max, float *walk_ms, float *lens
   int32_t k = 0;
   for (int64_t j = first; j < c->pt_next; ++j, ++k) {
     hipEvent_t *e = c->pt_ev[j % 64];
-    HIPCHK(c, hipEventSynchronize(e[3]));
-    HIPCHK(c, hipEventElapsedTime(&walk_ms[k], e[0], e[1]));
-    HIPCHK(c, hipEventElapsedTime(&lens_ms[k], e[1], e[2]));
-    HIPCHK(c, hipEventElapsedTime(&post_ms[k], e[2], e[3]));
+    if (c->pt_split[j % 64]) {
+      HIPCHK(c, hipEventSynchronize(e[3]));
+      HIPCHK(c, hipEventElapsedTime(&walk_ms[k], e[0], e[1]));
+      HIPCHK(c, hipEventElapsedTime(&lens_ms[k], e[1], e[2]));
+      HIPCHK(c, hipEventElapsedTime(&post_ms[k], e[2], e[3]));
+    } else {
+      HIPCHK(c, hipEventSynchronize(e[1]));
+      HIPCHK(c, hipEventElapsedTime(&lens_ms[k], e[0], e[1]));
+      walk_ms[k] = post_ms[k] = 0.0f;
+    }
   }
   c->pt_read = c->pt_next;
   *count = k;

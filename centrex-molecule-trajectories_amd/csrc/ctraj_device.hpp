@@ -148,11 +148,12 @@ __device__ __forceinline__ WsRef ws_of(const Tables &T, int state) {
 // min(u, umax) as one v_min_f64.  __builtin_fmin would also canonicalise umax on every call (a
 // second FP64 instruction per look-up) because the compiler cannot see that the table bound, a
 // normal double, needs none; u is an arithmetic result and umax a normal number, so the IEEE-mode
-// minimum is fmin's result, including umax for a NaN u.  (umax stays a VGPR operand: pinning it
-// to scalar registers costs the lens kernel a spill.)
+// minimum is fmin's result, including umax for a NaN u.  umax is wave-uniform and read from scalar
+// registers (a VOP3 operand may be one scalar register pair), so no vector copy of it is made per
+// look-up.
 __device__ __forceinline__ double clamp_to(double u, double umax) {
   double r;
-  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(u), "v"(umax));
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(u), "s"(umax));
   return r;
 }
 

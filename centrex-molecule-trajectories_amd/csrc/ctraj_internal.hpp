@@ -44,6 +44,7 @@ struct ctraj_ctx {
   // CTRAJ_RUN_PHASE_TIMES: per launch, events before the first kernel, between the kernels and after the
   // last one, in a ring of 64 launches; launches [pt_read, pt_next) are not yet read
   hipEvent_t pt_ev[64][4] = {};
+  bool pt_split[64] = {};  // the launch ran as separate kernels of one slice (events 0-3), else events 0-1 bracket it
   int64_t pt_next = 0, pt_read = 0;
   // smooth objective: per-wave checkpoint/recompute store and per-item partial sums
   ctraj_devbuf sm_ws, sm_part;

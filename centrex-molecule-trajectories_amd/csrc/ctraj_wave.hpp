@@ -43,14 +43,40 @@ __device__ __forceinline__ int lanes_below(uint64_t mask) {
 }
 
 // A lens-entry queue entry is 64 bytes, four 16-byte vectors: (x, y) (z, vx) (vy, vz) (t, key) with
-// key = molecule index within the run | lens state << 56 (the pipeline runs at most 2^27
-// molecules).  Each field is kept as its bit pattern.
+// key = molecule index within the slice (bits 0-39) | status bits << 40 | lens outcome << 48 | lens state
+// << 56; the walk to the lens queues a molecule with status and outcome 0, and the lens kernel writes
+// the molecule's state where the lens left it back into the same entry with both set.  Each field is
+// kept as its bit pattern.
+constexpr int64_t QKEY_INDEX = ((int64_t)1 << 40) - 1;
+
 __device__ __forceinline__ void queue_put(longlong2 *q, int64_t e, const MState &m, int64_t i, int st) {
   longlong2 *p = q + 4 * e;
   p[0] = make_longlong2(__double_as_longlong(m.x), __double_as_longlong(m.y));
   p[1] = make_longlong2(__double_as_longlong(m.z), __double_as_longlong(m.vx));
   p[2] = make_longlong2(__double_as_longlong(m.vy), __double_as_longlong(m.vz));
   p[3] = make_longlong2(__double_as_longlong(m.t), (long long)(i | ((int64_t)st << 56)));
+}
+
+__device__ __forceinline__ void queue_put_done(longlong2 *q, int64_t e, double x, double y, double z, double vx,
+                                               double vy, double vz, double t, int64_t key) {
+  longlong2 *p = q + 4 * e;
+  p[0] = make_longlong2(__double_as_longlong(x), __double_as_longlong(y));
+  p[1] = make_longlong2(__double_as_longlong(z), __double_as_longlong(vx));
+  p[2] = make_longlong2(__double_as_longlong(vy), __double_as_longlong(vz));
+  p[3] = make_longlong2(__double_as_longlong(t), (long long)key);
+}
+
+__device__ __forceinline__ void queue_get_done(const longlong2 *q, int64_t e, MState &m, int64_t &key) {
+  const longlong2 *p = q + 4 * e;
+  const longlong2 a = p[0], b = p[1], c = p[2], d = p[3];
+  m.x = __longlong_as_double(a.x);
+  m.y = __longlong_as_double(a.y);
+  m.z = __longlong_as_double(b.x);
+  m.vx = __longlong_as_double(b.y);
+  m.vy = __longlong_as_double(c.x);
+  m.vz = __longlong_as_double(c.y);
+  m.t = __longlong_as_double(d.x);
+  key = (int64_t)d.y;
 }
 
 __device__ __forceinline__ void queue_get(const longlong2 *q, int64_t e, MState &m, int64_t &i, int &st) {
@@ -63,7 +89,7 @@ __device__ __forceinline__ void queue_get(const longlong2 *q, int64_t e, MState 
   m.vy = __longlong_as_double(c.x);
   m.vz = __longlong_as_double(c.y);
   m.t = __longlong_as_double(d.x);
-  i = (int64_t)d.y & (((int64_t)1 << 56) - 1);
+  i = (int64_t)d.y & QKEY_INDEX;
   st = (int)((uint64_t)d.y >> 56);
 }
 
@@ -82,6 +108,14 @@ __device__ __forceinline__ void wave_hist(uint32_t *s_hist, unsigned long long *
     if (lane == leader) atomicAdd(&s_hist[lk], (uint32_t)__popcll(same));
     todo &= ~same;
   }
+}
+
+// Lane 0's value as a wave-uniform (scalar) value; every lane must be active.  Unlike a shuffle, the
+// compiler knows the result is uniform, so what is computed from it stays in scalar registers.
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
 }
 
 // Sum over the 64 lanes by an xor butterfly: every lane receives the same, bitwise identical

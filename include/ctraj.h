@@ -175,8 +175,9 @@ typedef struct ctraj_run {
 int32_t ctraj_propagate(ctraj_ctx *ctx, const ctraj_run *run);
 /* Device times (ms) of the kernels of the launches made with CTRAJ_RUN_PHASE_TIMES since the last call, oldest
  * first: walk_ms[i] is the walk to and through the lens entrance of a run split into kernels, lens_ms[i] its lens
- * kernel and post_ms[i] its walk of the elements after the lens (walk and post are 0 for a single-kernel run, whose
- * whole kernel is lens_ms[i]).  No counterpart in the reference (it has no device).  Waits for those launches;
+ * kernel and post_ms[i] its walk of the elements after the lens and recording of the lens molecules (walk and post
+ * are 0 for a single-kernel run, and for a run of several slices -- more than 2^26 molecules --, whose whole time
+ * is lens_ms[i]).  No counterpart in the reference (it has no device).  Waits for those launches;
  * writes at most max entries (the most recent ones; at most 64 are kept) to the host arrays, sets *count, and
  * forgets every pending measurement. */
 int32_t ctraj_phase_times(ctraj_ctx *ctx, int32_t max, float *walk_ms, float *lens_ms, float *post_ms,

@@ -118,3 +118,45 @@ def test_config4_streamed_at_1e8_state_tallies_detected_set_and_samples(ctx):
         want = O.zgrid_from_rows(ro["rows"][q], int(ro["n_rows"][q]), run.z_start, run.z_step, res.n_samples).T
         same = (got[q] == want) | (np.isnan(got[q]) & np.isnan(want))
         assert same.all(), (int(sel[q]), np.argwhere(~same)[:5])
+
+
+def test_a_run_of_several_slices_equals_the_lane_walker(ctx):
+    """More than 2^26 molecules: the split kernels run the lens-entry queue slice by slice inside one
+    ctraj_propagate; every molecule's label and final phase space, the tallies, the work counters and the
+    kept molecules equal the single-kernel lane walker's (which needs no queue)."""
+    from trajectories.engine import draw_gaussian
+
+    n = (1 << 26) + 1_000_003
+    eng = Engine(configs.config1(), ctx)
+    xs = draw_gaussian(ctx, n, configs.SOURCE_MEAN, configs.SOURCE_SIGMA, seed=123)
+    inp = dict(zip(("x", "y", "z", "vx", "vy", "vz"), xs))
+    res = []
+    for flags in (0, N.RUN_LANES):
+        tally = ctx.alloc(eng.cb.n_labels, np.int64).fill(0)
+        work = ctx.alloc(N.WORK_N, np.uint64).fill(0)
+        label = ctx.alloc(n, np.int32)
+        out = {k: ctx.alloc(n) for k in ("x", "y", "t")}
+        count = ctx.alloc(1, np.int64).fill(0)
+        cap = n // 50
+        index = ctx.alloc(cap, np.int64)
+        kept = {k: ctx.alloc(cap) for k in FIELDS}
+        eng.launch(inp, out=out, label=label, tally=tally, work=work, aoi_mask=1 << eng.cb.label_detected,
+                   aoi_cap=cap, aoi_count=count, aoi_index=index, aoi_out=kept, n=n, flags=flags)
+        ctx.sync()
+        cnt = int(count.to_host()[0])
+        assert 0 < cnt <= cap
+        o = np.argsort(index.to_host()[:cnt])
+        res.append({"tally": tally.to_host(), "work": work.to_host(), "label": label.to_host(),
+                    "out": {k: v.to_host() for k, v in out.items()}, "index": index.to_host()[:cnt][o],
+                    "kept": {k: v.to_host()[:cnt][o] for k, v in kept.items()}})
+        del label, out, kept
+    a, b = res
+    assert int(a["tally"].sum()) == n
+    np.testing.assert_array_equal(a["tally"], b["tally"])
+    np.testing.assert_array_equal(a["work"][:4], b["work"][:4])
+    np.testing.assert_array_equal(a["label"], b["label"])
+    for k in a["out"]:
+        np.testing.assert_array_equal(a["out"][k], b["out"][k])
+    np.testing.assert_array_equal(a["index"], b["index"])
+    for k in FIELDS:
+        np.testing.assert_array_equal(a["kept"][k], b["kept"][k])
