@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1214,8 +1215,29 @@ static RecFn rec_kernel(int f, int i, int mode, int km) {
   return nullptr;
 }
 
+// Device memory of ctraj_propagate_noalloc: one array per device (each device that loads the library
+// holds its own copy until the process ends).  In host code `scratch` names a host-side stand-in, not
+// the device array: the array's address on each device is resolved once with hipGetSymbolAddress
+// (scratch_on_device) and only that address is handed to kernels.
+static __device__ __align__(256) char scratch[CTRAJ_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static char *g_scratch_dev[256];  // per device ordinal, resolved by ctraj_ctx_create
+
+// The calling thread's current device must be `device`.
+static hipError_t scratch_on_device(int device, char **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= 256) return hipErrorInvalidDevice;
+  if (!g_scratch_dev[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[device] = (char *)p;
+  }
+  *out = g_scratch_dev[device];
+  return hipSuccess;
+}
+
 // Scratch layout of ctraj_propagate_noalloc (byte offsets into the device's scratch array).
-constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
 constexpr size_t SCR_CLAIM = 0;            // [8][32] claim counters
 constexpr size_t SCR_COUNTER = 2048;       // lane walker's molecule counter
 constexpr size_t SCR_RUN = 4096;           // device copy of the run (lane walker)
@@ -1252,9 +1274,10 @@ __global__ void k_put_run(ctraj_run R, ctraj_run *dst) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = R;
 }
 
-// ctraj_propagate and ctraj_propagate_noalloc.  scratch == nullptr: the context's buffers (grown as
-// needed); otherwise every device buffer the launches need lies in `scratch` and nothing is allocated.
-static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *scratch) {
+// ctraj_propagate and ctraj_propagate_noalloc.  sdev == nullptr: the context's buffers (grown as
+// needed); otherwise sdev is this device's scratch array (its resolved device address), every device
+// buffer the launches need lies in it, and nothing is allocated.
+static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *sdev) {
   if (!c || !R) return fail(c, CTRAJ_EINVAL, "null argument");
   if (!c->have_beamline) return fail(c, CTRAJ_ESTATE, "no beamline set");
   if (R->n < 0) return fail(c, CTRAJ_EINVAL, "n < 0");
@@ -1264,7 +1287,7 @@ static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *scratch) {
     return fail(c, CTRAJ_EINVAL, "aoi compaction needs aoi_count, aoi_index and aoi_cap");
   if (c->phys.n_labels < 64 && (R->aoi_mask >> c->phys.n_labels))
     return fail(c, CTRAJ_EINVAL, "aoi_mask names labels beyond n_labels");
-  if (scratch && (R->flags & CTRAJ_RUN_PHASE_TIMES))
+  if (sdev && (R->flags & CTRAJ_RUN_PHASE_TIMES))
     return fail(c, CTRAJ_EINVAL, "CTRAJ_RUN_PHASE_TIMES creates events: not available without allocation");
   int rc0 = check_tables(c, c->h_elems.data(), c->n_elems, c->phys.n_states);
   if (rc0) return rc0;
@@ -1332,15 +1355,18 @@ static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *scratch) {
     int gpre = (grid_for(c, (const void *)kpre, blk, lds_aoi, n_first, R->grid) + 7) & ~7;  // whole XCD groups
     int64_t slice = n_first;
     PipeArgs P{};
-    if (scratch) {
-      const size_t q_entries = (NOALLOC_SCRATCH_BYTES - SCR_QUEUE) / 64;
+    if (sdev) {
+      const size_t q_entries = (CTRAJ_NOALLOC_SCRATCH_BYTES - SCR_QUEUE) / 64;
       const int g_cap = (int)std::min<size_t>(q_entries / blk, SCR_WCOUNT_MAX / (blk / 64)) & ~7;
+      if (R->grid > g_cap)
+        return fail(c, CTRAJ_EINVAL, "grid of " + std::to_string(R->grid) + " blocks exceeds the " +
+                                         std::to_string(g_cap) + " the scratch array's queue counts can take");
       gpre = std::min(gpre, g_cap);
       const int64_t threads = (int64_t)gpre * blk;
       slice = std::min<int64_t>(R->n, (int64_t)(q_entries / threads) * threads);
-      P.q = (longlong2 *)(scratch + SCR_QUEUE);
-      P.claim = (unsigned long long *)(scratch + SCR_CLAIM);
-      P.wcount = (uint32_t *)(scratch + SCR_WCOUNT);
+      P.q = (longlong2 *)(sdev + SCR_QUEUE);
+      P.claim = (unsigned long long *)(sdev + SCR_CLAIM);
+      P.wcount = (uint32_t *)(sdev + SCR_WCOUNT);
     } else {
       const int64_t threads = (int64_t)gpre * blk;
       const size_t cap = (size_t)threads * (size_t)((slice + threads - 1) / threads);
@@ -1390,9 +1416,9 @@ static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *scratch) {
     if (rc) return rc;
     void *counter = nullptr;
     ctraj_run *d_run = nullptr;
-    if (scratch) {
-      counter = scratch + SCR_COUNTER;
-      d_run = (ctraj_run *)(scratch + SCR_RUN);
+    if (sdev) {
+      counter = sdev + SCR_COUNTER;
+      d_run = (ctraj_run *)(sdev + SCR_RUN);
       hipLaunchKernelGGL(k_put_run, dim3(1), dim3(64), 0, c->stream, *R, d_run);
       HIPCHK(c, hipGetLastError());
     } else {
@@ -1484,6 +1510,12 @@ int32_t ctraj_ctx_create(int32_t device, ctraj_ctx **out) {
   if (std::strncmp(prop.gcnArchName, "gfx", 3) == 0) c->arch = (int)std::strtol(prop.gcnArchName + 3, nullptr, 10);
   c->wave = prop.warpSize;
   for (int i = 0; i < 64; ++i) c->ev[i] = nullptr;
+  // the device's scratch array (ctraj_propagate_noalloc), resolved once per device with it current;
+  // a failure here only disables ctraj_propagate_noalloc
+  if (scratch_on_device(device, &c->scratch) != hipSuccess) {
+    c->scratch = nullptr;
+    (void)hipGetLastError();
+  }
   *out = c;
   return CTRAJ_OK;
 }
@@ -1679,6 +1711,12 @@ int32_t ctraj_set_wspline(ctraj_ctx *c, int32_t n_states, int32_t n_seg, const d
 }
 
 int32_t ctraj_propagate(ctraj_ctx *c, const ctraj_run *R) { return propagate_impl(c, R, nullptr); }
+
+int32_t ctraj_propagate_noalloc(ctraj_ctx *c, const ctraj_run *R) {
+  if (!c || !R) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (!c->scratch) return fail(c, CTRAJ_EHIP, "the scratch array's device address could not be resolved");
+  return propagate_impl(c, R, c->scratch);
+}
 
 int32_t ctraj_phase_times(ctraj_ctx *c, int32_t max, float *walk_ms, float *lens_ms, float *post_ms, int32_t *count) {
   if (!c || !count || max < 0 || (max > 0 && (!walk_ms || !lens_ms || !post_ms)))

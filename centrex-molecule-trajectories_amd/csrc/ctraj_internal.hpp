@@ -50,8 +50,10 @@ struct ctraj_ctx {
   ctraj_devbuf sm_ws, sm_part;
   // radix sort scratch (ctraj_sort_i64)
   ctraj_devbuf sort_tmp, sort_alt;
-  // three-phase propagation: lens-entry / lens-exit queues and their counters
+  // propagation in separate kernels: the lens-entry queue, its claim and count words
   ctraj_devbuf pipe, pipe_cnt;
+  // this device's scratch array of ctraj_propagate_noalloc (device address), or nullptr
+  char *scratch = nullptr;
 };
 
 namespace ctraj {

@@ -192,6 +192,7 @@ SIGNATURES = [
     ("ctraj_set_rtable", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]),
     ("ctraj_set_wspline", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
     ("ctraj_propagate", C.c_int32, [C.c_void_p, C.POINTER(Run)]),
+    ("ctraj_propagate_noalloc", C.c_int32, [C.c_void_p, C.POINTER(Run)]),
     ("ctraj_phase_times", C.c_int32,
      [C.c_void_p, C.c_int32] + [C.POINTER(C.c_float)] * 3 + [C.POINTER(C.c_int32)]),
     ("ctraj_record_trajectories", C.c_int32, [C.c_void_p, C.POINTER(Record)]),

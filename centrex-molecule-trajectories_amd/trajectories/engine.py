@@ -63,8 +63,9 @@ class Engine:
                first_index=None, aoi_mask: int = 0, aoi_cap: int = 0, aoi_count=None, aoi_index=None,
                aoi_label=None, aoi_out: dict | None = None, work=None, a0=None, state=None, n: int | None = None,
                index_base: int = 0, z_const: float = 0.0, t_const: float = 0.0, block: int = 0, grid: int = 0,
-               flags: int = 0, refill: int = 0) -> None:
-        """Queue one ``ctraj_propagate`` on device buffers (no synchronisation)."""
+               flags: int = 0, refill: int = 0, noalloc: bool = False) -> None:
+        """Queue one ``ctraj_propagate`` on device buffers (no synchronisation); ``noalloc`` calls
+        ``ctraj_propagate_noalloc``, which allocates no device memory."""
         bind(self.ctx, self.cb)
         R = N.Run()
         R.n = int(n if n is not None else len(inp["x"]))
@@ -83,7 +84,11 @@ class Engine:
         R.aoi_out = _soa(aoi_out)
         R.work = N.addr(work)
         R.block, R.grid, R.flags, R.refill = int(block), int(grid), int(flags), int(refill)
-        N.check("ctraj_propagate", N.lib().ctraj_propagate(self.ctx.handle, C.byref(R)), self.ctx.handle)
+        if noalloc:
+            N.check("ctraj_propagate_noalloc", N.lib().ctraj_propagate_noalloc(self.ctx.handle, C.byref(R)),
+                    self.ctx.handle)
+        else:
+            N.check("ctraj_propagate", N.lib().ctraj_propagate(self.ctx.handle, C.byref(R)), self.ctx.handle)
 
     # ------------------------------------------------------------------ host-array form
     def _upload_inputs(self, x, y, z, vx, vy, vz, t, a0, state):
@@ -105,7 +110,7 @@ class Engine:
 
     def propagate(self, x, y, z, vx, vy, vz, t=None, a0=None, state=None, *, outputs: bool = True,
                   want_accel: bool = False, aoi_labels=(), index_base: int = 0, block: int = 0, flags: int = 0,
-                  refill: int = 0) -> dict:
+                  refill: int = 0, noalloc: bool = False) -> dict:
         """Propagate host arrays; returns labels, tallies, work counters and (optionally) final states."""
         ctx = self.ctx
         n, inp, a0d, st = self._upload_inputs(x, y, z, vx, vy, vz, t, a0, state)
@@ -133,7 +138,7 @@ class Engine:
                         aoi_mask=mask, aoi_cap=n if mask else 0, aoi_count=aoi.get("count"),
                         aoi_index=aoi.get("index"), aoi_label=aoi.get("label"), aoi_out=aoi.get("out"),
                         work=work, a0=a0d, state=st,
-                        n=n, index_base=index_base, block=block, flags=flags, refill=refill)
+                        n=n, index_base=index_base, block=block, flags=flags, refill=refill, noalloc=noalloc)
         ctx.sync()
         res["tally"] = tally.to_host().reshape(ns, nl)
         res["first_index"] = first.to_host()

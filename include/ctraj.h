@@ -173,6 +173,18 @@ typedef struct ctraj_run {
   int32_t refill;        /* lane refill: idle lanes per wavefront that trigger a refill (0 = 12 two-phase, 24 single kernel) */
 } ctraj_run;
 int32_t ctraj_propagate(ctraj_ctx *ctx, const ctraj_run *run);
+/* ctraj_propagate for callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid allocation).  No counterpart in the reference.  Same run, same results, bit for bit.
+ * Every device buffer its kernels use beyond the caller's inputs and outputs -- the lens-entry queue, the queue's
+ * claim and count words, the lane walker's molecule counter and its copy of the run -- lies in a library-owned
+ * array of CTRAJ_NOALLOC_SCRATCH_BYTES per device, reserved when the library is loaded on a device and held until
+ * the process ends; a beamline with a lens is processed in slices of at most as many molecules as that queue holds
+ * (about a million).  No hipMalloc, hipFree or event creation happens in the call: CTRAJ_RUN_PHASE_TIMES, and a
+ * run->grid larger than the scratch array's queue counts can take, are refused with CTRAJ_EINVAL.  The beamline and
+ * tables must be set beforehand (those calls allocate).  Calls on one device share its array, which keeps what the
+ * last call left there: calls that overlap on one device (several host threads or streams) race. */
+#define CTRAJ_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int32_t ctraj_propagate_noalloc(ctraj_ctx *ctx, const ctraj_run *run);
 /* Device times (ms) of the kernels of the launches made with CTRAJ_RUN_PHASE_TIMES since the last call, oldest
  * first: walk_ms[i] is the walk to and through the lens entrance of a run split into kernels, lens_ms[i] its lens
  * kernel and post_ms[i] its walk of the elements after the lens and recording of the lens molecules (walk and post
