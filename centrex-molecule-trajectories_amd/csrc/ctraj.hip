@@ -1313,9 +1313,10 @@ static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *sdev) {
   const bool lanes = n_lens == 1 && !(R->flags & CTRAJ_RUN_SIMPLE) && !pipeline;
   // The lane-refill kernels keep their tables once per workgroup: with several lens-state tables
   // (more than ~40 KB) one large workgroup per CU holds as many wavefronts as its registers allow,
-  // where 256-thread workgroups would be limited by LDS to one per CU.
-  // (one 1024-thread lens workgroup per CU holds 16 wavefronts beside several state tables)
-  const int block = R->block ? R->block : (pipeline && lds > 40960 ? 1024 : (lanes && lds > 40960 ? 768 : 256));
+  // where 256-thread workgroups would be limited by LDS to one per CU.  The lens kernel of the split
+  // path takes 1024-thread workgroups always: at 64 VGPRs two of them hold 8 wavefronts per SIMD
+  // beside two copies of one state's table (256-thread workgroups: 6, limited by LDS).
+  const int block = R->block ? R->block : (pipeline ? 1024 : (lanes && lds > 40960 ? 768 : 256));
   const int max_block = pipeline ? 1024 : (lanes ? 768 : 256);
   if (block < 64 || block > max_block || block % 64)
     return fail(c, CTRAJ_EINVAL, "block must be a multiple of 64 up to " + std::to_string(max_block));
@@ -1380,7 +1381,7 @@ static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *sdev) {
     const int64_t threads = (int64_t)gpre * blk;
     P.n_pre_waves = gpre * (blk / 64);
     P.lens_e = lens_e;
-    const int refill = R->refill > 0 ? std::min(R->refill, 64) : 12;
+    const int refill = R->refill > 0 ? std::min(R->refill, 64) : 8;
     const int grid = grid_for(c, (const void *)kl, block, lds, (int64_t)1 << 40, 0);
     const StepK K = step_k(c->h_elems[lens_e].p[2], c->h_elems[lens_e].p[4], c->phys.g);
     const bool one_slice = slice >= R->n;

@@ -133,17 +133,20 @@ int32_t ctraj_set_rtable(ctraj_ctx *ctx, int32_t id, const double *r, const doub
 int32_t ctraj_set_wspline(ctraj_ctx *ctx, int32_t n_states, int32_t n_seg, const double *coef);
 
 /* ---- forward propagation: replaces run_simulation's per-molecule loop (trajectory_simulator.py:52-78)
- *      and Beamline.propagate_through (beamline.py:20-38).  A beamline with exactly one lens runs in
- *      two phases queued on the stream: a streaming walk to and through the lens entrance that queues
- *      the molecules entering the lens (a context-owned queue of 64 bytes per molecule), then a
- *      persistent lane-refill kernel that integrates the lens and walks the elements after it.  Runs
- *      of more than 2^27 molecules, CTRAJ_RUN_LANES, or a WSPLINE_HEX lens whose force scale p[4]
- *      lies outside [2^-500, 2^500] in magnitude (V0 = 0, say: its RK4 step cannot fold ks into the
- *      step constants) use the single-kernel lane-refill walker; other beamlines, or
- *      CTRAJ_RUN_SIMPLE, run one molecule per thread.  All give bit-identical results. ---- */
+ *      and Beamline.propagate_through (beamline.py:20-38).  A beamline with exactly one lens runs as
+ *      three kernels queued on the stream, per slice of at most 2^26 molecules: a streaming walk to
+ *      and through the lens entrance that queues the molecules entering the lens (a context-owned
+ *      queue of 64 bytes per molecule of the slice), a persistent lane-refill kernel that integrates
+ *      the lens and writes each molecule's state where the lens left it back into its queue entry,
+ *      and a streaming walk of the elements after the lens that records the lens molecules.
+ *      CTRAJ_RUN_LANES, a WSPLINE_HEX lens whose force scale p[4] lies outside [2^-500, 2^500] in
+ *      magnitude (V0 = 0, say: its RK4 step cannot fold ks into the step constants), or a
+ *      reference-mode lens with stored accelerations requested use the single-kernel lane-refill
+ *      walker; other beamlines, or CTRAJ_RUN_SIMPLE, run one molecule per thread.  All give
+ *      bit-identical results. ---- */
 #define CTRAJ_RUN_STATUS_OR 1 /* OR status bits into run->status instead of overwriting */
 #define CTRAJ_RUN_SIMPLE 2    /* one molecule per thread start to finish (no lane refill) */
-#define CTRAJ_RUN_LANES 4     /* one lens: the single-kernel lane-refill walker instead of the two phases */
+#define CTRAJ_RUN_LANES 4     /* one lens: the single-kernel lane-refill walker instead of the split kernels */
 #define CTRAJ_RUN_PHASE_TIMES 8 /* time the launch's kernels with HIP events on the stream (ctraj_phase_times) */
 typedef struct ctraj_run {
   int64_t n;             /* molecules */
@@ -165,12 +168,12 @@ typedef struct ctraj_run {
   int32_t *aoi_label;    /* [aoi_cap] their labels (optional) */
   ctraj_soa aoi_out;     /* [aoi_cap] their final phase space (optional fields) */
   uint64_t *work;        /* [CTRAJ_WORK_N] counters, accumulated (optional) */
-  int32_t block;         /* threads per block, multiple of 64: up to 1024 for the two-phase lens kernel and 768
-                          * for the lane walker (one lens), else 256; 0 = 256, or 1024 / 768 when the lens
-                          * tables exceed 40 KB of LDS */
+  int32_t block;         /* threads per block, multiple of 64: up to 1024 for the split path's lens kernel and 768
+                          * for the lane walker (one lens), else 256; 0 = 1024 for the split path's lens kernel,
+                          * 768 for the lane walker when the lens tables exceed 40 KB of LDS, else 256 */
   int32_t grid;          /* blocks (0 = enough for n, capped at 8 per CU x n_cu x 8) */
   int32_t flags;         /* CTRAJ_RUN_* */
-  int32_t refill;        /* lane refill: idle lanes per wavefront that trigger a refill (0 = 12 two-phase, 24 single kernel) */
+  int32_t refill;        /* lane refill: idle lanes per wavefront that trigger a refill (0 = 8 split path, 24 single kernel) */
 } ctraj_run;
 int32_t ctraj_propagate(ctraj_ctx *ctx, const ctraj_run *run);
 /* ctraj_propagate for callers that cannot allocate device memory (code under HIP graph capture, serving loops,
