@@ -91,6 +91,16 @@ __host__ __device__ inline StepK step_k(double h, double ks, double g) {
   return StepK{hh, h6, ks * hh, ks * h, ks * h6, hex_folds(ks) ? -g / ks : 0.0};
 }
 
+// Claim order of the lens kernel.  A wavefront's lanes stay busy only while its molecules end
+// together, and the lens kernel ends when the last-claimed molecules do, so the queue is claimed
+// longest-lived first: PRE files each queued molecule under a bucket of its predicted number of
+// lens steps (ORD_WIDTH steps per bucket, the last bucket open-ended), k_ord_scan and
+// k_ord_scatter list the queue's slots by bucket, highest first, and the lens kernel claims that
+// list in 64-slot chunks.  The order changes which lane integrates a molecule when, never its
+// arithmetic: every molecule's result is the same in any order.
+constexpr int ORD_BUCKETS = 256;
+constexpr float ORD_WIDTH = 2.0f;
+
 struct PipeArgs {
   longlong2 *q;                // lens-entry queue, 4 vectors per entry
   uint32_t *wcount;            // PRE: entries queued by each wavefront (a prefix of its region)
@@ -98,7 +108,50 @@ struct PipeArgs {
   int n_pre_waves;
   unsigned long long *claim;   // lens kernel: next unclaimed 64-entry chunk of each region group, [8][32]
   int lens_e;
+  // claim order (all null: the lens kernel claims the regions in order)
+  uint8_t *bkt;                // PRE: lifetime bucket of each queue entry
+  uint32_t *ord_hist;          // PRE: bucket counts of each PRE workgroup [block][ORD_BUCKETS]; k_ord_scan turns
+                               // each bucket's column into exclusive prefixes over the workgroups
+  uint32_t *ord_tot;           // [ORD_BUCKETS] entries per bucket, then the queue's total entry count
+  uint32_t *perm;              // queue slots in claim order
+  float ord_k2[CTRAJ_MAX_STATES];  // per state: the acceleration per unit radius at the W' table's end
+  int ord_ref;                 // the lens integrates in the reference's steps of dz (else of dt)
 };
+
+// Predicted lens steps of a molecule entering a hexapole lens at (x, y) with transverse velocity
+// (vx, vy) and axial speed vz, as the ordering bucket: the time until its radius reaches the bore
+// under the linear acceleration k2 r (the force where the W' table saturates; k2 > 0 pushes out,
+// r^2(t) = P cosh 2kt + Q sinh 2kt + S; k2 < 0 pulls in, r^2(t) = A + D cos(2wt - phi)), in steps,
+// at most the lens's full steps.  Single precision: the prediction orders work and is not a result.
+__device__ __forceinline__ int ord_bucket(float k2, float x, float y, float vx, float vy, float vz, float bore2,
+                                          float len, float h, float cap, int ref) {
+  const float r0s = x * x + y * y, v0s = vx * vx + vy * vy, rv = x * vx + y * vy;
+  float t = __builtin_inff();  // the molecule stays inside the bore
+  if (k2 > 0.0f) {
+    const float k = __builtin_sqrtf(k2), iv = v0s / k2;
+    const float P = 0.5f * (r0s + iv), Q = rv / k, S = 0.5f * (r0s - iv);
+    // u = e^(2kt) solves (P + Q) u^2 + 2 (S - bore2) u + (P - Q) = 0 (larger root)
+    const float a = P + Q, b = 2.0f * (S - bore2), c = P - Q;
+    const float u = (__builtin_sqrtf(__builtin_fmaxf(b * b - 4.0f * a * c, 0.0f)) - b) / (2.0f * a);
+    t = __logf(__builtin_fmaxf(u, 1.0f)) / (2.0f * k);
+  } else if (k2 < 0.0f) {
+    const float w = __builtin_sqrtf(-k2), iv = v0s / -k2;
+    const float A = 0.5f * (r0s + iv), Bc = 0.5f * (r0s - iv), C = rv / w;
+    const float D = __builtin_sqrtf(Bc * Bc + C * C);
+    if (A + D > bore2) {  // first upward crossing of bore2: 2wt = (phi - acos((bore2 - A) / D)) mod 2 pi
+      const float al = acosf(__builtin_fminf(__builtin_fmaxf((bore2 - A) / D, -1.0f), 1.0f));
+      float th = atan2f(C, Bc) - al;
+      th -= 6.28318531f * __builtin_floorf(th * 0.159154943f);
+      t = th / (2.0f * w);
+    }
+  }
+  // full steps: floor(L / (vz dt)) capped (configuration mode), n = rint(L/dz) (reference mode)
+  const float steps_max = ref ? cap : __builtin_fminf(__builtin_floorf(len / (vz * h)), cap);
+  float s = t * (ref ? vz / h : 1.0f / h);
+  s = __builtin_fminf(s, steps_max);  // a NaN time gives the full steps
+  s = s >= 0.0f ? s : 0.0f;           // and a NaN speed bucket 0
+  return (int)__builtin_fminf(s * (1.0f / ORD_WIDTH), (float)(ORD_BUCKETS - 1));
+}
 
 // Outcome of a lens molecule, written by k_lens into its queue entry's key (queue_put_done).
 enum : int { LENS_QUEUED = 0, LENS_STOPPED = 1, LENS_LEFT = 2 };
@@ -157,8 +210,11 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_work[5];
+  __shared__ uint32_t s_ord[PHASE == PHASE_PRE ? ORD_BUCKETS : 1];  // PRE: this workgroup's lifetime buckets
   const int nbins = B.n_states * B.n_labels;
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0;
+  if constexpr (PHASE == PHASE_PRE)
+    for (int b = threadIdx.x; b < ORD_BUCKETS; b += blockDim.x) s_ord[b] = 0;
   for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x) s_first[b] = ~0ull >> 1;
   if (threadIdx.x < 5) s_work[threadIdx.x] = 0;
   Tables T{};
@@ -189,11 +245,16 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
   // for the prefetched molecules and the queue stores in flight)
   double lens_z0 = 0.0, lens_r = 0.0;
   int lens_hit = 0;
+  float ord_bore2 = 0.0f, ord_len = 0.0f, ord_h = 0.0f, ord_cap = 0.0f;  // lifetime prediction (P.bkt)
   if constexpr (PHASE == PHASE_PRE) {
     const ctraj_elem L = elem_at(B.elems, P.lens_e);
     lens_z0 = L.z0;
     lens_r = L.p[0];
     lens_hit = L.label_hit;
+    ord_bore2 = (float)L.p[1];
+    ord_h = (float)L.p[2];
+    ord_len = (float)L.p[3];
+    ord_cap = P.ord_ref ? (float)L.n_steps : (float)B.cap;
   }
   int lens_inside = 0;
   if constexpr (PHASE == PHASE_POST) lens_inside = elem_at(B.elems, P.lens_e).label_inside;
@@ -272,7 +333,14 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
       // wavefront-private region of the queue: no atomics
       const uint64_t qm = __ballot(enq);
       if (enq) {
-        queue_put(P.q, gwave * P.region + qn + lanes_below(qm), m, i, st);
+        const int64_t e = gwave * P.region + qn + lanes_below(qm);
+        queue_put(P.q, e, m, i, st);
+        if (P.bkt) {
+          const int b = ord_bucket(P.ord_k2[st], (float)m.x, (float)m.y, (float)m.vx, (float)m.vy, (float)m.vz,
+                                   ord_bore2, ord_len, ord_h, ord_cap, P.ord_ref);
+          P.bkt[e] = (uint8_t)b;
+          atomicAdd(&s_ord[b], 1u);
+        }
       }
       qn += __popcll(qm);
     }
@@ -359,6 +427,11 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
       if (s_first[b] != (~0ull >> 1)) atomicMin((unsigned long long *)&R.first_index[b], s_first[b]);
   if (R.work && threadIdx.x < 5 && s_work[threadIdx.x])
     atomicAdd((unsigned long long *)&R.work[threadIdx.x], s_work[threadIdx.x]);
+  if constexpr (PHASE == PHASE_PRE) {
+    if (P.bkt)  // every workgroup writes its whole row
+      for (int b = threadIdx.x; b < ORD_BUCKETS; b += blockDim.x)
+        P.ord_hist[(size_t)blockIdx.x * ORD_BUCKETS + b] = s_ord[b];
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -639,15 +712,81 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
 // the next one when it runs dry, which spreads the claim atomics over eight addresses.  Chunk c of
 // group g is entries [64 (c % cpr), +64) of the group's region c / cpr (cpr = region / 64), of
 // which the first wcount are filled, so a claim needs no search and a chunk past a region's count
-// is simply claimed over.  The per-molecule arithmetic is the one of walk().
+// is simply claimed over.  With a claim list (longest-lived first, see ORD_BUCKETS) counter g hands
+// out the list's chunks g, g + 8, g + 16, ... instead.  The per-molecule arithmetic is the one of
+// walk().
 // ---------------------------------------------------------------------------------------------
+
+// Block-wide exclusive scan of one value per thread (256 threads); returns the sum of the values of
+// the threads below, and the total in *total.
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t *s, uint32_t *total) {
+  const int t = threadIdx.x;
+  s[t] = v;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    const uint32_t add = t >= d ? s[t - d] : 0u;
+    __syncthreads();
+    s[t] += add;
+    __syncthreads();
+  }
+  const uint32_t incl = s[t];
+  *total = s[255];
+  __syncthreads();
+  return incl - v;
+}
+
+// Claim order, step 1: block b (one per bucket, 256 threads) replaces column b of the PRE
+// workgroups' bucket counts by its exclusive prefix over the workgroups and writes the bucket's
+// total; block 0 also leaves the queue's total in ord_tot[ORD_BUCKETS] (below, after step 2).
+__global__ void __launch_bounds__(256) k_ord_scan(uint32_t *hist, int n_rows, uint32_t *tot) {
+  __shared__ uint32_t s[256];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int per = (n_rows + 255) / 256;
+  const int r0 = min(t * per, n_rows), r1 = min(r0 + per, n_rows);
+  uint32_t sum = 0;
+  for (int r = r0; r < r1; ++r) sum += hist[(size_t)r * ORD_BUCKETS + b];
+  uint32_t total;
+  uint32_t run = block_excl_scan256(sum, s, &total);
+  for (int r = r0; r < r1; ++r) {
+    const size_t k = (size_t)r * ORD_BUCKETS + b;
+    const uint32_t v = hist[k];
+    hist[k] = run;
+    run += v;
+  }
+  if (t == 0) tot[b] = total;
+}
+
+// Claim order, step 2: PRE workgroup g's entries of bucket b go to positions base(b) + prefix(g, b)
+// on, where base(b) counts the entries of the buckets above b; each of the workgroup's four
+// wavefronts lists the filled slots of its region (the order within a bucket is immaterial).
+__global__ void __launch_bounds__(256) k_ord_scatter(PipeArgs P) {
+  __shared__ uint32_t s[256];
+  __shared__ uint32_t cur[ORD_BUCKETS];
+  const int t = threadIdx.x;
+  // base(b) = sum of the totals of buckets b+1 .. ORD_BUCKETS-1: an exclusive scan in reversed order
+  const uint32_t tb = P.ord_tot[ORD_BUCKETS - 1 - t];
+  uint32_t total;
+  const uint32_t above = block_excl_scan256(tb, s, &total);
+  cur[ORD_BUCKETS - 1 - t] = above + P.ord_hist[(size_t)blockIdx.x * ORD_BUCKETS + (ORD_BUCKETS - 1 - t)];
+  if (blockIdx.x == 0 && t == 0) P.ord_tot[ORD_BUCKETS] = total;
+  __syncthreads();
+  const int64_t w = (int64_t)blockIdx.x * 4 + (t >> 6);
+  const int64_t e0 = w * P.region;
+  const int64_t cnt = P.wcount[w];
+  for (int64_t k = t & 63; k < cnt; k += 64) {
+    const int64_t e = e0 + k;
+    const uint32_t pos = atomicAdd(&cur[P.bkt[e]], 1u);
+    P.perm[pos] = (uint32_t)e;
+  }
+}
 
 // Six wavefronts per SIMD (at most 80 VGPRs) for the hexapole RK4 instantiation, which fits them
 // without spilling (with one lens state, 256-thread workgroups and 24 KB of tables, LDS holds six
 // workgroups per CU); the others need more registers than that.
 constexpr int REM_EMPTY = -1, REM_STOPPED = -2;
 
-template <int FORCE, int INTEG>
+// ORD: the wavefronts claim the list P.perm (longest-lived first) instead of the regions in order.
+template <int FORCE, int INTEG, bool ORD>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
     FORCE == CTRAJ_FORCE_WSPLINE_HEX && INTEG == CTRAJ_INTEG_RK4 ? 6 : 1)))
 k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long *work) {
@@ -656,9 +795,12 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
   if (threadIdx.x < 2) s_work[threadIdx.x] = 0;
   const Tables T = stage_tables(lds, KT);
   __syncthreads();
-  const uint32_t cpr = (uint32_t)(P.region >> 6);
-  // eight claim counters, one per group of PRE regions written by one XCD's workgroups
-  const unsigned long long n_chunks = (unsigned long long)(P.n_pre_waves >> 3) * cpr;
+  // eight claim counters: counter g serves the regions of one XCD's PRE workgroups (n_chunks over
+  // cpr chunks per region) or, with the claim list, the list's chunks g, g + 8, g + 16, ...
+  const uint32_t cpr = ORD ? 0u : (uint32_t)(P.region >> 6);
+  const uint32_t n_q = ORD ? P.ord_tot[ORD_BUCKETS] : 0u;
+  const unsigned long long n_chunks =
+      ORD ? (unsigned long long)((n_q + 63u) >> 6) : (unsigned long long)(P.n_pre_waves >> 3) * cpr;
   int grp = blockIdx.x & 7, tried = 0;
 
   const int lane = lane_id();
@@ -730,6 +872,17 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
           unsigned long long c = 0;
           if (lane == 0) c = atomicAdd(P.claim + 32 * grp, 1ull);
           c = uniform_u64(c);  // every lane is active here, lane 0 first
+          if constexpr (ORD) {
+            const unsigned long long j = c * 8 + (unsigned)grp;
+            if (j >= n_chunks) {
+              if (++tried == 8) exhausted = true;
+              grp = (grp + 1) & 7;
+              continue;
+            }
+            next = (int64_t)j * 64;
+            end = next + 64 < (int64_t)n_q ? next + 64 : (int64_t)n_q;
+            continue;
+          }
           if (c >= n_chunks) {
             if (++tried == 8) exhausted = true;
             grp = (grp + 1) & 7;
@@ -748,6 +901,7 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
           const int rank = lanes_below(need);
           if (rank < avail) {
             slot = next + rank;
+            if constexpr (ORD) slot = P.perm[slot];
             queue_get_done(P.q, slot, m, key);
             m.ax = 0.0;  // the lens entrance row stores (0, -g, 0)
             m.ay = -B.g;
@@ -1168,9 +1322,14 @@ static PropFn prop_kernel(int f, int i, bool has_lens, int km, bool out, bool mi
   return nullptr;
 }
 
-static LensFn lens_kernel(int f, int i) {
+static LensFn lens_kernel(int f, int i, bool ord) {
+  if (ord) {  // the claim list is built for hexapole lenses only
+    if (f == CTRAJ_FORCE_WSPLINE_HEX && i == CTRAJ_INTEG_RK4) return k_lens<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_RK4, true>;
+    if (f == CTRAJ_FORCE_WSPLINE_HEX && i == CTRAJ_INTEG_REF) return k_lens<CTRAJ_FORCE_WSPLINE_HEX, CTRAJ_INTEG_REF, true>;
+    return nullptr;
+  }
 #define X(F, I) \
-  if (f == F && i == I) return k_lens<F, I>;
+  if (f == F && i == I) return k_lens<F, I, false>;
   CT_FI_LIST(X)
 #undef X
   return nullptr;
@@ -1330,7 +1489,11 @@ static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *sdev) {
     pev = c->pt_ev[slot];
   }
   if (pipeline) {
-    LensFn kl = lens_kernel(c->force, c->integ);
+    // claim order, longest-lived first: hexapole lenses (whose W' table gives the prediction's force
+    // scale), unless the run asks for queue order; the scratch array of the allocation-free entry
+    // point holds no room for it
+    const bool ordered = !sdev && c->force == CTRAJ_FORCE_WSPLINE_HEX && !(R->flags & CTRAJ_RUN_QUEUE_ORDER);
+    LensFn kl = lens_kernel(c->force, c->integ, ordered);
     if (!kl) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
     const bool out = wants_outputs(R);
     // PRE: walk to the lens and through its entrance; each wavefront queues the molecules that enter
@@ -1381,6 +1544,20 @@ static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *sdev) {
     const int64_t threads = (int64_t)gpre * blk;
     P.n_pre_waves = gpre * (blk / 64);
     P.lens_e = lens_e;
+    if (ordered) {
+      const size_t cap = (size_t)threads * (size_t)((slice + threads - 1) / threads);
+      const size_t b_bkt = (cap + 255) & ~(size_t)255, b_perm = cap * 4, b_tot = 2048;
+      const size_t b_hist = (size_t)gpre * ORD_BUCKETS * 4;
+      if ((rc = ensure_buf(c, c->ord, b_bkt + b_perm + b_tot + b_hist)) != CTRAJ_OK) return rc;
+      char *o = (char *)c->ord.p;
+      P.ord_tot = (uint32_t *)o;
+      P.ord_hist = (uint32_t *)(o + b_tot);
+      P.perm = (uint32_t *)(o + b_tot + b_hist);
+      P.bkt = (uint8_t *)(o + b_tot + b_hist + b_perm);
+      const double ks = c->h_elems[lens_e].p[4];
+      for (int st = 0; st < c->ws_states && st < CTRAJ_MAX_STATES; ++st) P.ord_k2[st] = (float)(ks * c->ws_end[st]);
+      P.ord_ref = c->integ == CTRAJ_INTEG_REF;
+    }
     const int refill = R->refill > 0 ? std::min(R->refill, 64) : 8;
     const int grid = grid_for(c, (const void *)kl, block, lds, (int64_t)1 << 40, 0);
     const StepK K = step_k(c->h_elems[lens_e].p[2], c->h_elems[lens_e].p[4], c->phys.g);
@@ -1393,6 +1570,11 @@ static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *sdev) {
       if (pev && s0 == 0) HIPCHK(c, hipEventRecord(pev[0], c->stream));
       hipLaunchKernelGGL(kpre, dim3(gpre), dim3(blk), lds_aoi, c->stream, B, kt, S, 0, P);
       HIPCHK(c, hipGetLastError());
+      if (ordered) {
+        hipLaunchKernelGGL(k_ord_scan, dim3(ORD_BUCKETS), dim3(256), 0, c->stream, P.ord_hist, gpre, P.ord_tot);
+        hipLaunchKernelGGL(k_ord_scatter, dim3(gpre), dim3(256), 0, c->stream, P);
+        HIPCHK(c, hipGetLastError());
+      }
       if (pev && one_slice) HIPCHK(c, hipEventRecord(pev[1], c->stream));
       hipLaunchKernelGGL(kl, dim3(grid), dim3(block), lds, c->stream, B, kt, P, refill, K,
                          (unsigned long long *)R->work);
@@ -1535,6 +1717,7 @@ int32_t ctraj_ctx_destroy(ctraj_ctx *c) {
   if (c->sort_tmp.p) (void)hipFree(c->sort_tmp.p);
   if (c->pipe.p) (void)hipFree(c->pipe.p);
   if (c->pipe_cnt.p) (void)hipFree(c->pipe_cnt.p);
+  if (c->ord.p) (void)hipFree(c->ord.p);
   if (c->sort_alt.p) (void)hipFree(c->sort_alt.p);
   for (int i = 0; i < 64; ++i) {
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -1708,6 +1891,10 @@ int32_t ctraj_set_wspline(ctraj_ctx *c, int32_t n_states, int32_t n_seg, const d
   HIPCHK(c, hipMemcpy(c->d_ws, coef, cnt * sizeof(double), hipMemcpyHostToDevice));
   c->ws_states = n_states;
   c->ws_nseg = n_seg;
+  for (int st = 0; st < n_states; ++st) {
+    const double *last = coef + ((size_t)st * n_seg + n_seg - 1) * 4;
+    c->ws_end[st] = (last[0] + last[1]) + last[2];
+  }
   return CTRAJ_OK;
 }
 

@@ -30,6 +30,7 @@ struct ctraj_ctx {
   // W' spline tables [state][seg][4]
   double *d_ws = nullptr;
   int ws_nseg = 0, ws_states = 0;
+  double ws_end[CTRAJ_MAX_STATES] = {};  // W' at each state table's end (A + B + C of its last segment)
   // r tables, concatenated [knot][4]
   double *d_rt = nullptr;
   int rt_total = 0;
@@ -52,6 +53,8 @@ struct ctraj_ctx {
   ctraj_devbuf sort_tmp, sort_alt;
   // propagation in separate kernels: the lens-entry queue, its claim and count words
   ctraj_devbuf pipe, pipe_cnt;
+  // lens kernel claim order: per-entry lifetime buckets, per-workgroup bucket counts, slot permutation
+  ctraj_devbuf ord;
   // this device's scratch array of ctraj_propagate_noalloc (device address), or nullptr
   char *scratch = nullptr;
 };

@@ -26,7 +26,7 @@ INTEG_REF, INTEG_RK4 = 0, 1
 
 ST_OOT, ST_CAPPED = 1, 2
 WORK_FACES, WORK_STEPS, WORK_OOT, WORK_CAPPED, WORK_FACES_POST, WORK_N = 0, 1, 2, 3, 4, 8
-RUN_STATUS_OR, RUN_SIMPLE, RUN_LANES, RUN_PHASE_TIMES = 1, 2, 4, 8
+RUN_STATUS_OR, RUN_SIMPLE, RUN_LANES, RUN_PHASE_TIMES, RUN_QUEUE_ORDER = 1, 2, 4, 8, 16
 REC_ROWS, REC_ZGRID, REC_ZGRID_MOL = 1, 2, 3
 H2D, D2H, D2D = 1, 2, 3
 

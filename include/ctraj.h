@@ -138,7 +138,9 @@ int32_t ctraj_set_wspline(ctraj_ctx *ctx, int32_t n_states, int32_t n_seg, const
  *      and through the lens entrance that queues the molecules entering the lens (a context-owned
  *      queue of 64 bytes per molecule of the slice), a persistent lane-refill kernel that integrates
  *      the lens and writes each molecule's state where the lens left it back into its queue entry,
- *      and a streaming walk of the elements after the lens that records the lens molecules.
+ *      and a streaming walk of the elements after the lens that records the lens molecules.  For a
+ *      hexapole lens two small kernels between the first two list the queue longest predicted lens
+ *      time first (5 more bytes per molecule), the order the lens kernel takes it in.
  *      CTRAJ_RUN_LANES, a WSPLINE_HEX lens whose force scale p[4] lies outside [2^-500, 2^500] in
  *      magnitude (V0 = 0, say: its RK4 step cannot fold ks into the step constants), or a
  *      reference-mode lens with stored accelerations requested use the single-kernel lane-refill
@@ -148,6 +150,8 @@ int32_t ctraj_set_wspline(ctraj_ctx *ctx, int32_t n_states, int32_t n_seg, const
 #define CTRAJ_RUN_SIMPLE 2    /* one molecule per thread start to finish (no lane refill) */
 #define CTRAJ_RUN_LANES 4     /* one lens: the single-kernel lane-refill walker instead of the split kernels */
 #define CTRAJ_RUN_PHASE_TIMES 8 /* time the launch's kernels with HIP events on the stream (ctraj_phase_times) */
+#define CTRAJ_RUN_QUEUE_ORDER 16 /* split path: the lens kernel takes the queue in the order the walk filled it
+                                  * instead of longest predicted lens time first (results are the same) */
 typedef struct ctraj_run {
   int64_t n;             /* molecules */
   int64_t index_base;    /* global index of molecule 0 (for first_index / aoi_index in chunked runs) */

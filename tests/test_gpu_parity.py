@@ -218,8 +218,8 @@ def test_beamline_propagate_single_molecule_smoke_kat():
 
 
 def test_lane_refill_kernel_matches_simple_kernel():
-    """The three-phase pipeline and the single-kernel lane-refill walker only reorder work: every
-    refill threshold, both of them and the one-thread-per-molecule kernel give identical bits, in
+    """The split kernels (in either claim order) and the single-kernel lane-refill walker only reorder
+    work: every refill threshold, all of them and the one-thread-per-molecule kernel give identical bits, in
     configuration mode (one and four lens states) and in reference mode.  Molecules stopped inside
     the lens are kept too, so the lens phase's own output path is compared."""
     cases = ((configs.config1(), 41, None), (reference_example_beamline(), 42, None), (configs.config4(), 43, 4))
@@ -235,7 +235,10 @@ def test_lane_refill_kernel_matches_simple_kernel():
                              aoi_labels=keep)
         post_faces = set()
         ref_mode = eng.cb.integrator == N.INTEG_REF
+        # flags 0: the lens kernel takes hexapole lens molecules longest-lived first; RUN_QUEUE_ORDER in
+        # the order the walk queued them
         runs = ((0, 1, True), (0, 24, True), (0, 64, True), (0, 1, False), (0, 12, False), (0, 64, False),
+                (N.RUN_QUEUE_ORDER, 1, True), (N.RUN_QUEUE_ORDER, 8, False),
                 (N.RUN_LANES, 1, True), (N.RUN_LANES, 16, True), (N.RUN_LANES, 64, True))
         for flags, refill, accel in runs:
             res = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], state=st, refill=refill, want_accel=accel,
@@ -252,7 +255,7 @@ def test_lane_refill_kernel_matches_simple_kernel():
             np.testing.assert_array_equal(res["work"][:4], base["work"][:4])
             # the split kernels count the faces after the lens apart, the same at every threshold; a
             # reference-mode lens asked for stored accelerations runs the lane walker
-            if flags == 0 and not (ref_mode and accel):
+            if flags in (0, N.RUN_QUEUE_ORDER) and not (ref_mode and accel):
                 post_faces.add(int(res["work"][N.WORK_FACES_POST]))
             else:
                 assert int(res["work"][N.WORK_FACES_POST]) == 0
