@@ -59,6 +59,10 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint measurement")
     ap.add_argument("--adjoint-only", action="store_true", help="only the forward+adjoint measurement")
     ap.add_argument("--adjoint-n", type=int, default=1_000_000)
+    ap.add_argument("--scan-n", type=int, default=100_000, help="molecules per parameter set of the gradient scan")
+    ap.add_argument("--scan-sets", type=int, default=64, help="parameter sets of the gradient scan")
+    ap.add_argument("--no-extras", action="store_true", help="skip the ballistic and reference-mode measurements")
+    ap.add_argument("--extras-only", action="store_true", help="only the ballistic and reference-mode measurements")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scan measurement")
     ap.add_argument("--batch-only", action="store_true", help="only the batched-scan measurement")
     ap.add_argument("--no-stream", action="store_true", help="skip the streamed four-state run")
@@ -85,18 +89,36 @@ def copy_calibration():
     print(json.dumps({"copy_bytes": n * 8, "reps": 3}))
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_cores() -> tuple:
+    """(cores used, cores in this process's affinity set): every core of the affinity set, within the
+    thread budget the host gives a job (OMP_NUM_THREADS) where one is set."""
+    aff = len(os.sched_getaffinity(0))
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(aff, cap) if cap > 0 else aff), aff
+
+
 def cpu_baseline(bl, X, V, seconds: float):
     """pyref.run_simulation_parallel over host cores on the first `sample` molecules."""
     from oracle import pyref
 
-    cores = min(16, len(os.sched_getaffinity(0)))
+    cores, aff = cpu_cores()
     # time one molecule chunk single-threaded to size the sample
     t0 = time.perf_counter()
     m0 = 200
     pyref.propagate_block(bl.elements, bl.g, X[:, :m0], V[:, :m0])
     per_mol = (time.perf_counter() - t0) / m0
     loops = 100 * cores
-    per_chunk = max(1, int(seconds * cores / per_mol / loops))
+    per_chunk = max(1, min(int(seconds * cores / per_mol / loops), X.shape[1] // loops))
     sample = per_chunk * loops
     t0 = time.perf_counter()
     counter, kept = pyref.run_simulation_parallel(bl.elements, bl.g, X[:, :sample], V[:, :sample], ("Detected",),
@@ -105,8 +127,10 @@ def cpu_baseline(bl, X, V, seconds: float):
     assert sum(counter.values()) == sample
     return {"value": sample / wall, "unit": "molecules/s", "cores": cores, "kind": "port",
             "sample": f"first {sample} molecules of the same workload ({loops} joblib chunks of {per_chunk}), "
-                      f"per-molecule numpy restatement of run_simulation(n_jobs={cores}); wall {wall:.1f} s",
-            "counter": counter}
+                      f"per-molecule numpy restatement of run_simulation(n_jobs={cores}); wall {wall:.1f} s; "
+                      f"{cores} of the {aff} cores in the affinity set ({cpu_model()}); the restatement skips the "
+                      f"reference's per-molecule trajectory preallocation, so it is faster than the reference",
+            "cpu_model": cpu_model(), "cores_in_affinity": aff, "counter": counter}
 
 
 def timed_ms(ctx, fn, reps: int, warmup: int = 2) -> float:
@@ -122,17 +146,39 @@ def timed_ms(ctx, fn, reps: int, warmup: int = 2) -> float:
     return ctx.event_elapsed_ms(0, 1) / reps
 
 
-def bench_adjoint(ctx, n: int, reps: int = 5) -> dict:
+def smooth_flop(theta: np.ndarray, eps: np.ndarray, bl, K: int, max_steps: int, grad: bool) -> float:
+    """FLOPs of one k_smooth launch for one parameter set, from each molecule's lens step count: the
+    forward sweep, and with grad the reverse sweep's recomputation (per checkpoint segment of length
+    Ls: the state every 8 steps, 8 (ceil(Ls/8) - 1) steps, then each 8-step window again, Ls - ceil(Ls/8)
+    steps), the adjoint of every step and the faces' reverse."""
+    from trajectories import configs
+
+    L, dt = bl.elements[1].L, bl.elements[1].dt
+    vz = theta[-1] + configs.VZ_SIGMA * eps
+    nfull = np.clip(np.floor(L / (vz * dt)), 0, max_steps)
+    S = np.where(vz > 0, nfull + 1, 0).astype(np.int64)
+    faces = 4 * int((vz > 0).sum())  # three aperture faces and the lens entrance
+    f = FLOP_PER_RK4_STEP * float(S.sum()) + FLOP_SMOOTH_FACE * faces
+    if not grad:
+        return f
+    full, last = S // K, S % K
+    def recompute(Ls):
+        nsub = -(-Ls // 8)
+        return np.where(Ls > 0, 8 * (nsub - 1) + (Ls - nsub), 0)
+    rec = full * recompute(np.full_like(S, K)) + recompute(last)
+    return f + FLOP_PER_RK4_STEP * float(rec.sum()) + FLOP_ADJ_STEP * float(S.sum()) + FLOP_SMOOTH_FACE_REV * faces
+
+
+def bench_adjoint(ctx, n: int, K: int, reps: int = 5) -> dict:
     """Forward+adjoint of the smoothed detected flux (config-1 geometry, 10^6 molecules, theta =
-    (r1, V0, z0, r2, r3, mu_vz), sigmoid width 50 um, checkpoints every 8 steps) against the
+    (r1, V0, z0, r2, r3, mu_vz), sigmoid width 50 um, checkpoints every K steps) against the
     2 n_params objective-only evaluations that central differences need."""
     from trajectories import configs
     from trajectories.smooth import SmoothFlux
 
     bl = configs.config1()
     src = configs.config3_source(n, 3)
-    prob = SmoothFlux(bl, *src, sigma_vz=configs.VZ_SIGMA, softness=configs.CONFIG3_SOFTNESS,
-                      ckpt_every=configs.CONFIG3_CKPT, ctx=ctx)
+    prob = SmoothFlux(bl, *src, sigma_vz=configs.VZ_SIGMA, softness=configs.CONFIG3_SOFTNESS, ckpt_every=K, ctx=ctx)
     theta = configs.config3_theta()
     npar = theta.size
     ms = prob.steps_needed(theta)
@@ -163,29 +209,89 @@ def bench_adjoint(ctx, n: int, reps: int = 5) -> dict:
     gs, fs = grad * theta, fd * theta
     big = np.abs(fs) > 1e-8 * np.max(np.abs(fs))
     fd_err = float(np.max(np.abs(gs[big] - fs[big]) / np.abs(fs[big])))
-
-    # FLOPs of one forward+adjoint launch, from each molecule's step count
-    L, dt = bl.elements[1].L, bl.elements[1].dt
-    vz = theta[-1] + configs.VZ_SIGMA * src[4]
-    nfull = np.clip(np.floor(L / (vz * dt)), 0, ms)
-    S = nfull + 1
-    segs = np.ceil(S / configs.CONFIG3_CKPT)
-    faces = 4 * n  # three aperture faces and the lens entrance
-    f_fwd = FLOP_PER_RK4_STEP * S.sum() + FLOP_SMOOTH_FACE * faces
-    f_adj = f_fwd + FLOP_PER_RK4_STEP * (S - segs).sum() + FLOP_ADJ_STEP * S.sum() + FLOP_SMOOTH_FACE_REV * faces
+    f_adj = smooth_flop(theta, src[4], bl, K, ms, True)
+    f_fwd = smooth_flop(theta, src[4], bl, K, ms, False)
     achieved = f_adj / (t_adj * 1e-3) / 1e12
     return {
-        "workload": "smoothed detected flux over the ES-lens beamline, theta=(r1,V0,z0,r2,r3,mu_vz), sigmoid 50 um, "
-                    "RK4 dt=10 us, checkpoints every 8 steps, source RandomState(3)",
-        "molecules": n, "n_params": npar,
+        "workload": f"smoothed detected flux over the ES-lens beamline, theta=(r1,V0,z0,r2,r3,mu_vz), sigmoid 50 um, "
+                    f"RK4 dt=10 us, checkpoints every {K} steps, source RandomState(3)",
+        "molecules": n, "n_params": npar, "ckpt_every": K,
         "fwd_adjoint_ms": t_adj, "fwd_adjoint_molecules_per_s": n / (t_adj * 1e-3),
-        "fwd_only_ms": t_fwd, "fwd_only_molecules_per_s": n / (t_fwd * 1e-3),
+        "fwd_only_ms": t_fwd, "fwd_only_molecules_per_s": n / (t_fwd * 1e-3), "adjoint_over_fwd": t_adj / t_fwd,
+        "fwd_only_tflops": f_fwd / (t_fwd * 1e-3) / 1e12,
         "fd_batched_ms": t_fd_b, "fd_separate_ms": t_fd_s, "fd_evaluations": 2 * npar,
         "speedup_vs_fd_separate": t_fd_s / t_adj, "speedup_vs_fd_batched": t_fd_b / t_adj,
         "grad_vs_fd_max_rel_err": fd_err, "J": float(J1.to_host()[0]),
         "roofline": {"bound": "mfma", "engine": ENGINE, "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "kernel": "k_smooth<true>", "flop_per_launch": float(f_adj),
-                     "rk4_steps_forward": float(S.sum())},
+                     "frac": achieved / PEAK_FP64, "kernel": "k_smooth<true>", "flop_per_launch": f_adj,
+                     "flop_rule": "forward 102/RK4 step + 23/face; reverse: recomputed steps 102 each (two-level "
+                                  "checkpoints), 204 per step adjoint, 34 per face"},
+    }
+
+
+def bench_adjoint_scan(ctx, n: int, n_sets: int = 64, K: int = 64, reps: int = 3) -> dict:
+    """Batched forward+adjoint scan: n_sets parameter sets (V0, lens z0 and detector radius from the
+    batched scan's RandomState(2) draws; r1, r2, mu_vz at their config-3 values) over the same n
+    molecules, J and the full gradient of every set in one launch, against the central differences of
+    every set (12 objective evaluations per set) as one batched launch and as one launch per set."""
+    from trajectories import configs
+    from trajectories.smooth import SmoothFlux
+
+    bl = configs.config1()
+    src = configs.config3_source(n, 3)
+    prob = SmoothFlux(bl, *src, sigma_vz=configs.VZ_SIGMA, softness=configs.CONFIG3_SOFTNESS, ckpt_every=K, ctx=ctx)
+    V0, z0, r3, _ = configs.config2_params(n_sets, 2)
+    base = configs.config3_theta()
+    Th = np.repeat(base[None], n_sets, axis=0)
+    Th[:, 1], Th[:, 2], Th[:, 4] = V0, z0, r3
+    npar = base.size
+    ms = prob.steps_needed(Th)
+    dTh = ctx.to_device(Th.ravel())
+    J = ctx.alloc(n_sets)
+    G = ctx.alloc(n_sets * npar)
+    hstep = 1e-6 * np.abs(Th)
+    Pfd = np.repeat(Th, 2 * npar, axis=0).reshape(n_sets, 2 * npar, npar)
+    for j in range(npar):
+        Pfd[:, 2 * j, j] += hstep[:, j]
+        Pfd[:, 2 * j + 1, j] -= hstep[:, j]
+    dPfd = ctx.to_device(Pfd.ravel())
+    Jfd = ctx.alloc(n_sets * 2 * npar)
+    t_adj = timed_ms(ctx, lambda: prob.launch(dTh.ptr, n_sets, J.ptr, G.ptr, ms), reps, warmup=1)
+    t_fd = timed_ms(ctx, lambda: prob.launch(dPfd.ptr, n_sets * 2 * npar, Jfd.ptr, None, ms), reps, warmup=1)
+
+    def fd_per_set():
+        for c in range(n_sets):
+            prob.launch(dPfd.ptr + 8 * c * 2 * npar * npar, 2 * npar, Jfd.ptr + 8 * c * 2 * npar, None, ms)
+
+    t_fd_sets = timed_ms(ctx, fd_per_set, 1, warmup=1)
+    grad = G.to_host().reshape(n_sets, npar)
+    Jf = Jfd.to_host().reshape(n_sets, 2 * npar)
+    tp = Pfd[:, 0::2, :][:, np.arange(npar), np.arange(npar)]
+    tm = Pfd[:, 1::2, :][:, np.arange(npar), np.arange(npar)]
+    fd = (Jf[:, 0::2] - Jf[:, 1::2]) / (tp - tm)
+    # gradients scaled by theta (d J / d log theta); each set's error relative to its largest component
+    gs, fs = grad * Th, fd * Th
+    scale = np.max(np.abs(fs), axis=1, keepdims=True)
+    rel = np.abs(gs - fs) / scale
+    big = np.abs(fs) > 1e-3 * scale
+    rel_comp = np.where(big, np.abs(gs - fs) / np.where(big, np.abs(fs), 1.0), 0.0)
+    f_adj = sum(smooth_flop(Th[c], src[4], bl, K, ms, True) for c in range(n_sets))
+    achieved = f_adj / (t_adj * 1e-3) / 1e12
+    return {
+        "workload": f"{n_sets} parameter sets (V0~U[10,40] kV, lens z0~U[0.7,0.9] m, detector r~U[2,5] mm from "
+                    f"RandomState(2)) x {n} molecules of the config-3 source, smoothed detected flux and its "
+                    f"gradient in theta=(r1,V0,z0,r2,r3,mu_vz), checkpoints every {K} steps",
+        "parameter_sets": n_sets, "molecules_per_set": n, "ckpt_every": K,
+        "fwd_adjoint_ms": t_adj, "fd_batched_ms": t_fd, "fd_per_set_launches_ms": t_fd_sets,
+        "fd_evaluations": n_sets * 2 * npar,
+        "speedup_vs_fd_batched": t_fd / t_adj, "speedup_vs_fd_per_set": t_fd_sets / t_adj,
+        "grad_vs_fd_max_rel_err": float(rel.max()), "sets_within_1e-5": int((rel.max(axis=1) <= 1e-5).sum()),
+        "grad_vs_fd_rel_err_rule": "per set max_j |g_j - fd_j| theta_j / max_j |fd_j theta_j| (central differences, "
+                                   "step 1e-6 relative)",
+        "grad_vs_fd_max_componentwise_err": float(rel_comp.max()),
+        "componentwise_rule": "|g_j - fd_j| / |fd_j| over components above 1e-3 of the set's largest",
+        "roofline": {"bound": "mfma", "engine": ENGINE, "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64, "kernel": "k_smooth<true>", "flop_per_launch": f_adj},
     }
 
 
@@ -244,14 +350,14 @@ def bench_batch(ctx, reps: int = 5, tune_reps: int = 2) -> dict:
     }
 
 
-def bench_stream(ctx, n: int) -> dict:
-    """The streamed run: n molecules of the ES-lens beamline in 2^25-molecule chunks drawn on the
+def bench_stream(ctx, n: int, chunk: int = 1 << 22) -> dict:
+    """The streamed run: n molecules of the ES-lens beamline in `chunk`-molecule chunks drawn on the
     device, four lens states assigned uniformly, per-state tallies, and the 1-mm z-grid samples of
     every detected molecule written to one compacted device buffer."""
     from trajectories import configs
     from trajectories.stream import StreamedRun
 
-    run = StreamedRun(configs.config4(), n, chunk=1 << 25, seed=4, ctx=ctx)
+    run = StreamedRun(configs.config4(), n, chunk=chunk, seed=4, ctx=ctx)
     # one untimed run first: kernel code objects load and the context's sort buffers grow on first use
     warm = run.run(trajectories=True)
     del warm
@@ -264,8 +370,9 @@ def bench_stream(ctx, n: int) -> dict:
     steps = float(res.work[1])
     return {
         "workload": "ES-lens beamline, 4 Stark states (signs -/+, dipole x0.5, x0.8) assigned uniformly, "
-                    "2^25-molecule chunks drawn on the device (seed 4), detected trajectories every 1 mm in z",
-        "molecules": n, "chunks": (n + (1 << 25) - 1) >> 25,
+                    f"2^{chunk.bit_length() - 1}-molecule chunks drawn on the device (seed 4), detected trajectories "
+                    "every 1 mm in z",
+        "molecules": n, "chunk": chunk, "chunks": (n + chunk - 1) // chunk,
         "pass1_device_ms": res.timings["pass1_device_ms"], "pass1_wall_s": res.timings["pass1_wall_s"],
         "pass1_molecules_per_s": n / (res.timings["pass1_device_ms"] * 1e-3),
         "pass2_device_ms": p2, "pass2_alloc_s": res.timings["pass2_alloc_s"], "detected": res.n_detected, "detected_per_state": det.tolist(),
@@ -273,6 +380,124 @@ def bench_stream(ctx, n: int) -> dict:
         "trajectory_write_GBps": tb / (p2 * 1e-3) / 1e9 if p2 > 0 else None,
         "total_wall_s": wall, "rk4_steps_pass1": steps,
     }
+
+
+def bench_ballistic(ctx, n: int = 100_000_000, reps: int = 10) -> dict:
+    """The ballistic beamline (no lens) over n molecules drawn on the device (Philox source, seed 0):
+    per-element tallies and the compacted detected phase space; HBM roofline on the algorithmic bytes
+    (48 B of inputs per molecule read once, 64 B per kept molecule written once)."""
+    from trajectories import _native as N
+    from trajectories import configs
+    from trajectories.engine import Engine
+
+    eng = Engine(configs.config0(), ctx)
+    xs = eng.draw_gaussian(n, configs.SOURCE_MEAN, configs.SOURCE_SIGMA, 0)
+    inp = dict(zip(("x", "y", "z", "vx", "vy", "vz"), xs))
+    tally = ctx.alloc(eng.cb.n_labels, np.int64).fill(0)
+    det = eng.cb.label_detected
+    cap = n // 10 + 1024
+    cnt = ctx.alloc(1, np.int64).fill(0)
+    idx = ctx.alloc(cap, np.int64)
+    out = {k: ctx.alloc(cap) for k in ("x", "y", "z", "vx", "vy", "vz", "t")}
+
+    def step():
+        cnt.fill(0)
+        eng.launch(inp, tally=tally, aoi_mask=1 << det, aoi_cap=cap, aoi_count=cnt, aoi_index=idx, aoi_out=out, n=n)
+
+    for _ in range(2):
+        step()
+    ctx.sync()
+    tally.fill(0)
+    ctx.phase_times()
+    for _ in range(reps):
+        cnt.fill(0)
+        eng.launch(inp, tally=tally, aoi_mask=1 << det, aoi_cap=cap, aoi_count=cnt, aoi_index=idx, aoi_out=out, n=n,
+                   flags=N.RUN_PHASE_TIMES)
+    w, ln, po = ctx.phase_times()
+    ms = float(np.mean(w + ln + po))  # one kernel: its whole time is reported as the lens phase
+    tal = tally.to_host() // reps
+    assert int(tal.sum()) == n
+    n_det = int(cnt.to_host()[0])
+    nbytes = 48.0 * n + 64.0 * n_det
+    gbps = nbytes / (ms * 1e-3) / 1e9
+    return {
+        "workload": "ballistic beamline: apertures r=6/4/2.5 mm @0.3/0.8/1.2 m, detector r=5 mm @2.0 m; g=9.81; "
+                    "source drawn on the device (Philox4x32-10, seed 0)",
+        "molecules": n, "ms": ms, "molecules_per_s": n / (ms * 1e-3), "detected": n_det,
+        "counts": dict(zip(eng.labels, tal.tolist())),
+        "roofline": {"bound": "hbm", "achieved": gbps, "peak": PEAK_HBM, "unit": "GB/s", "frac": gbps / PEAK_HBM,
+                     "kernel": "k_propagate<circular apertures, no lens>", "algorithmic_bytes_per_launch": nbytes,
+                     "timing": "HIP events recorded by ctraj_propagate around its kernel, mean over %d launches" % reps},
+    }
+
+
+FLOP_PER_REF_STEP = 142  # reference-mode lens step: 4 a(r) evaluations (radius 4, table 5, components 5: 14
+#                          each), the reference's stage arithmetic on x, y, z and velocities, time, bore radius
+
+
+def bench_reference_mode(ctx, n: int = 1_000_000, reps: int = 10, cpu_seconds: float = 10.0, no_cpu: bool = False):
+    """The reference's example beamline (examples/lens_simulation_beamline.py:18-72) with its own lens
+    model: integrator 'ref' (N = rint(L/dz) steps of dt = dz/vz), linear-field a(r) table; the
+    reference's default CeNTREX source replayed with numpy seed 5; per-element tallies and the detected
+    molecules.  Beside it the per-molecule CPU restatement of run_simulation on host cores."""
+    from trajectories import _native as N
+    from trajectories import configs
+    from trajectories.distributions import CeNTREXPositionDistribution, CeNTREXVelocityDistribution
+    from trajectories.engine import Engine
+
+    bl = configs.reference_example()
+    np.random.seed(5)
+    vd, xd = CeNTREXVelocityDistribution(), CeNTREXPositionDistribution()
+    loops, m = 100, n // 100
+    V = np.concatenate([np.asarray(vd.draw(m)).reshape(3, m) for _ in range(loops)], axis=1)
+    X = np.concatenate([np.asarray(xd.draw(m)).reshape(3, m) for _ in range(loops)], axis=1)
+    eng = Engine(bl, ctx)
+    inp = {k: ctx.to_device(a) for k, a in zip(("x", "y", "z", "vx", "vy", "vz"), (X[0], X[1], X[2], V[0], V[1], V[2]))}
+    tally = ctx.alloc(eng.cb.n_labels, np.int64).fill(0)
+    work = ctx.alloc(N.WORK_N, np.uint64).fill(0)
+    det = eng.cb.label_detected
+    cap = n // 10 + 1024
+    cnt = ctx.alloc(1, np.int64).fill(0)
+    idx = ctx.alloc(cap, np.int64)
+    out = {k: ctx.alloc(cap) for k in ("x", "y", "z", "vx", "vy", "vz", "t")}
+    kw = dict(tally=tally, work=work, aoi_mask=1 << det, aoi_cap=cap, aoi_count=cnt, aoi_index=idx, aoi_out=out, n=n)
+    for _ in range(2):
+        cnt.fill(0)
+        eng.launch(inp, **kw)
+    ctx.sync()
+    tally.fill(0)
+    work.fill(0)
+    ctx.phase_times()
+    for _ in range(reps):
+        cnt.fill(0)
+        eng.launch(inp, flags=N.RUN_PHASE_TIMES, **kw)
+    w, ln, po = ctx.phase_times()
+    tot = w + ln + po
+    ms = float(np.mean(tot))
+    tal = tally.to_host() // reps
+    wk = work.to_host() // reps
+    assert int(tal.sum()) == n
+    n_oot = int(wk[N.WORK_OOT])
+    faces, steps_ = float(wk[N.WORK_FACES]), float(wk[N.WORK_STEPS])
+    flop = FLOP_PER_FACE * faces + FLOP_PER_REF_STEP * steps_
+    lens_ms = float(np.mean(ln))
+    res = {
+        "workload": "reference example beamline (4K/40K shields, BB exit, ES lens L=0.6 m in the reference's model: "
+                    "integrator 'ref', linear-field a(r) table from the two-level Stark form, field plates, DR "
+                    "aperture); CeNTREX default source, numpy seed 5",
+        "molecules": n, "ms": ms, "molecules_per_s": n / (ms * 1e-3), "counts": dict(zip(eng.labels, tal.tolist())),
+        "out_of_table_molecules": n_oot, "faces": faces, "ref_steps": steps_,
+        "roofline": {"bound": "mfma", "engine": ENGINE, "achieved": flop / (ms * 1e-3) / 1e12, "peak": PEAK_FP64,
+                     "unit": "TFLOP/s", "frac": flop / (ms * 1e-3) / 1e12 / PEAK_FP64, "flop_per_launch": flop,
+                     "flop_rule": "%d per element face, %d per reference lens step" % (FLOP_PER_FACE, FLOP_PER_REF_STEP),
+                     "kernels": "PRE + k_lens<RTABLE,REF> + POST" if lens_ms > 0 else "one kernel",
+                     "lens_kernel_ms": lens_ms},
+    }
+    if not no_cpu:
+        cb = cpu_baseline(bl, X, V, cpu_seconds)
+        res["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        res["gpu_over_cpu"] = res["molecules_per_s"] / cb["value"]
+    return res
 
 
 def main():
@@ -292,7 +517,13 @@ def main():
 
     ctx = get_context(0)
     if args.adjoint_only:
-        print(json.dumps(bench_adjoint(ctx, args.adjoint_n, max(1, args.steps // 4))), flush=True)
+        print(json.dumps({"K64": bench_adjoint(ctx, args.adjoint_n, 64, max(1, args.steps // 4)),
+                          "K8": bench_adjoint(ctx, args.adjoint_n, 8, max(1, args.steps // 4)),
+                          "scan": bench_adjoint_scan(ctx, args.scan_n, args.scan_sets)}), flush=True)
+        return
+    if args.extras_only:
+        print(json.dumps({"ballistic": bench_ballistic(ctx),
+                          "reference_mode": bench_reference_mode(ctx, no_cpu=args.no_cpu)}), flush=True)
         return
     if args.batch_only:
         print(json.dumps(bench_batch(ctx)), flush=True)
@@ -403,8 +634,10 @@ def main():
         # molecule entering the lens; the lens kernel rewrites each entry in place; POST reads the
         # entries back and writes each kept molecule (8 B index + 7 fields)
         n_enter = n - counts.get("Aperture 1", 0) - counts.get("Lens entrance", 0)
-        wbytes = 48.0 * n + 64.0 * n_enter
-        walk_phase = {"kernel": "k_propagate<PRE> (walk to and through the lens entrance, queue the lens molecules)",
+        # and the claim list: a bucket byte written and read back and a 4-byte slot per queued molecule
+        wbytes = 48.0 * n + 64.0 * n_enter + 6.0 * n_enter
+        walk_phase = {"kernel": "k_propagate<PRE> (walk to and through the lens entrance, queue the lens molecules) "
+                                "+ k_ord_scan + k_ord_scatter (claim list, longest predicted lens time first)",
                       "ms_avg": walk_avg, "bytes_per_launch": wbytes, "molecules_queued": n_enter,
                       "GBps": wbytes / (walk_avg * 1e-3) / 1e9, "hbm_frac": wbytes / (walk_avg * 1e-3) / 1e9 / PEAK_HBM,
                       "faces": faces - faces_lens}
@@ -461,14 +694,21 @@ def main():
         line["walk_phase"] = walk_phase
         line["post_phase"] = post_phase
     if args.config == "lens" and not args.no_adjoint:
-        line["fwd_adjoint"] = bench_adjoint(ctx, args.adjoint_n)
+        line["fwd_adjoint"] = bench_adjoint(ctx, args.adjoint_n, configs.CONFIG3_CKPT)
+        line["fwd_adjoint_K8"] = bench_adjoint(ctx, args.adjoint_n, 8)
+        line["fwd_adjoint_scan"] = bench_adjoint_scan(ctx, args.scan_n, args.scan_sets, configs.CONFIG3_CKPT)
     if args.config == "lens" and not args.no_batch:
         line["batch_scan"] = bench_batch(ctx)
     if args.config == "lens" and not args.no_stream:
-        line["streamed"] = bench_stream(ctx, args.stream_n)
+        line["streamed"] = bench_stream(ctx, args.stream_n, 1 << 22)
+        line["streamed_2e25"] = bench_stream(ctx, args.stream_n, 1 << 25)
+    if args.config == "lens" and not args.no_extras:
+        line["ballistic"] = bench_ballistic(ctx)
+        line["reference_mode"] = bench_reference_mode(ctx, cpu_seconds=args.cpu_seconds * 2 / 3, no_cpu=args.no_cpu)
     if not args.no_cpu:
         cb = cpu_baseline(bl, X, V, args.cpu_seconds)
-        line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample", "cpu_model",
+                                                   "cores_in_affinity")}
         line["gpu_over_cpu"] = value / cb["value"]
     print(json.dumps(line), flush=True)
 

@@ -28,7 +28,7 @@ using namespace ctraj;
 namespace {
 
 constexpr int kMaxBins = 4096;  // configurations x labels tallied in LDS
-constexpr int64_t kPipeMax = (int64_t)1 << 27;  // larger batches run the single-kernel walker (64 B of queue each)
+constexpr int64_t kPipeMax = (int64_t)1 << 27;  // molecules per slice of the two-phase path (64 B of queue each)
 
 enum : int { LN_IDLE = 0, LN_LENS = 1, LN_EXIT = 2, LN_FIN = 3, LN_DONE = 4 };
 
@@ -41,7 +41,7 @@ struct BCfg {
 
 // Lens constants of one configuration, staged in the two-phase lens kernel's LDS.
 struct BLensC {
-  double ks, cu, R2, z1, Hs, Ll, gk;  // gk = -g/ks (every two-phase lens folds ks into its step)
+  double ks, cu, R2, z1, Hs, Ll, gk;  // gk = -g/ks where the lens folds ks into its step (hex_folds), else 0
   int32_t label_inside, e0, ne, lens;  // element slice and the lens's index in it
 };
 
@@ -68,6 +68,7 @@ struct BPipe {
   unsigned long long *claim;    // [8][32]
   int64_t region;
   int n_pre_waves;
+  int64_t s0, s1;               // the slice of batch molecule indices [s0, s1) this PRE launch walks
 };
 
 struct BLane {
@@ -295,9 +296,9 @@ __global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
   if (threadIdx.x < 4 && s_work[threadIdx.x]) atomicAdd(&A.work[threadIdx.x], s_work[threadIdx.x]);
 }
 
-// Two-phase scan, phase 1: every molecule is drawn and walked to and through its configuration's
-// lens entrance (grid-stride, wave-uniform trip count); the molecules that enter a lens are queued
-// in the wavefront's region, the others are tallied.
+// Two-phase scan, phase 1: every molecule of the slice [s0, s1) is drawn and walked to and through
+// its configuration's lens entrance (grid-stride, wave-uniform trip count); the molecules that enter
+// a lens are queued in the wavefront's region, the others are tallied.
 template <int KM>
 __global__ void __launch_bounds__(256) k_batch_pre(BArgs A, BPipe P) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
@@ -316,9 +317,9 @@ __global__ void __launch_bounds__(256) k_batch_pre(BArgs A, BPipe P) {
   const int64_t gwave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   WalkCounters wc{0u, 0u};
   int64_t qn = 0;
-  for (int64_t wbase = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < A.total; wbase += stride) {
+  for (int64_t wbase = P.s0 + (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < P.s1; wbase += stride) {
     const int64_t gi = wbase + lane;
-    const bool active = gi < A.total;
+    const bool active = gi < P.s1;
     MState m;
     int lab = 0, c = 0;
     bool enq = false;
@@ -345,8 +346,10 @@ __global__ void __launch_bounds__(256) k_batch_pre(BArgs A, BPipe P) {
 // configuration's lens with lane refill (the configuration's lens constants staged in LDS, its field
 // constants and step in the lane's registers), walk the survivors through the elements after the
 // lens and tally.  The arithmetic per molecule is the one of k_batch_lanes.  UH: every configuration's
-// lens has the same dt, whose half and sixth come formed by the host (Hh, H6; otherwise unused).  KM:
-// the element kinds after the lenses (KM_CIRC keeps the walk's code and registers out).
+// lens has the same dt, whose half and sixth come formed by the host (Hh, H6; otherwise unused), and
+// folds its force scale; otherwise each step chooses the folded or the plain RK4 step for its
+// molecule's lens (lens_step).  KM: the element kinds after the lenses (KM_CIRC keeps the walk's code
+// and registers out).
 template <bool UH, int KM>
 __global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT, double Hh, double H6) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
@@ -459,7 +462,7 @@ __global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT, 
     if (ph == LN_LENS) {
       const Force<CTRAJ_FORCE_WSPLINE_HEX> F{ks, cu, g, gk};
       const bool last = (i == nfull);
-      if (!last) {  // every lens here folds (the host sends batches with one that does not to the fused kernel)
+      if (!last) {  // with UH every lens folds
         if constexpr (UH)
           rk4_step_hex_k(wsb, T.umax, cu, gk, Hs, Hh, H6, ks * Hh, ks * Hs, ks * H6, m.x, m.y, m.vx, m.vy, r2);
         else
@@ -513,8 +516,9 @@ struct ctraj_batch {
   void *d_pipe = nullptr, *d_pipe_cnt = nullptr;
   int pre_grid = 0, n_pre_waves = 0;
   int64_t region = 0;
-  bool uniform_dt = false;  // every lens has the same dt
+  bool uniform_dt = false;  // every lens has the same dt and folds its force scale (hex_folds)
   double dt = 0.0;
+  int64_t slice = 0;        // molecules per slice of the two-phase path
   bool circ_only = false;   // only circular apertures (and lenses) in every configuration
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // captured launches by (block, chunk, grid, refill, fused)
@@ -593,13 +597,21 @@ static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refi
   hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<int64_t>((nt + 255) / 256, 1024)), dim3(256), 0, c->stream,
                      (unsigned long long *)b->d_tally, nt);
   using PreK = void (*)(BArgs, BPipe);
-  hipLaunchKernelGGL((PreK)kp, dim3(b->pre_grid), dim3(256), lds_pre, c->stream, A, P);
-  HIPCHK(c, hipGetLastError());
+  using LensK = void (*)(BArgs, BPipe, KTab, double, double);
   // the RK4 half and sixth steps, rounded as rk4_step forms them
   const double Hh = 0.5 * b->dt, H6 = b->dt * (1.0 / 6.0);
-  using LensK = void (*)(BArgs, BPipe, KTab, double, double);
-  hipLaunchKernelGGL((LensK)kl, dim3(grid), dim3(block), lds_lens, c->stream, A, P, kt, Hh, H6);
-  HIPCHK(c, hipGetLastError());
+  // slices of at most b->slice molecules reuse the queue; the tallies and work counters accumulate
+  for (int64_t s0 = 0; s0 < b->total; s0 += b->slice) {
+    P.s0 = s0;
+    P.s1 = std::min(b->total, s0 + b->slice);
+    if (s0 > 0)
+      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, c->stream, (unsigned long long *)b->d_pipe_cnt,
+                         (int64_t)8 * 32);
+    hipLaunchKernelGGL((PreK)kp, dim3(b->pre_grid), dim3(256), lds_pre, c->stream, A, P);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL((LensK)kl, dim3(grid), dim3(block), lds_lens, c->stream, A, P, kt, Hh, H6);
+    HIPCHK(c, hipGetLastError());
+  }
   return CTRAJ_OK;
 }
 
@@ -709,15 +721,18 @@ int32_t ctraj_batch_create(ctraj_ctx *c, const ctraj_elem *elems, int32_t n_elem
       }
     }
   }
-  if (any_lens && total > 0 && total <= kPipeMax && all_fold) {
+  b->uniform_dt = b->uniform_dt && all_fold;
+  if (any_lens && total > 0) {
+    const int64_t slice = std::min(total, kPipeMax);
     const size_t lds_pre = (size_t)n_cfgs * sizeof(int64_t) + nt * sizeof(uint32_t);
     const void *kp = b->circ_only ? (const void *)k_batch_pre<KM_CIRC> : (const void *)k_batch_pre<KM_ALL>;
-    int gpre = grid_for(c, kp, 256, lds_pre, total, 0);
+    int gpre = grid_for(c, kp, 256, lds_pre, slice, 0);
     gpre = (gpre + 7) & ~7;  // whole groups of eight workgroups: one queue-region group per XCD
     const int64_t threads = (int64_t)gpre * 256;
     b->pre_grid = gpre;
     b->n_pre_waves = gpre * 4;
-    b->region = (total + threads - 1) / threads * 64;
+    b->slice = slice;
+    b->region = (slice + threads - 1) / threads * 64;
     const size_t qbytes = (size_t)b->n_pre_waves * (size_t)b->region * 64;
     if (hipMalloc((void **)&b->d_lensk, sizeof(BLensC) * n_cfgs) != hipSuccess ||
         hipMalloc(&b->d_pipe, qbytes) != hipSuccess ||
@@ -801,6 +816,8 @@ int32_t ctraj_batch_run(ctraj_batch *b, const ctraj_batch_tune *tune, int64_t *t
   if (elapsed_ms) HIPCHK(c, hipEventElapsedTime(elapsed_ms, b->ev0, b->ev1));
   return CTRAJ_OK;
 }
+
+int32_t ctraj_batch_split_path(const ctraj_batch *b) { return b && b->d_pipe ? 1 : 0; }
 
 int32_t ctraj_batch_destroy(ctraj_batch *b) {
   if (!b) return CTRAJ_OK;

@@ -205,6 +205,7 @@ SIGNATURES = [
     ("ctraj_batch_run", C.c_int32,
      [C.c_void_p, C.POINTER(BatchTune), C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     ("ctraj_batch_destroy", C.c_int32, [C.c_void_p]),
+    ("ctraj_batch_split_path", C.c_int32, [C.c_void_p]),
     ("ctraj_draw_uniform_u8", C.c_int32, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int64, C.c_int32, C.c_void_p]),
     ("ctraj_draw_gaussian_at", C.c_int32,
      [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),

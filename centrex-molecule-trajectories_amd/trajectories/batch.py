@@ -78,6 +78,11 @@ class BatchScan:
         self.total = int(counts.sum())
         self.tune = N.BatchTune(block=0, chunk=128, grid=0, refill=0, use_graph=1, fused=0)
 
+    @property
+    def split_path(self) -> bool:
+        """True when fused = 0 runs the two-phase path (walk and queue, then the lens kernel)."""
+        return bool(N.lib().ctraj_batch_split_path(self.handle))
+
     def run(self, tune: N.BatchTune | None = None, graph: bool | None = None):
         """One launch over every configuration.  Returns (tally [n_cfg][n_labels], work, device ms)."""
         t = N.BatchTune()

@@ -12,7 +12,7 @@ from __future__ import annotations
 import numpy as np
 
 from .beamline import Beamline
-from .beamline_elements import CircularAperture, ElectrostaticLens
+from .beamline_elements import CircularAperture, ElectrostaticLens, FieldPlates, RectangularAperture
 from .distributions import CeNTREXVelocityDistribution, GaussianPositionDistribution
 from .stark import TwoLevelStark, WSpline
 
@@ -131,7 +131,7 @@ def config2_params(n_cfg: int = 256, seed: int = 2):
 # order, sigmoid width 50 um, lens checkpoints every 8 steps (each checkpoint segment is then one
 # recompute window: every step is recomputed once; 64 steps took 7.5 ms against 6.3 ms).
 CONFIG3_SOFTNESS = 50e-6
-CONFIG3_CKPT = 8
+CONFIG3_CKPT = 64
 
 
 def config3_source(n: int, seed: int = 3):
@@ -153,3 +153,17 @@ def config3_source(n: int, seed: int = 3):
 
 def config3_theta() -> np.ndarray:
     return np.array([6e-3, 30e3, 0.8, 3e-3, 4e-3, VZ_MEAN])
+
+
+def reference_example(sign: float = +1.0) -> Beamline:
+    """The beamline of the reference's examples/lens_simulation_beamline.py:18-72 with its lens in the
+    reference's own model (integrator 'ref', linear field, a(r) table built as
+    electrostatic_lens.py:193-209 does), here from the two-level Stark form of the given sign."""
+    m = 0.0254
+    fourK = CircularAperture(z0=1.7 * m, L=0.25 * m, d=1 * m, name="4K shield")
+    fortyK = CircularAperture(z0=fourK.z1 + 1.25 * m, L=0.25 * m, d=1 * m, name="40K shield")
+    bb_exit = CircularAperture(z0=fortyK.z1 + 2.5 * m, L=0.75 * m, d=4 * m, name="BB exit")
+    lens = ElectrostaticLens(z0=bb_exit.z1 + 33 * m, L=0.6, name="ES lens", state=TwoLevelStark(sign=sign))
+    plates = FieldPlates(z0=2.43, L=3.0, w=0.02, name="Field plates")
+    dr = RectangularAperture(z0=plates.z1 + 39.9 * m, L=0.25 * m, name="DR aperture", w=0.018, h=0.03)
+    return Beamline([fourK, fortyK, bb_exit, lens, plates, dr])

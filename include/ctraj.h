@@ -287,11 +287,12 @@ typedef struct ctraj_batch_cfg {
   uint64_t seed;               /* device source stream of this configuration */
   double mean[6], sigma[6];    /* Gaussian source for x, y, z, vx, vy, vz */
 } ctraj_batch_cfg;
-/* A batch of at most 2^27 molecules with a lens runs in two phases, like ctraj_propagate: every
- * molecule is drawn and walked to its lens entrance, the molecules entering a lens are queued (64 B
- * each, batch-owned), then persistent wavefronts integrate the queue with lane refill.  Larger
- * batches, batches with a lens whose force scale does not fold (see ctraj_propagate), or fused = 1,
- * run one lane-refill kernel that draws, walks and integrates. */
+/* A batch with a lens runs in two phases per slice of at most 2^27 molecules, like ctraj_propagate:
+ * every molecule of the slice is drawn and walked to its lens entrance, the molecules entering a lens
+ * are queued (64 B each, a batch-owned queue reused by every slice), then persistent wavefronts
+ * integrate the queue with lane refill.  A lens whose force scale does not fold (see ctraj_propagate)
+ * takes the plain RK4 step for its own molecules only.  fused = 1, or a batch whose queue could not be
+ * allocated, runs one lane-refill kernel that draws, walks and integrates. */
 typedef struct ctraj_batch_tune {
   int32_t block;     /* threads per block of the lens kernel: a multiple of 64 up to 1024 (0 = 1024); with
                       * fused: 64, 128, 192 or 256 (0 = 256) */
@@ -309,6 +310,8 @@ int32_t ctraj_batch_create(ctraj_ctx *ctx, const ctraj_elem *elems, int32_t n_el
  * (may be NULL); elapsed_ms (host, may be NULL) is the device time of the launch measured with HIP events */
 int32_t ctraj_batch_run(ctraj_batch *b, const ctraj_batch_tune *tune, int64_t *tally_host, uint64_t *work_host,
                         float *elapsed_ms);
+/* 1 if the batch runs the two-phase path by default (fused = 0), 0 if it runs the single kernel */
+int32_t ctraj_batch_split_path(const ctraj_batch *b);
 int32_t ctraj_batch_destroy(ctraj_batch *b);
 
 /* ---- differentiable detected flux and its adjoint (no counterpart in the reference: the objective

@@ -120,17 +120,46 @@ def test_config2_scan_runs_and_matches_sampled_configurations(ctx):
 
 def test_batch_with_an_unfolded_lens_equals_per_configuration_propagation(ctx):
     """A configuration whose lens is at V0 = 0 (force scale 0, not folded into the step constants)
-    sends the whole batch to the single-kernel walker, which steps that lens with the plain RK4 step
-    and the others with the folded one: tallies still equal per-configuration propagation and the
+    keeps the batch on the two-phase path: that lens takes the plain RK4 step for its own molecules and
+    the others the folded one, in both paths; tallies equal per-configuration propagation and the
     oracle."""
     bls = [configs.config1(V0=30e3), configs.config1(V0=0.0), configs.config1(V0=18e3, r3=3e-3)]
     n = np.array([9_000, 6_000, 4_321])
     seeds = np.arange(len(bls), dtype=np.uint64) + np.uint64(777)
     scan = BatchScan(bls, n, seeds, configs.SOURCE_MEAN, configs.SOURCE_SIGMA, ctx=ctx)
-    tally, work, ms = scan.run(graph=False)
+    assert scan.split_path
+    T = scan.tune.__class__
     dev, orc = per_config_reference(ctx, scan, bls, n, seeds)
+    for fused in (0, 1):
+        tally, work, ms = scan.run(T(block=0, chunk=128, grid=0, refill=0, use_graph=0, fused=fused))
+        assert tally.sum() == n.sum()
+        for c in range(len(bls)):
+            got = as_dict(scan, tally, c)
+            assert got == {k: v for k, v in dev[c].items() if v}, (fused, c, got, dev[c])
+            assert got == {k: v for k, v in orc[c].items() if v}, (fused, c)
+
+
+def test_batch_beyond_one_slice_equals_per_configuration_propagation(ctx):
+    """A batch of more than 2^27 molecules runs the two-phase path in slices that reuse the queue
+    (a configuration straddles the slice boundary): per-configuration tallies equal ctraj_propagate's
+    over the same device draws, and the work counters equal the single kernel's."""
+    bls = [configs.config1(V0=30e3), configs.config1(V0=22e3, lens_z0=0.75, r3=3e-3)]
+    n = np.array([70_000_000, 70_000_001])
+    seeds = np.arange(len(bls), dtype=np.uint64) + np.uint64(4242)
+    scan = BatchScan(bls, n, seeds, configs.SOURCE_MEAN, configs.SOURCE_SIGMA, ctx=ctx)
+    assert scan.split_path and scan.total > (1 << 27)
+    T = scan.tune.__class__
+    tally, work, _ = scan.run(T(block=0, chunk=128, grid=0, refill=0, use_graph=0, fused=0))
     assert tally.sum() == n.sum()
-    for c in range(len(bls)):
-        got = as_dict(scan, tally, c)
-        assert got == {k: v for k, v in dev[c].items() if v}, (c, got, dev[c])
-        assert got == {k: v for k, v in orc[c].items() if v}, c
+    for c, bl in enumerate(bls):
+        eng = Engine(bl, ctx)
+        cols = eng.draw_gaussian(int(n[c]), configs.SOURCE_MEAN, configs.SOURCE_SIGMA, int(seeds[c]))
+        inp = dict(zip(("x", "y", "z", "vx", "vy", "vz"), cols))
+        tal = ctx.alloc(eng.cb.n_labels, np.int64).fill(0)
+        eng.launch(inp, tally=tal, n=int(n[c]))
+        want = dict(zip(eng.labels, tal.to_host().tolist()))
+        del cols, inp
+        assert as_dict(scan, tally, c) == {k: v for k, v in want.items() if v}, c
+    t1, w1, _ = scan.run(T(block=256, chunk=128, grid=0, refill=16, use_graph=0, fused=1))
+    np.testing.assert_array_equal(t1, tally)
+    np.testing.assert_array_equal(w1[:4], work[:4])

@@ -160,3 +160,41 @@ def test_a_run_of_several_slices_equals_the_lane_walker(ctx):
     np.testing.assert_array_equal(a["index"], b["index"])
     for k in FIELDS:
         np.testing.assert_array_equal(a["kept"][k], b["kept"][k])
+
+
+def test_a_billion_molecules_on_the_split_path_equal_the_lane_walker(ctx):
+    """10^9 molecules (the reference's large SPA runs, examples/SPA/SPA_distributions.py:99-102) drawn
+    on the device run the split kernels in sixteen slices inside one ctraj_propagate: the tallies, the
+    work counters and every detected molecule's index and final phase space equal the single-kernel
+    lane walker's."""
+    from trajectories.engine import draw_gaussian
+
+    n = 1_000_000_000
+    eng = Engine(configs.config1(), ctx)
+    xs = draw_gaussian(ctx, n, configs.SOURCE_MEAN, configs.SOURCE_SIGMA, seed=2024)
+    inp = dict(zip(("x", "y", "z", "vx", "vy", "vz"), xs))
+    cap = n // 200
+    res = []
+    for flags in (0, N.RUN_LANES):
+        tally = ctx.alloc(eng.cb.n_labels, np.int64).fill(0)
+        work = ctx.alloc(N.WORK_N, np.uint64).fill(0)
+        count = ctx.alloc(1, np.int64).fill(0)
+        index = ctx.alloc(cap, np.int64)
+        kept = {k: ctx.alloc(cap) for k in FIELDS}
+        eng.launch(inp, tally=tally, work=work, aoi_mask=1 << eng.cb.label_detected, aoi_cap=cap, aoi_count=count,
+                   aoi_index=index, aoi_out=kept, n=n, flags=flags)
+        ctx.sync()
+        cnt = int(count.to_host()[0])
+        assert 0 < cnt <= cap
+        o = np.argsort(index.to_host()[:cnt])
+        res.append({"tally": tally.to_host(), "work": work.to_host(), "index": index.to_host()[:cnt][o],
+                    "kept": {k: v.to_host()[:cnt][o] for k, v in kept.items()}})
+        del kept, index
+    a, b = res
+    assert int(a["tally"].sum()) == n
+    np.testing.assert_array_equal(a["tally"], b["tally"])
+    np.testing.assert_array_equal(a["work"][:4], b["work"][:4])
+    assert a["work"][N.WORK_FACES_POST] > 0 and b["work"][N.WORK_FACES_POST] == 0  # split kernels, lane walker
+    np.testing.assert_array_equal(a["index"], b["index"])
+    for k in FIELDS:
+        np.testing.assert_array_equal(a["kept"][k], b["kept"][k])
