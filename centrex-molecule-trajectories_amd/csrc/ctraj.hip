@@ -21,6 +21,7 @@
 #include "ctraj_device.hpp"
 #include "ctraj_internal.hpp"
 #include "ctraj_lanes.hpp"
+#include "ctraj_order.hpp"
 #include "ctraj_sinks.hpp"
 #include "ctraj_wave.hpp"
 
@@ -91,16 +92,6 @@ __host__ __device__ inline StepK step_k(double h, double ks, double g) {
   return StepK{hh, h6, ks * hh, ks * h, ks * h6, hex_folds(ks) ? -g / ks : 0.0};
 }
 
-// Claim order of the lens kernel.  A wavefront's lanes stay busy only while its molecules end
-// together, and the lens kernel ends when the last-claimed molecules do, so the queue is claimed
-// longest-lived first: PRE files each queued molecule under a bucket of its predicted number of
-// lens steps (ORD_WIDTH steps per bucket, the last bucket open-ended), k_ord_scan and
-// k_ord_scatter list the queue's slots by bucket, highest first, and the lens kernel claims that
-// list in 64-slot chunks.  The order changes which lane integrates a molecule when, never its
-// arithmetic: every molecule's result is the same in any order.
-constexpr int ORD_BUCKETS = 256;
-constexpr float ORD_WIDTH = 2.0f;
-
 struct PipeArgs {
   longlong2 *q;                // lens-entry queue, 4 vectors per entry
   uint32_t *wcount;            // PRE: entries queued by each wavefront (a prefix of its region)
@@ -117,41 +108,6 @@ struct PipeArgs {
   float ord_k2[CTRAJ_MAX_STATES];  // per state: the acceleration per unit radius at the W' table's end
   int ord_ref;                 // the lens integrates in the reference's steps of dz (else of dt)
 };
-
-// Predicted lens steps of a molecule entering a hexapole lens at (x, y) with transverse velocity
-// (vx, vy) and axial speed vz, as the ordering bucket: the time until its radius reaches the bore
-// under the linear acceleration k2 r (the force where the W' table saturates; k2 > 0 pushes out,
-// r^2(t) = P cosh 2kt + Q sinh 2kt + S; k2 < 0 pulls in, r^2(t) = A + D cos(2wt - phi)), in steps,
-// at most the lens's full steps.  Single precision: the prediction orders work and is not a result.
-__device__ __forceinline__ int ord_bucket(float k2, float x, float y, float vx, float vy, float vz, float bore2,
-                                          float len, float h, float cap, int ref) {
-  const float r0s = x * x + y * y, v0s = vx * vx + vy * vy, rv = x * vx + y * vy;
-  float t = __builtin_inff();  // the molecule stays inside the bore
-  if (k2 > 0.0f) {
-    const float k = __builtin_sqrtf(k2), iv = v0s / k2;
-    const float P = 0.5f * (r0s + iv), Q = rv / k, S = 0.5f * (r0s - iv);
-    // u = e^(2kt) solves (P + Q) u^2 + 2 (S - bore2) u + (P - Q) = 0 (larger root)
-    const float a = P + Q, b = 2.0f * (S - bore2), c = P - Q;
-    const float u = (__builtin_sqrtf(__builtin_fmaxf(b * b - 4.0f * a * c, 0.0f)) - b) / (2.0f * a);
-    t = __logf(__builtin_fmaxf(u, 1.0f)) / (2.0f * k);
-  } else if (k2 < 0.0f) {
-    const float w = __builtin_sqrtf(-k2), iv = v0s / -k2;
-    const float A = 0.5f * (r0s + iv), Bc = 0.5f * (r0s - iv), C = rv / w;
-    const float D = __builtin_sqrtf(Bc * Bc + C * C);
-    if (A + D > bore2) {  // first upward crossing of bore2: 2wt = (phi - acos((bore2 - A) / D)) mod 2 pi
-      const float al = acosf(__builtin_fminf(__builtin_fmaxf((bore2 - A) / D, -1.0f), 1.0f));
-      float th = atan2f(C, Bc) - al;
-      th -= 6.28318531f * __builtin_floorf(th * 0.159154943f);
-      t = th / (2.0f * w);
-    }
-  }
-  // full steps: floor(L / (vz dt)) capped (configuration mode), n = rint(L/dz) (reference mode)
-  const float steps_max = ref ? cap : __builtin_fminf(__builtin_floorf(len / (vz * h)), cap);
-  float s = t * (ref ? vz / h : 1.0f / h);
-  s = __builtin_fminf(s, steps_max);  // a NaN time gives the full steps
-  s = s >= 0.0f ? s : 0.0f;           // and a NaN speed bucket 0
-  return (int)__builtin_fminf(s * (1.0f / ORD_WIDTH), (float)(ORD_BUCKETS - 1));
-}
 
 // Outcome of a lens molecule, written by k_lens into its queue entry's key (queue_put_done).
 enum : int { LENS_QUEUED = 0, LENS_STOPPED = 1, LENS_LEFT = 2 };
@@ -717,69 +673,6 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
 // walk().
 // ---------------------------------------------------------------------------------------------
 
-// Block-wide exclusive scan of one value per thread (256 threads); returns the sum of the values of
-// the threads below, and the total in *total.
-__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t *s, uint32_t *total) {
-  const int t = threadIdx.x;
-  s[t] = v;
-  __syncthreads();
-  for (int d = 1; d < 256; d <<= 1) {
-    const uint32_t add = t >= d ? s[t - d] : 0u;
-    __syncthreads();
-    s[t] += add;
-    __syncthreads();
-  }
-  const uint32_t incl = s[t];
-  *total = s[255];
-  __syncthreads();
-  return incl - v;
-}
-
-// Claim order, step 1: block b (one per bucket, 256 threads) replaces column b of the PRE
-// workgroups' bucket counts by its exclusive prefix over the workgroups and writes the bucket's
-// total; block 0 also leaves the queue's total in ord_tot[ORD_BUCKETS] (below, after step 2).
-__global__ void __launch_bounds__(256) k_ord_scan(uint32_t *hist, int n_rows, uint32_t *tot) {
-  __shared__ uint32_t s[256];
-  const int b = blockIdx.x, t = threadIdx.x;
-  const int per = (n_rows + 255) / 256;
-  const int r0 = min(t * per, n_rows), r1 = min(r0 + per, n_rows);
-  uint32_t sum = 0;
-  for (int r = r0; r < r1; ++r) sum += hist[(size_t)r * ORD_BUCKETS + b];
-  uint32_t total;
-  uint32_t run = block_excl_scan256(sum, s, &total);
-  for (int r = r0; r < r1; ++r) {
-    const size_t k = (size_t)r * ORD_BUCKETS + b;
-    const uint32_t v = hist[k];
-    hist[k] = run;
-    run += v;
-  }
-  if (t == 0) tot[b] = total;
-}
-
-// Claim order, step 2: PRE workgroup g's entries of bucket b go to positions base(b) + prefix(g, b)
-// on, where base(b) counts the entries of the buckets above b; each of the workgroup's four
-// wavefronts lists the filled slots of its region (the order within a bucket is immaterial).
-__global__ void __launch_bounds__(256) k_ord_scatter(PipeArgs P) {
-  __shared__ uint32_t s[256];
-  __shared__ uint32_t cur[ORD_BUCKETS];
-  const int t = threadIdx.x;
-  // base(b) = sum of the totals of buckets b+1 .. ORD_BUCKETS-1: an exclusive scan in reversed order
-  const uint32_t tb = P.ord_tot[ORD_BUCKETS - 1 - t];
-  uint32_t total;
-  const uint32_t above = block_excl_scan256(tb, s, &total);
-  cur[ORD_BUCKETS - 1 - t] = above + P.ord_hist[(size_t)blockIdx.x * ORD_BUCKETS + (ORD_BUCKETS - 1 - t)];
-  if (blockIdx.x == 0 && t == 0) P.ord_tot[ORD_BUCKETS] = total;
-  __syncthreads();
-  const int64_t w = (int64_t)blockIdx.x * 4 + (t >> 6);
-  const int64_t e0 = w * P.region;
-  const int64_t cnt = P.wcount[w];
-  for (int64_t k = t & 63; k < cnt; k += 64) {
-    const int64_t e = e0 + k;
-    const uint32_t pos = atomicAdd(&cur[P.bkt[e]], 1u);
-    P.perm[pos] = (uint32_t)e;
-  }
-}
-
 // Six wavefronts per SIMD (at most 80 VGPRs) for the hexapole RK4 instantiation, which fits them
 // without spilling (with one lens state, 256-thread workgroups and 24 KB of tables, LDS holds six
 // workgroups per CU); the others need more registers than that.
@@ -799,6 +692,8 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
   // cpr chunks per region) or, with the claim list, the list's chunks g, g + 8, g + 16, ...
   const uint32_t cpr = ORD ? 0u : (uint32_t)(P.region >> 6);
   const uint32_t n_q = ORD ? P.ord_tot[ORD_BUCKETS] : 0u;
+  // chunks: per counter, its group's regions x cpr; with the list, the list's ceil(n_q / 64) (counter
+  // g's claim c is chunk c * 8 + g)
   const unsigned long long n_chunks =
       ORD ? (unsigned long long)((n_q + 63u) >> 6) : (unsigned long long)(P.n_pre_waves >> 3) * cpr;
   int grp = blockIdx.x & 7, tried = 0;
@@ -872,28 +767,22 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
           unsigned long long c = 0;
           if (lane == 0) c = atomicAdd(P.claim + 32 * grp, 1ull);
           c = uniform_u64(c);  // every lane is active here, lane 0 first
-          if constexpr (ORD) {
-            const unsigned long long j = c * 8 + (unsigned)grp;
-            if (j >= n_chunks) {
-              if (++tried == 8) exhausted = true;
-              grp = (grp + 1) & 7;
-              continue;
-            }
-            next = (int64_t)j * 64;
-            end = next + 64 < (int64_t)n_q ? next + 64 : (int64_t)n_q;
-            continue;
-          }
-          if (c >= n_chunks) {
+          if (ORD ? c * 8 + (unsigned)grp >= n_chunks : c >= n_chunks) {
             if (++tried == 8) exhausted = true;
             grp = (grp + 1) & 7;
             continue;
           }
-          const uint32_t r = (uint32_t)c / cpr;
-          const uint32_t w = ((r >> 2) << 5) | ((uint32_t)grp << 2) | (r & 3u);
-          const int64_t base = (int64_t)((uint32_t)c - r * cpr) * 64;
-          const int64_t filled = (int64_t)P.wcount[w] - base;
-          next = (int64_t)w * P.region + base;
-          end = next + (filled < 64 ? filled : 64);  // next >= end: an empty chunk, claim again
+          if constexpr (ORD) {
+            next = (int64_t)(c * 8 + (unsigned)grp) * 64;
+            end = next + 64 < (int64_t)n_q ? next + 64 : (int64_t)n_q;
+          } else {
+            const uint32_t r = (uint32_t)c / cpr;
+            const uint32_t w = ((r >> 2) << 5) | ((uint32_t)grp << 2) | (r & 3u);
+            const int64_t base = (int64_t)((uint32_t)c - r * cpr) * 64;
+            const int64_t filled = (int64_t)P.wcount[w] - base;
+            next = (int64_t)w * P.region + base;
+            end = next + (filled < 64 ? filled : 64);  // next >= end: an empty chunk, claim again
+          }
           continue;
         }
         const int64_t avail = end - next;
@@ -1572,7 +1461,8 @@ static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *sdev) {
       HIPCHK(c, hipGetLastError());
       if (ordered) {
         hipLaunchKernelGGL(k_ord_scan, dim3(ORD_BUCKETS), dim3(256), 0, c->stream, P.ord_hist, gpre, P.ord_tot);
-        hipLaunchKernelGGL(k_ord_scatter, dim3(gpre), dim3(256), 0, c->stream, P);
+        hipLaunchKernelGGL(k_ord_scatter, dim3(gpre), dim3(256), 0, c->stream,
+                           OrdList{P.bkt, P.ord_hist, P.ord_tot, P.perm, P.wcount, P.region});
         HIPCHK(c, hipGetLastError());
       }
       if (pev && one_slice) HIPCHK(c, hipEventRecord(pev[1], c->stream));
