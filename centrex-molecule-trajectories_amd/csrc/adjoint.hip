@@ -84,7 +84,7 @@ __device__ __forceinline__ double drift_g_adj(double vx, double vy_after, double
 __device__ __forceinline__ void hex_eval_d(const WsRef &wsb, double umax, double ks, double cu, double kc2,
                                            double invV, double r2, double &f, double &d, double &gv) {
   const double u0 = r2 * cu;
-  const double u = __builtin_fmin(u0, umax);
+  const double u = clamp_to(u0, umax);
   const unsigned k = (unsigned)(int)u;
   const double s = __builtin_amdgcn_fract(u);
   const dbl2_t ab = wsb.ab[k];
@@ -94,13 +94,37 @@ __device__ __forceinline__ void hex_eval_d(const WsRef &wsb, double umax, double
   if (u0 > umax) dP = 0.0;
   f = P * ks;
   d = kc2 * dP;
-  gv = (f + (0.5 * d) * r2) * invV;
+  gv = __builtin_fma(0.5 * d, r2, f) * invV;
 }
 
 struct LensK {
   WsRef wsb;
   double umax, ks, cu, kc2, invV, g;
 };
+
+// The forward sweep's lens step and its recomputation in the reverse sweep: full steps of the
+// lens's uniform h take the folded hexapole step with the constants formed once per lens (as the
+// propagation's lens kernel does), the final partial step hp takes lens_step; where the force scale
+// does not fold every step takes the plain RK4 step.  The forward sweep and both recomputations call
+// this one function, so a recomputed state equals the forward state bit for bit.
+struct SmoothStep {
+  double h, hh, h6, khh, kh, kh6;
+  bool fold;
+};
+
+__device__ __forceinline__ SmoothStep smooth_step_k(const Force<CTRAJ_FORCE_WSPLINE_HEX> &F, double h) {
+  const double hh = 0.5 * h, h6 = h * kSixth;
+  return SmoothStep{h, hh, h6, F.ks * hh, F.ks * h, F.ks * h6, hex_folds(F.ks)};
+}
+
+__device__ __forceinline__ void smooth_step(const Force<CTRAJ_FORCE_WSPLINE_HEX> &F, const Tables &T, const WsRef &wsb,
+                                            const SmoothStep &S, bool full, double hp, double &x, double &y,
+                                            double &vx, double &vy, double r2, uint32_t &st) {
+  if (full && S.fold)
+    rk4_step_hex_k(wsb, T.umax, F.cu, F.gk, S.h, S.hh, S.h6, S.khh, S.kh, S.kh6, x, y, vx, vy, r2);
+  else
+    lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, full ? S.h : hp, x, y, vx, vy, r2, st);
+}
 
 // Reverse of one classical RK4 step (rk4_step) of size h from input state (x, y, vx, vy).
 // In: (pbx, pby, vbx, vby) = adjoint of the step's output.  Out: adjoint of its input.
@@ -125,44 +149,46 @@ __device__ __forceinline__ double rk4_adjoint(const LensK &L, double h, double x
   const double p4x = __builtin_fma(h, v3x, x), p4y = __builtin_fma(h, v3y, y);
   hex_eval_d(L.wsb, L.umax, L.ks, L.cu, L.kc2, L.invV, __builtin_fma(p4x, p4x, p4y * p4y), f4, d4, g4);
 
+  // The reverse sweep below forms each sum of products with fused multiply-adds (it is checked
+  // against the CPU adjoint to a tolerance, not bit for bit).
   // stage 4: a4 = A(p4), p4 = p1 + h v3, v4 = v1 + h a3
   const double ab4x = h6 * vbx, ab4y = h6 * vby;
-  double q = p4x * ab4x + p4y * ab4y;
+  double q = __builtin_fma(p4x, ab4x, p4y * ab4y);
   double dq = d4 * q;
-  const double pb4x = f4 * ab4x + dq * p4x, pb4y = f4 * ab4y + dq * p4y;
-  Vbar = Vbar + q * g4;
+  const double pb4x = __builtin_fma(f4, ab4x, dq * p4x), pb4y = __builtin_fma(f4, ab4y, dq * p4y);
+  Vbar = __builtin_fma(q, g4, Vbar);
   const double vb4x = h6 * pbx, vb4y = h6 * pby;
   double Px = pbx + pb4x, Py = pby + pb4y;
-  const double vb3x = h3 * pbx + h * pb4x, vb3y = h3 * pby + h * pb4y;
+  const double vb3x = __builtin_fma(h3, pbx, h * pb4x), vb3y = __builtin_fma(h3, pby, h * pb4y);
   double Vx = vbx + vb4x, Vy = vby + vb4y;
-  const double ab3x = h3 * vbx + h * vb4x, ab3y = h3 * vby + h * vb4y;
+  const double ab3x = __builtin_fma(h3, vbx, h * vb4x), ab3y = __builtin_fma(h3, vby, h * vb4y);
   // stage 3: a3 = A(p3), p3 = p1 + hh v2, v3 = v1 + hh a2
-  q = p3x * ab3x + p3y * ab3y;
+  q = __builtin_fma(p3x, ab3x, p3y * ab3y);
   dq = d3 * q;
-  const double pb3x = f3 * ab3x + dq * p3x, pb3y = f3 * ab3y + dq * p3y;
-  Vbar = Vbar + q * g3;
+  const double pb3x = __builtin_fma(f3, ab3x, dq * p3x), pb3y = __builtin_fma(f3, ab3y, dq * p3y);
+  Vbar = __builtin_fma(q, g3, Vbar);
   Px = Px + pb3x;
   Py = Py + pb3y;
-  const double vb2x = h3 * pbx + hh * pb3x, vb2y = h3 * pby + hh * pb3y;
+  const double vb2x = __builtin_fma(h3, pbx, hh * pb3x), vb2y = __builtin_fma(h3, pby, hh * pb3y);
   Vx = Vx + vb3x;
   Vy = Vy + vb3y;
-  const double ab2x = h3 * vbx + hh * vb3x, ab2y = h3 * vby + hh * vb3y;
+  const double ab2x = __builtin_fma(h3, vbx, hh * vb3x), ab2y = __builtin_fma(h3, vby, hh * vb3y);
   // stage 2: a2 = A(p2), p2 = p1 + hh v1, v2 = v1 + hh a1
-  q = p2x * ab2x + p2y * ab2y;
+  q = __builtin_fma(p2x, ab2x, p2y * ab2y);
   dq = d2 * q;
-  const double pb2x = f2 * ab2x + dq * p2x, pb2y = f2 * ab2y + dq * p2y;
-  Vbar = Vbar + q * g2;
+  const double pb2x = __builtin_fma(f2, ab2x, dq * p2x), pb2y = __builtin_fma(f2, ab2y, dq * p2y);
+  Vbar = __builtin_fma(q, g2, Vbar);
   Px = Px + pb2x;
   Py = Py + pb2y;
-  Vx = ((Vx + hh * pb2x) + vb2x) + h6 * pbx;
-  Vy = ((Vy + hh * pb2y) + vb2y) + h6 * pby;
-  const double ab1x = h6 * vbx + hh * vb2x, ab1y = h6 * vby + hh * vb2y;
+  Vx = __builtin_fma(h6, pbx, __builtin_fma(hh, pb2x, Vx) + vb2x);
+  Vy = __builtin_fma(h6, pby, __builtin_fma(hh, pb2y, Vy) + vb2y);
+  const double ab1x = __builtin_fma(h6, vbx, hh * vb2x), ab1y = __builtin_fma(h6, vby, hh * vb2y);
   // stage 1: a1 = A(p1)
-  q = x * ab1x + y * ab1y;
+  q = __builtin_fma(x, ab1x, y * ab1y);
   dq = d1 * q;
-  Px = Px + (f1 * ab1x + dq * x);
-  Py = Py + (f1 * ab1y + dq * y);
-  Vbar = Vbar + q * g1;
+  Px = Px + __builtin_fma(f1, ab1x, dq * x);
+  Py = Py + __builtin_fma(f1, ab1y, dq * y);
+  Vbar = __builtin_fma(q, g1, Vbar);
   double hbar = 0.0;
   if constexpr (WANT_H) {
     const double a4x = f4 * p4x, a4y = __builtin_fma(f4, p4y, -g);
@@ -277,8 +303,10 @@ __global__ void __launch_bounds__(512) k_smooth(KBeam B, KTab KT, SmoothArgs A) 
       } else {  // CTRAJ_LENS
         const double V = P[pidx], z0 = P[pidx + 1];
         const double scale = V / E.p[7];
-        const Force<CTRAJ_FORCE_WSPLINE_HEX> F{E.p[4] * scale, E.p[5] * scale, g};
+        const double ks0 = E.p[4] * scale;
+        const Force<CTRAJ_FORCE_WSPLINE_HEX> F{ks0, E.p[5] * scale, g, hex_folds(ks0) ? -g / ks0 : 0.0};
         const double h = E.p[2], Ll = E.p[3];
+        const SmoothStep SK = smooth_step_k(F, h);
         if (!(scale >= 0.0)) {  // a negative or NaN field scale has no table: the set contributes 0
           ok = false;
           Jw = 0.0;
@@ -295,7 +323,7 @@ __global__ void __launch_bounds__(512) k_smooth(KBeam B, KTab KT, SmoothArgs A) 
             if constexpr (GRAD) {
               if (s % K == 0) put4(W, A.off_ck + cslot + s / K, lane, x, y, vx, vy);
             }
-            rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, s < nfull ? h : hp, x, y, vx, vy, r2, st);
+            smooth_step(F, T, wsb, SK, s < nfull, hp, x, y, vx, vy, r2, st);
             r2 = __builtin_fma(x, x, y * y);
           }
         }
@@ -350,8 +378,9 @@ __global__ void __launch_bounds__(512) k_smooth(KBeam B, KTab KT, SmoothArgs A) 
           const double scale = V / E.p[7];
           const double ks = E.p[4] * scale, cu = E.p[5] * scale;
           const LensK LK{wsb, T.umax, ks, cu, (2.0 * ks) * cu, 1.0 / V, g};
-          const Force<CTRAJ_FORCE_WSPLINE_HEX> F{ks, cu, g};
+          const Force<CTRAJ_FORCE_WSPLINE_HEX> F{ks, cu, g, hex_folds(ks) ? -g / ks : 0.0};
           const double h = E.p[2], Ll = E.p[3];
+          const SmoothStep SK = smooth_step_k(F, h);
           double Vbar = 0.0, z0bar = zexit_bar, zfrom_bar = 0.0;
           cslot -= A.ck_per_lens;
           --fslot;
@@ -374,7 +403,7 @@ __global__ void __launch_bounds__(512) k_smooth(KBeam B, KTab KT, SmoothArgs A) 
                 if (nsub > 1) put4(W, A.off_sub + q, lane, x, y, vx, vy);
                 if (q + 1 < nsub) {
                   for (int s = s0 + q * kSub; s < s0 + (q + 1) * kSub; ++s) {
-                    rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, s < nfull ? h : hp, x, y, vx, vy, r2, st);
+                    smooth_step(F, T, wsb, SK, s < nfull, hp, x, y, vx, vy, r2, st);
                     r2 = __builtin_fma(x, x, y * y);
                   }
                 }
@@ -390,7 +419,7 @@ __global__ void __launch_bounds__(512) k_smooth(KBeam B, KTab KT, SmoothArgs A) 
                   r[64] = y;
                   r[128] = vx;
                   r[192] = vy;
-                  rk4_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, s < nfull ? h : hp, x, y, vx, vy, r2, st);
+                  smooth_step(F, T, wsb, SK, s < nfull, hp, x, y, vx, vy, r2, st);
                   r2 = __builtin_fma(x, x, y * y);
                 }
                 for (int s = t1 - 1; s >= t0; --s) {
