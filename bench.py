@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint measurement")
     ap.add_argument("--adjoint-only", action="store_true", help="only the forward+adjoint measurement")
     ap.add_argument("--adjoint-n", type=int, default=1_000_000)
+    ap.add_argument("--adjoint-k", type=int, default=0,
+                    help="with --adjoint-only: only the forward+adjoint at this checkpoint interval")
     ap.add_argument("--scan-n", type=int, default=100_000, help="molecules per parameter set of the gradient scan")
     ap.add_argument("--scan-sets", type=int, default=64, help="parameter sets of the gradient scan")
     ap.add_argument("--no-extras", action="store_true", help="skip the ballistic and reference-mode measurements")
@@ -517,6 +519,10 @@ def main():
 
     ctx = get_context(0)
     if args.adjoint_only:
+        if args.adjoint_k:
+            print(json.dumps({f"K{args.adjoint_k}": bench_adjoint(ctx, args.adjoint_n, args.adjoint_k,
+                                                                  max(1, args.steps // 4))}), flush=True)
+            return
         print(json.dumps({"K64": bench_adjoint(ctx, args.adjoint_n, 64, max(1, args.steps // 4)),
                           "K8": bench_adjoint(ctx, args.adjoint_n, 8, max(1, args.steps // 4)),
                           "scan": bench_adjoint_scan(ctx, args.scan_n, args.scan_sets)}), flush=True)
