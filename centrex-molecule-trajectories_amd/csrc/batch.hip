@@ -28,9 +28,8 @@ using namespace ctraj;
 namespace {
 
 constexpr int kMaxBins = 4096;  // configurations x labels tallied in LDS
-constexpr int64_t kPipeMax = (int64_t)1 << 27;  // molecules per slice of the two-phase path (64 B of queue each)
+constexpr int64_t kPipeMax = (int64_t)1 << 27;  // molecules per slice of the split path (64 B of queue each)
 
-enum : int { LN_IDLE = 0, LN_LENS = 1, LN_EXIT = 2, LN_FIN = 3, LN_DONE = 4 };
 
 struct BCfg {
   int64_t begin, n;
@@ -39,7 +38,7 @@ struct BCfg {
   int32_t e0, ne, lens, pad;  // element slice; lens index within the slice (-1: none)
 };
 
-// Lens constants of one configuration, staged in the two-phase lens kernel's LDS.
+// Lens constants of one configuration, staged in the split path's lens kernel's LDS.
 struct BLensC {
   double ks, cu, R2, z1, Hs, Ll, gk;  // gk = -g/ks where the lens folds ks into its step (hex_folds), else 0
   int32_t label_inside, e0, ne, lens;  // element slice and the lens's index in it
@@ -55,11 +54,11 @@ struct BArgs {
   int64_t total;
   double g;
   int n_cfg, n_labels, label_detected, cap, chunk, refill_min;
-  double Hs;                    // the lens dt shared by every configuration (two-phase lens kernel, UH)
+  double Hs;                    // the lens dt shared by every configuration (split path's lens kernel, UH)
   int off_begin, off_hist;      // in doubles from the LDS base (off_begin: the lens constants in the lens kernel)
 };
 
-// Two-phase scan: the PRE kernel queues the molecules that enter their configuration's lens into
+// Split scan: the PRE kernel queues the molecules that enter their configuration's lens into
 // per-wavefront regions (the single-run queue layout, key = configuration index); the lens kernel
 // claims 64-entry chunks from eight per-XCD counters.
 struct BPipe {
@@ -296,7 +295,7 @@ __global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
   if (threadIdx.x < 4 && s_work[threadIdx.x]) atomicAdd(&A.work[threadIdx.x], s_work[threadIdx.x]);
 }
 
-// Two-phase scan, phase 1: every molecule of the slice [s0, s1) is drawn and walked to and through
+// Split scan, phase 1: every molecule of the slice [s0, s1) is drawn and walked to and through
 // its configuration's lens entrance (grid-stride, wave-uniform trip count); the molecules that enter
 // a lens are queued in the wavefront's region, the others are tallied.
 template <int KM>
@@ -342,25 +341,28 @@ __global__ void __launch_bounds__(256) k_batch_pre(BArgs A, BPipe P) {
   if (threadIdx.x < 4 && s_work[threadIdx.x]) atomicAdd(&A.work[threadIdx.x], s_work[threadIdx.x]);
 }
 
-// Two-phase scan, phase 2: persistent wavefronts integrate the queued molecules through their
-// configuration's lens with lane refill (the configuration's lens constants staged in LDS, its field
-// constants and step in the lane's registers), walk the survivors through the elements after the
-// lens and tally.  The arithmetic per molecule is the one of k_batch_lanes.  UH: every configuration's
-// lens has the same dt, whose half and sixth come formed by the host (Hh, H6; otherwise unused), and
-// folds its force scale; otherwise each step chooses the folded or the plain RK4 step for its
-// molecule's lens (lens_step).  KM: the element kinds after the lenses (KM_CIRC keeps the walk's code
-// and registers out).
-template <bool UH, int KM>
-__global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT, double Hh, double H6) {
+// Split scan, phase 2: persistent wavefronts integrate the queued molecules through their
+// configuration's lens with lane refill and write each molecule's state where the lens left it back
+// into its queue entry, with its outcome (the single-run lens kernel's scheme, with the lens
+// constants per lane: the configuration's lens constants staged in LDS, its field constants and
+// step in the lane's registers).  A lane's rem is its full steps left (> 0), 0 at the lens end (one
+// partial step of L/vz - n dt to go), REM_STOPPED once the bore stopped it, REM_EMPTY when it holds
+// no molecule.  UH: every configuration's lens has the same dt, whose half and sixth come formed by
+// the host (Hh, H6), and folds its force scale (the step constants ks h/2, ks h, ks h/6 are formed per
+// molecule); otherwise each step chooses the folded or the plain RK4 step for its molecule's lens
+// (lens_step).  The arithmetic per molecule is the one of k_batch_lanes.
+constexpr int BREM_EMPTY = -1, BREM_STOPPED = -2;
+enum : int { BLENS_STOPPED = 1, BLENS_LEFT = 2 };
+
+template <bool UH>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(UH ? 6 : 1)))
+k_batch_integ(BArgs A, BPipe P, KTab KT, double Hh, double H6) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
-  __shared__ unsigned long long s_work[4];
+  __shared__ unsigned long long s_steps;
+  if (threadIdx.x == 0) s_steps = 0;
   const Tables T = stage_tables(lds, KT);
   BLensC *s_cfg = reinterpret_cast<BLensC *>(lds + A.off_begin);
-  uint32_t *s_hist = reinterpret_cast<uint32_t *>(lds + A.off_hist);
-  const int nbins = A.n_cfg * A.n_labels;
   for (int i = threadIdx.x; i < A.n_cfg; i += blockDim.x) s_cfg[i] = A.lensk[i];
-  for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0;
-  if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
   __syncthreads();
 
   const WsRef wsb = ws_of(T, 0);
@@ -369,49 +371,48 @@ __global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT, 
   const uint32_t cpr = (uint32_t)(P.region >> 6);
   const unsigned long long n_chunks = (unsigned long long)(P.n_pre_waves >> 3) * cpr;
   int grp = blockIdx.x & 7, tried = 0;
-  WalkCounters wc{0u, 0u};
-  uint32_t n_oot = 0, n_cap = 0;
-  unsigned long long steps_w = 0;
+  uint32_t steps_w = 0;  // wave-uniform
   int64_t next = 0, end = 0;
   bool exhausted = false;
 
   MState m;
-  m.ax = 0.0; m.ay = -g; m.az = 0.0;
-  int c = 0, ph = LN_IDLE, label = 0, i = 0, nfull = 0;
+  m.x = m.y = m.z = m.vx = m.vy = m.vz = m.t = 0.0;
+  double r2 = 0.0, dzs = 0.0, ks = 0.0, cu = 0.0, gk = 0.0, R2 = 0.0, Hs = A.Hs, khh = 0.0, kh = 0.0, kh6 = 0.0;
+  int rem = BREM_EMPTY, nfull = 0, c = 0;
   uint32_t status = 0;
-  double ks = 0.0, cu = 0.0, gk = 0.0, R2 = 0.0, Hs = A.Hs, dzs = 0.0, r2 = 0.0;
+  int64_t slot = 0, key = 0;
 
   for (;;) {
-    const uint64_t lensm = __ballot(ph == LN_LENS);
-    if (lensm == 0 || 64 - __popcll(lensm) >= A.refill_min) {
+    bool act = rem > 0;
+    uint64_t am = __builtin_amdgcn_ballot_w64(act);
+    if (64 - __popcll(am) >= A.refill_min) {
       for (;;) {
-        if (ph == LN_EXIT) {  // the lens exit row stores (0, -g, 0); then the elements after the lens
-          m.ax = 0.0;
-          m.ay = -g;
-          m.az = 0.0;
-          const BLensC &K = s_cfg[c];
-          label = A.label_detected;
-          walk_plain<KM>(opaque(A.elems) + K.e0, K.lens + 1, K.ne, g, m, label, wc);
-          ph = LN_FIN;
-        }
-        if (__ballot(ph == LN_FIN)) {
-          const bool fin = ph == LN_FIN;
-          if (fin) {
-            n_oot += (status & CTRAJ_ST_OOT) ? 1u : 0u;
-            n_cap += (status & CTRAJ_ST_CAPPED) ? 1u : 0u;
+        // finish the molecules that take no more full steps: stopped by the bore, or at the lens end
+        // with its partial step to take
+        const bool fin = rem == 0 || rem == BREM_STOPPED;
+        steps_w += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(rem == 0));
+        if (fin) {
+          int outcome = BLENS_STOPPED;
+          if (rem == 0) {
+            const BLensC &K = s_cfg[c];
+            const double hp = K.Ll / m.vz - (double)nfull * Hs;
+            const Force<CTRAJ_FORCE_WSPLINE_HEX> F{ks, cu, g, gk};
+            lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, hp, m.x, m.y, m.vx, m.vy, r2, status);
+            m.z = K.z1;
+            m.t = m.t + hp;
+            r2 = __builtin_fma(m.x, m.x, m.y * m.y);
+            outcome = r2 > R2 ? BLENS_STOPPED : BLENS_LEFT;
           }
-          batch_hist(s_hist, fin, fin ? c * A.n_labels + label : nbins + lane, lane);
-          if (fin) ph = LN_IDLE;
+          queue_put_done(P.q, slot, m.x, m.y, m.z, m.vx, m.vy, m.vz, m.t,
+                         key | ((int64_t)status << 40) | ((int64_t)outcome << 48));
+          rem = BREM_EMPTY;
         }
-        const uint64_t need = __ballot(ph == LN_IDLE);
-        if (!need || exhausted) {
-          if (ph == LN_IDLE && exhausted) ph = LN_DONE;
-          break;
-        }
+        const uint64_t need = __builtin_amdgcn_ballot_w64(rem == BREM_EMPTY);
+        if (!need || exhausted) break;
         if (next >= end) {
           unsigned long long cl = 0;
           if (lane == 0) cl = atomicAdd(P.claim + 32 * grp, 1ull);
-          cl = __shfl(cl, 0);
+          cl = uniform_u64(cl);  // every lane is active here, lane 0 first
           if (cl >= n_chunks) {
             if (++tried == 8) exhausted = true;
             grp = (grp + 1) & 7;
@@ -426,66 +427,108 @@ __global__ void __launch_bounds__(1024) k_batch_lens(BArgs A, BPipe P, KTab KT, 
           continue;
         }
         const int64_t avail = end - next;
-        if (ph == LN_IDLE) {
+        if (rem == BREM_EMPTY) {
           const int rank = lanes_below(need);
           if (rank < avail) {
-            int64_t key;
-            int st;
-            queue_get(P.q, next + rank, m, key, st);
-            c = (int)key;
+            slot = next + rank;
+            queue_get_done(P.q, slot, m, key);
+            c = (int)(key & QKEY_INDEX);
             const BLensC &K = s_cfg[c];
             ks = K.ks;
             cu = K.cu;
             gk = K.gk;
             R2 = K.R2;
             if constexpr (!UH) Hs = K.Hs;
-            m.ax = 0.0;
-            m.ay = -g;
-            m.az = 0.0;
+            if constexpr (UH) {
+              khh = ks * Hh;
+              kh = ks * Hs;
+              kh6 = ks * H6;
+            }
             status = 0;
-            i = 0;
             nfull = rk4_full_steps(K.Ll, m.vz, Hs, A.cap, status);
             dzs = m.vz * Hs;
             r2 = __builtin_fma(m.x, m.x, m.y * m.y);
-            ph = LN_LENS;
+            rem = nfull;
           }
         }
         next += avail < (int64_t)__popcll(need) ? avail : (int64_t)__popcll(need);
       }
-      if (!__ballot(ph == LN_LENS)) {
-        if (!__ballot(ph == LN_EXIT || ph == LN_FIN)) break;
-        continue;
-      }
+      act = rem > 0;
+      am = __builtin_amdgcn_ballot_w64(act);
+      if (!am) break;  // the queue is exhausted and every molecule has been written back
     }
 
-    steps_w += (unsigned long long)__popcll(__ballot(ph == LN_LENS));
-    if (ph == LN_LENS) {
-      const Force<CTRAJ_FORCE_WSPLINE_HEX> F{ks, cu, g, gk};
-      const bool last = (i == nfull);
-      if (!last) {  // with UH every lens folds
-        if constexpr (UH)
-          rk4_step_hex_k(wsb, T.umax, cu, gk, Hs, Hh, H6, ks * Hh, ks * Hs, ks * H6, m.x, m.y, m.vx, m.vy, r2);
-        else
-          lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, Hs, m.x, m.y, m.vx, m.vy, r2, status);
-        m.z = m.z + dzs;
-        m.t = m.t + Hs;
-      } else {  // the final partial step, formed here, once per molecule that reaches the lens end
-        const BLensC &K = s_cfg[c];
-        const double hp = K.Ll / m.vz - (double)nfull * Hs;
-        lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, hp, m.x, m.y, m.vx, m.vy, r2, status);
-        m.z = K.z1;
-        m.t = m.t + hp;
+    steps_w += (uint32_t)__popcll(am);
+    if (act) {
+      if constexpr (UH) {
+        rk4_step_hex_k(wsb, T.umax, cu, gk, Hs, Hh, H6, khh, kh, kh6, m.x, m.y, m.vx, m.vy, r2);
+      } else {
+        const Force<CTRAJ_FORCE_WSPLINE_HEX> F{ks, cu, g, gk};
+        lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, Hs, m.x, m.y, m.vx, m.vy, r2, status);
       }
+      m.z = m.z + dzs;
+      m.t = m.t + Hs;
       r2 = __builtin_fma(m.x, m.x, m.y * m.y);
-      ++i;
-      const bool dead = r2 > R2;
-      if (dead) label = s_cfg[c].label_inside;
-      ph = dead ? (int)LN_FIN : (last ? (int)LN_EXIT : (int)LN_LENS);
+      rem = r2 > R2 ? BREM_STOPPED : rem - 1;
     }
   }
 
+  if (lane == 0 && steps_w) atomicAdd(&s_steps, (unsigned long long)steps_w);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_steps) atomicAdd(&A.work[1], s_steps);
+}
+
+// Split scan, phase 3: every queue entry the lens kernel wrote back (grid-stride, wave-uniform trip
+// count): a molecule that left its lens walks its configuration's elements after the lens (the lens
+// exit row stores (0, -g, 0)), one stopped by the bore is recorded inside the lens; (configuration,
+// label) tallied in LDS.
+template <int KM>
+__global__ void __launch_bounds__(256) k_batch_post(BArgs A, BPipe P) {
+  extern __shared__ __attribute__((aligned(16))) double lds[];
+  __shared__ unsigned long long s_work[4];
+  uint32_t *s_hist = reinterpret_cast<uint32_t *>(lds);
+  const int nbins = A.n_cfg * A.n_labels;
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0;
+  if (threadIdx.x < 4) s_work[threadIdx.x] = 0;
+  __syncthreads();
+
+  const int lane = lane_id();
+  const double g = A.g;
+  const int64_t n_items = (int64_t)P.n_pre_waves * P.region;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  WalkCounters wc{0u, 0u};
+  uint32_t n_oot = 0, n_cap = 0;
+  for (int64_t wbase = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < n_items; wbase += stride) {
+    const int64_t e = wbase + lane;
+    bool fin = false;
+    int c = 0, label = 0;
+    if (e < n_items) {
+      const int64_t w = e / P.region;
+      fin = e - w * P.region < (int64_t)P.wcount[w];
+    }
+    if (fin) {
+      MState m;
+      int64_t key;
+      queue_get_done(P.q, e, m, key);
+      c = (int)(key & QKEY_INDEX);
+      const uint32_t status = (uint32_t)((key >> 40) & 0xff);
+      const int outcome = (int)((key >> 48) & 0xff);
+      const BLensC &K = A.lensk[c];
+      if (outcome == BLENS_LEFT) {
+        m.ax = 0.0;
+        m.ay = -g;
+        m.az = 0.0;
+        label = A.label_detected;
+        walk_plain<KM>(opaque(A.elems) + K.e0, K.lens + 1, K.ne, g, m, label, wc);
+      } else {
+        label = K.label_inside;
+      }
+      n_oot += (status & CTRAJ_ST_OOT) ? 1u : 0u;
+      n_cap += (status & CTRAJ_ST_CAPPED) ? 1u : 0u;
+    }
+    if (__ballot(fin)) batch_hist(s_hist, fin, fin ? c * A.n_labels + label : nbins + lane, lane);
+  }
   if (wc.faces) atomicAdd(&s_work[0], (unsigned long long)wc.faces);
-  if (lane == 0 && steps_w) atomicAdd(&s_work[1], steps_w);
   if (n_oot) atomicAdd(&s_work[2], (unsigned long long)n_oot);
   if (n_cap) atomicAdd(&s_work[3], (unsigned long long)n_cap);
   __syncthreads();
@@ -511,14 +554,14 @@ struct ctraj_batch {
   int64_t *d_tally = nullptr;
   double *d_ws = nullptr;  // the batch's own copy of the W' spline (state 0), [nseg][4]
   int ws_nseg = 0;
-  // two-phase path: per-configuration lens constants, the lens-entry queue and its counters
+  // split path: per-configuration lens constants, the lens-entry queue and its counters
   BLensC *d_lensk = nullptr;
   void *d_pipe = nullptr, *d_pipe_cnt = nullptr;
   int pre_grid = 0, n_pre_waves = 0;
   int64_t region = 0;
   bool uniform_dt = false;  // every lens has the same dt and folds its force scale (hex_folds)
   double dt = 0.0;
-  int64_t slice = 0;        // molecules per slice of the two-phase path
+  int64_t slice = 0;        // molecules per slice of the split path
   bool circ_only = false;   // only circular apertures (and lenses) in every configuration
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // captured launches by (block, chunk, grid, refill, fused)
@@ -578,18 +621,19 @@ static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refi
   const size_t lds_pre = (size_t)b->n_cfg * sizeof(int64_t) + hist_bytes;
   const void *kp = b->circ_only ? (const void *)k_batch_pre<KM_CIRC> : (const void *)k_batch_pre<KM_ALL>;
   if ((rc = lds_limit(c, kp, lds_pre)) != CTRAJ_OK) return rc;
-  // phase 2: LDS = tables + lens constants + histogram
+  // phase 2: LDS = tables + lens constants
   const size_t cfg_dbl = (size_t)b->n_cfg * (sizeof(BLensC) / sizeof(double));
   A.off_begin = (int)tab;
-  A.off_hist = (int)(tab + cfg_dbl);
-  const size_t lds_lens = (tab + cfg_dbl) * sizeof(double) + hist_bytes;
-  const bool circ = b->circ_only;
-  const void *kl = b->uniform_dt ? (circ ? (const void *)k_batch_lens<true, KM_CIRC> : (const void *)k_batch_lens<true, KM_ALL>)
-                                 : (circ ? (const void *)k_batch_lens<false, KM_CIRC> : (const void *)k_batch_lens<false, KM_ALL>);
+  const size_t lds_lens = (tab + cfg_dbl) * sizeof(double);
+  const void *kl = b->uniform_dt ? (const void *)k_batch_integ<true> : (const void *)k_batch_integ<false>;
   A.Hs = b->dt;
   if ((rc = lds_limit(c, kl, lds_lens)) != CTRAJ_OK) return rc;
   if ((rc = block_ok(c, kl, block)) != CTRAJ_OK) return rc;
   if (grid <= 0) grid = grid_for(c, kl, block, lds_lens, (int64_t)1 << 40, 0);
+  // phase 3: LDS = histogram
+  const void *kq = b->circ_only ? (const void *)k_batch_post<KM_CIRC> : (const void *)k_batch_post<KM_ALL>;
+  if ((rc = lds_limit(c, kq, hist_bytes)) != CTRAJ_OK) return rc;
+  const int gpost = grid_for(c, kq, 256, hist_bytes, (int64_t)b->n_pre_waves * b->region, 0);
   // queue counters and tallies are reset by fill kernels (no memset nodes in the captured graph)
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, c->stream, (unsigned long long *)b->d_pipe_cnt,
                      (int64_t)8 * 32);
@@ -610,6 +654,8 @@ static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refi
     hipLaunchKernelGGL((PreK)kp, dim3(b->pre_grid), dim3(256), lds_pre, c->stream, A, P);
     HIPCHK(c, hipGetLastError());
     hipLaunchKernelGGL((LensK)kl, dim3(grid), dim3(block), lds_lens, c->stream, A, P, kt, Hh, H6);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL((PreK)kq, dim3(gpost), dim3(256), hist_bytes, c->stream, A, P);
     HIPCHK(c, hipGetLastError());
   }
   return CTRAJ_OK;
@@ -688,7 +734,7 @@ int32_t ctraj_batch_create(ctraj_ctx *c, const ctraj_elem *elems, int32_t n_elem
       return fail(c, CTRAJ_ENOMEM, "batch allocation failed");
     }
   }
-  // two-phase path: lens constants per configuration and a lens-entry queue sized for the batch
+  // split path: lens constants per configuration and a lens-entry queue sized for the batch
   std::vector<BLensC> hl(n_cfgs);
   b->uniform_dt = true;
   bool have_dt = false;

@@ -80,7 +80,7 @@ class BatchScan:
 
     @property
     def split_path(self) -> bool:
-        """True when fused = 0 runs the two-phase path (walk and queue, then the lens kernel)."""
+        """True when fused = 0 runs the split kernels (walk and queue, lens, walk after the lens)."""
         return bool(N.lib().ctraj_batch_split_path(self.handle))
 
     def run(self, tune: N.BatchTune | None = None, graph: bool | None = None):
@@ -106,7 +106,7 @@ class BatchScan:
                  cache: str | Path | None = None, verbose: bool = False) -> N.BatchTune:
         """Time every (block, chunk, refill, fused) candidate (median of ``reps`` device-timed launches
         after one warm-up) and keep the fastest as ``self.tune`` (graph replay on).  ``fused`` lists the
-        paths to try: 0 the two phases (``block`` is the lens kernel's workgroup, up to 1024; ``chunk``
+        paths to try: 0 the split kernels (``block`` is the lens kernel's workgroup, up to 1024; ``chunk``
         is unused), 1 the single-kernel walker (``block`` up to 256).  With ``cache`` (a JSON file), a
         previous result for the same device and batch signature is reused."""
         key = f"{self.ctx.arch}:{self.signature()}"
@@ -121,7 +121,7 @@ class BatchScan:
                     return self.tune
         results = []
         for f, b, c, r in itertools.product(fused, blocks, chunks, refills):
-            if (f and b > 256) or (not f and c != chunks[0]):  # the two phases take no chunk
+            if (f and b > 256) or (not f and c != chunks[0]):  # the split kernels take no chunk
                 continue
             t = N.BatchTune(block=b, chunk=c, grid=0, refill=r, use_graph=0, fused=f)
             self.run(t)

@@ -287,12 +287,14 @@ typedef struct ctraj_batch_cfg {
   uint64_t seed;               /* device source stream of this configuration */
   double mean[6], sigma[6];    /* Gaussian source for x, y, z, vx, vy, vz */
 } ctraj_batch_cfg;
-/* A batch with a lens runs in two phases per slice of at most 2^27 molecules, like ctraj_propagate:
+/* A batch with a lens runs as split kernels per slice of at most 2^27 molecules, like ctraj_propagate:
  * every molecule of the slice is drawn and walked to its lens entrance, the molecules entering a lens
- * are queued (64 B each, a batch-owned queue reused by every slice), then persistent wavefronts
- * integrate the queue with lane refill.  A lens whose force scale does not fold (see ctraj_propagate)
- * takes the plain RK4 step for its own molecules only.  fused = 1, or a batch whose queue could not be
- * allocated, runs one lane-refill kernel that draws, walks and integrates. */
+ * are queued (64 B each, a batch-owned queue reused by every slice), persistent wavefronts integrate
+ * the queue with lane refill and write each molecule back into its entry, and a streaming kernel walks
+ * the molecules that left their lens through the rest of their configuration and tallies.  A lens
+ * whose force scale does not fold (see ctraj_propagate) takes the plain RK4 step for its own molecules
+ * only.  fused = 1, or a batch whose queue could not be allocated, runs one lane-refill kernel that
+ * draws, walks and integrates. */
 typedef struct ctraj_batch_tune {
   int32_t block;     /* threads per block of the lens kernel: a multiple of 64 up to 1024 (0 = 1024); with
                       * fused: 64, 128, 192 or 256 (0 = 256) */
@@ -300,7 +302,7 @@ typedef struct ctraj_batch_tune {
   int32_t grid;      /* persistent blocks (0 = as many as are resident) */
   int32_t refill;    /* refill a wavefront when at least this many lanes are idle, 1..64 (0 = 8, fused 16) */
   int32_t use_graph; /* 1: capture the launches (counter and tally resets, kernels) once as a hipGraph and replay */
-  int32_t fused;     /* 1: the single-kernel walker instead of the two phases */
+  int32_t fused;     /* 1: the single-kernel walker instead of the split kernels */
 } ctraj_batch_tune;
 /* host arrays; the context's W' spline (ctraj_set_wspline, state 0) at creation is copied into the batch and
  * shared by all configurations */
@@ -310,7 +312,7 @@ int32_t ctraj_batch_create(ctraj_ctx *ctx, const ctraj_elem *elems, int32_t n_el
  * (may be NULL); elapsed_ms (host, may be NULL) is the device time of the launch measured with HIP events */
 int32_t ctraj_batch_run(ctraj_batch *b, const ctraj_batch_tune *tune, int64_t *tally_host, uint64_t *work_host,
                         float *elapsed_ms);
-/* 1 if the batch runs the two-phase path by default (fused = 0), 0 if it runs the single kernel */
+/* 1 if the batch runs the split kernels by default (fused = 0), 0 if it runs the single kernel */
 int32_t ctraj_batch_split_path(const ctraj_batch *b);
 int32_t ctraj_batch_destroy(ctraj_batch *b);
 
