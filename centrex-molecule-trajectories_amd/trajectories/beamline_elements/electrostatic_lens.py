@@ -77,6 +77,8 @@ class ElectrostaticLens(BeamlineElement):
     def save_to_hdf(self, filepath: Path, parent_group_path: str) -> None:
         from ..hdf import save_element
 
+        if self.stark is None and self.a_interp is None and hasattr(self.state, "stark_potential"):
+            self.radial_table()  # the state is stored as its repr: keep the table it defines
         save_element(self, filepath, parent_group_path, skip=("a_interp", "stark"), repr_keys=("state",))
 
     # ---------------------------------------------------------------- force tables

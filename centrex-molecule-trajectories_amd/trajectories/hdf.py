@@ -1,8 +1,16 @@
 """HDF5 output in the reference's layout (trajectory_simulator.py:160-254, molecule.py:97-106,169-185,
-apertures.py:56-80).  Requires h5py, which is optional: without it these raise ImportError."""
+apertures.py:56-80).  Requires h5py, which is optional: without it these raise ImportError.
+
+Beyond the reference's attributes, a lens group also holds its force tables as datasets (a
+``stark`` subgroup with one (E, W) knot pair per molecular state, an ``a_interp`` subgroup with the
+(r, a) knots) and the beamline group holds ``g``, so that a run read back with
+:mod:`trajectories.utils` propagates again.  Readers that look at attributes only, the reference's
+among them, see the reference's layout."""
 from __future__ import annotations
 
 from copy import copy
+
+import numpy as np
 
 
 def _h5py():
@@ -29,12 +37,40 @@ def save_element(el, filepath, parent_group_path, skip=(), repr_keys=()):
             if key in skip or value is None:
                 continue
             f[group_path].attrs[key] = repr(value) if key in repr_keys else value
+        if "stark" in skip and getattr(el, "stark", None) is not None:
+            _save_splines(f, group_path + "/stark", el.stark)
+        if "a_interp" in skip and getattr(el, "a_interp", None) is not None:
+            t = el.a_interp
+            if hasattr(t, "x") and hasattr(t, "y"):
+                f.create_group(group_path + "/a_interp")
+                f[group_path + "/a_interp"].create_dataset("x", data=np.asarray(t.x, dtype=np.float64))
+                f[group_path + "/a_interp"].create_dataset("y", data=np.asarray(t.y, dtype=np.float64))
+
+
+def _save_splines(f, path, stark):
+    """W(E) knots of each state's spline: datasets E_i, W_i; attrs count and is_list."""
+    splines = list(stark) if isinstance(stark, (list, tuple)) else [stark]
+    f.create_group(path)
+    f[path].attrs["count"] = len(splines)
+    f[path].attrs["is_list"] = isinstance(stark, (list, tuple))
+    for i, spl in enumerate(splines):
+        f[path].create_dataset(f"E_{i}", data=np.asarray(spl.E, dtype=np.float64))
+        f[path].create_dataset(f"W_{i}", data=np.asarray(spl.W, dtype=np.float64))
+
+
+def load_splines(group):
+    """The stark value stored by _save_splines: one WSpline or a list of them."""
+    from .stark import WSpline
+
+    out = [WSpline(group[f"E_{i}"][()], group[f"W_{i}"][()]) for i in range(int(group.attrs["count"]))]
+    return out if bool(group.attrs["is_list"]) else out[0]
 
 
 def save_beamline(beamline, filepath, run_name):
     h5py = _h5py()
     with h5py.File(filepath, "a") as f:
         f.create_group(run_name + "/beamline/")
+        f[run_name + "/beamline"].attrs["g"] = float(beamline.g)
     for element in beamline.elements:
         element.save_to_hdf(filepath, run_name + "/beamline/")
 

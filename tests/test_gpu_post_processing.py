@@ -139,3 +139,37 @@ def test_kept_molecules_with_lens_states_replay_their_state():
     ok = ~np.isnan(ref[0])
     np.testing.assert_array_equal(vals, ref[:, ok])
     np.testing.assert_array_equal(sel, np.nonzero(ok)[0])
+
+
+def test_post_processing_of_a_result_read_back_from_hdf(monkeypatch, tmp_path):
+    """The flow of the reference's lens_simulation_beamline_post_process.py:15-23: a run saved to
+    HDF5 and read back with trajectories.utils gives the same distributions at z as the run in
+    memory (h5py is absent here: the writer and readers run against the in-memory stand-in)."""
+    import sys
+
+    import h5stub
+    from trajectories.utils import import_sim_result_from_hdf
+
+    monkeypatch.setitem(sys.modules, "h5py", h5stub.make_module())
+    bl = configs.config1(stark=configs.stark_table(+1.0))
+    xdist, vdist = configs.source()
+    np.random.seed(95)
+    sim = TrajectorySimulator()
+    sim.run_simulation(bl, "pp", vdist=vdist, xdist=xdist, N_traj=20_000, apertures_of_interest=["Detected", "Inside lens"],
+                       n_jobs=1)
+    res = sim.result
+    mols = list(res.molecules)
+    assert len(mols) > 100
+    path = tmp_path / "lens_simulation_beamline.hdf"
+    res.save_to_hdf(path, "run")
+    back = import_sim_result_from_hdf(path, "run")
+    order = sorted(range(len(mols)), key=lambda i: f"molecule_{i}")
+    in_name_order = type("R", (), {"molecules": [mols[i] for i in order]})()
+    assert [m.aperture_hit for m in back.molecules] == [mols[i].aperture_hit for i in order]
+    for z in (back.beamline.find_element("Aperture 2").z0, 1.1234567, 2.5):
+        for elements in (None, ["Detected"]):
+            rho = PP.find_radial_pos_dist(back, z, elements)
+            np.testing.assert_array_equal(rho, PP.find_radial_pos_dist(in_name_order, z, elements))
+            np.testing.assert_array_equal(PP.find_vel_dist(back, z, elements),
+                                          PP.find_vel_dist(in_name_order, z, elements))
+            assert rho.shape[0] > 0 or z == 2.5
