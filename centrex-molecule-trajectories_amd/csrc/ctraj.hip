@@ -673,9 +673,10 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
 // walk().
 // ---------------------------------------------------------------------------------------------
 
-// Six wavefronts per SIMD (at most 80 VGPRs) for the hexapole RK4 instantiation, which fits them
-// without spilling (with one lens state, 256-thread workgroups and 24 KB of tables, LDS holds six
-// workgroups per CU); the others need more registers than that.
+// The hexapole RK4 instantiations are held to at most 80 VGPRs (six wavefronts per SIMD), which they
+// fit without spilling; they compile to 64, so the host's 1024-thread workgroups run two per CU
+// beside two copies of one state's 24 KB table, eight wavefronts per SIMD.  The others need more
+// registers than that.
 constexpr int REM_EMPTY = -1, REM_STOPPED = -2;
 
 // ORD: the wavefronts claim the list P.perm (longest-lived first) instead of the regions in order.
