@@ -621,7 +621,7 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
       } else {
         ref_step<FORCE>(F, T, s.wsb, s.h, m, s.status);
         ++s.i;
-        if (rho_of(m.x, m.y) > L0.p[0]) {
+        if (beyond_r2(m.x, m.y, L0.p[1])) {  // sqrt(x^2 + y^2) > d/2 (lens_ref)
           s.label = label_inside;
           s.fin = 1;
           s.phase = PH_NEED;
@@ -702,10 +702,11 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
   const int lane = lane_id();
   const ctraj_elem &L0 = B.elems[P.lens_e];
   const Force<FORCE> F = make_force<FORCE>(L0, B.g);
-  // the lens: z1, bore (r^2 for RK4, r for the reference), step (dt for RK4, dz for the reference),
-  // length; for RK4 the uniform step's half and sixth and, for a folding hexapole, the velocity
-  // constants ks Hs/2, ks Hs, ks Hs/6 come formed by the host in K, so all stay in scalar registers
-  const double R2 = L0.p[1], half_d = L0.p[0], z1 = L0.z1, Hs = L0.p[2], Ll = L0.p[3];
+  // the lens: z1, bore (x^2 + y^2 beyond R2 stops a molecule: (d/2)^2 for RK4, the squared-radius
+  // threshold of d/2 for the reference), step (dt for RK4, dz for the reference), length; for RK4
+  // the uniform step's half and sixth and, for a folding hexapole, the velocity constants ks Hs/2,
+  // ks Hs, ks Hs/6 come formed by the host in K, so all stay in scalar registers
+  const double R2 = L0.p[1], z1 = L0.z1, Hs = L0.p[2], Ll = L0.p[3];
   const int n_ref = L0.n_steps;
   uint32_t steps_w = 0, faces_w = 0;  // wave-uniform counts
   int64_t next = 0, end = 0;
@@ -829,7 +830,7 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
         rem = r2 > R2 ? REM_STOPPED : rem - 1;
       } else {
         ref_step<FORCE>(F, T, wsb, h, m, status);  // the row stores a = l1
-        rem = rho_of(m.x, m.y) > half_d ? REM_STOPPED : rem - 1;
+        rem = beyond_r2(m.x, m.y, R2) ? REM_STOPPED : rem - 1;  // R2: the threshold of d/2 (lens_ref)
       }
     }
   }
@@ -1060,8 +1061,8 @@ int check_tables(ctraj_ctx *ctx, const ctraj_elem *e, int n, int n_states) {
   return CTRAJ_OK;
 }
 
-// Device copy of an element table: an RTABLE lens carries its table's knot offset (table),
-// knot count (reserved) and inverse knot spacing (p[6]) instead of the table id.
+// The squared-radius threshold of a circular aperture of radius d: the largest double whose correctly
+// rounded square root is <= d (see beyond_r2).
 double circ_r2_threshold(double d) {
   if (std::isnan(d) || std::isinf(d)) return d;  // NaN: never beyond; +inf: never; -inf: always
   if (d < 0.0) return -1.0;                        // x^2 + y^2 >= 0 > -1: always beyond (NaN r2 never)
@@ -1075,15 +1076,24 @@ double circ_r2_threshold(double d) {
   return t;
 }
 
+// Device copy of an element table: a circular aperture carries its squared-radius threshold (p[1]);
+// so does a reference-mode lens for its bore test (its p[1] has no other use; configuration mode
+// compares x^2 + y^2 with (d/2)^2 itself); an RTABLE lens carries its table's knot offset (table),
+// knot count (reserved), inverse knot spacing (p[6]) and first and last knots (p[4], p[5], which
+// only the spline forces use) instead of the table id.
 void resolve_elems(const ctraj_ctx *ctx, const ctraj_elem *e, int n, ctraj_elem *out) {
   for (int i = 0; i < n; ++i) {
     out[i] = e[i];
     if (e[i].kind == CTRAJ_CIRC) out[i].p[1] = circ_r2_threshold(e[i].p[0]);
+    if (e[i].kind == CTRAJ_LENS && e[i].integrator == CTRAJ_INTEG_REF) out[i].p[1] = circ_r2_threshold(e[i].p[0]);
     if (e[i].kind == CTRAJ_LENS && e[i].force == CTRAJ_FORCE_RTABLE) {
       const int id = e[i].table;
       out[i].table = ctx->rt_off[id];
       out[i].reserved = ctx->rt_n[id];
       out[i].p[6] = ctx->rt_invh[id];
+      const std::vector<double> &tb = ctx->rt_host[id];
+      out[i].p[4] = tb.empty() ? 0.0 : tb[0];
+      out[i].p[5] = tb.empty() ? 0.0 : tb[tb.size() - 4];
     }
   }
 }

@@ -44,10 +44,11 @@ struct Tables {
 };
 
 // Where one r table sits in Tables::rt.  The host resolves a lens's table id into these before
-// launch: ctraj_elem.table = knot offset, .reserved = knot count, .p[6] = 1/(r_1 - r_0).
+// launch: ctraj_elem.table = knot offset, .reserved = knot count, .p[6] = 1/(r_1 - r_0), and the
+// first and last knots .p[4] = r_0, .p[5] = r_last (slots the spline forces use for their scales).
 struct RDesc {
   int off, n;
-  double invh;
+  double invh, r0, rlast;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -95,25 +96,37 @@ __device__ __forceinline__ bool beyond_r2(double x, double y, double thr2) { ret
 __device__ __forceinline__ double rtable_eval(const Tables &T, const RDesc &D, double r, uint32_t &status) {
   const lds_double *tb = T.rt + 4 * D.off;
   const int n = D.n;
-  const double rlast = tb[4 * (n - 1)];
-  if (r < tb[0]) {
+  if (r < D.r0) {
     status |= CTRAJ_ST_OOT;
     return tb[1];
   }
-  if (!(r < rlast)) {
-    if (r == rlast) return tb[4 * (n - 1) + 1];
+  if (!(r < D.rlast)) {
+    if (r == D.rlast) return tb[4 * (n - 1) + 1];
     if (r != r) return r;  // NaN propagates
     status |= CTRAJ_ST_OOT;
     return tb[4 * (n - 1) + 1];
   }
-  int j = (int)((r - tb[0]) * D.invh);
+  // Guess the segment from the first knot spacing, read its record (r_j, a_j, slope_j) and r_{j+1}
+  // and interpolate at once: one LDS round trip where the guess holds, which is every segment of a
+  // uniform grid except where rounding moves a knot across the guessed boundary; otherwise step to
+  // the segment with r_j <= r < r_{j+1} (unique for strictly increasing knots, np.interp's segment)
+  // and interpolate there.
+  int j = (int)((r - D.r0) * D.invh);
   j = j < 0 ? 0 : (j > n - 2 ? n - 2 : j);
-  while (j > 0 && r < tb[4 * j]) --j;
-  while (j < n - 2 && r >= tb[4 * (j + 1)]) ++j;
-  const double rj = tb[4 * j];
-  const double aj = tb[4 * j + 1];
-  if (r == rj) return aj;
-  return tb[4 * j + 2] * (r - rj) + aj;
+  const lds_dbl2 *rec = (const lds_dbl2 *)(tb + 4 * j);
+  const dbl2_t ra = rec[0];
+  const double slope = tb[4 * j + 2], rn = tb[4 * j + 4];
+  // all three reads in flight together (the compiler would otherwise defer the slope's read into
+  // the branch of r != r_j, a second round trip)
+  asm volatile("" : : "v"(ra.x), "v"(ra.y), "v"(slope), "v"(rn));
+  double res = r == ra.x ? ra.y : slope * (r - ra.x) + ra.y;
+  if (!((ra.x <= r) & (r < rn))) {
+    while (j > 0 && r < tb[4 * j]) --j;
+    while (j < n - 2 && r >= tb[4 * (j + 1)]) ++j;
+    const double rj = tb[4 * j], aj = tb[4 * j + 1];
+    res = r == rj ? aj : tb[4 * j + 2] * (r - rj) + aj;
+  }
+  return res;
 }
 
 // ElectrostaticLens.lens_acceleration (electrostatic_lens.py:168-228) with a tabulated a_r(r):
@@ -254,7 +267,7 @@ __device__ __forceinline__ Force<FORCE> make_force(const ctraj_elem &E, double g
 
 template <>
 __device__ __forceinline__ Force<CTRAJ_FORCE_RTABLE> make_force<CTRAJ_FORCE_RTABLE>(const ctraj_elem &E, double g) {
-  return Force<CTRAJ_FORCE_RTABLE>{RDesc{E.table, E.reserved, E.p[6]}, g};
+  return Force<CTRAJ_FORCE_RTABLE>{RDesc{E.table, E.reserved, E.p[6], E.p[4], E.p[5]}, g};
 }
 template <>
 __device__ __forceinline__ Force<CTRAJ_FORCE_WSPLINE_HEX> make_force<CTRAJ_FORCE_WSPLINE_HEX>(const ctraj_elem &E,
@@ -305,20 +318,22 @@ __device__ __forceinline__ void ref_step(const Force<FORCE> &F, const Tables &T,
 }
 
 // The reference's lens interior (electrostatic_lens.py:86-118): N = rint(L/dz) steps of dt = dz/vz,
-// bore test after each.  Returns false when the molecule leaves the bore.
+// bore test sqrt(x^2 + y^2) > d/2 after each, made as x^2 + y^2 > p[1] with the device copy's
+// squared-radius threshold of d/2 (beyond_r2; a reference-mode lens has no other use for p[1]).
+// Returns false when the molecule leaves the bore.
 template <int FORCE, class Sink>
 __device__ __forceinline__ bool lens_ref(const ctraj_elem &E, const Tables &T, int state, double g, MState &m,
                                          uint32_t &status, WalkCounters &wc, Sink &sink) {
   const Force<FORCE> F = make_force<FORCE>(E, g);
   const WsRef &wsb = ws_of(T, state);
-  const double half_d = E.p[0];
+  const double thr2 = E.p[1];
   const int N = E.n_steps;
   const double dt = E.p[2] / m.vz;
   for (int i = 0; i < N; ++i) {
     ref_step<FORCE>(F, T, wsb, dt, m, status);
     ++wc.steps;
     if constexpr (Sink::kRows) sink.row(m);
-    if (rho_of(m.x, m.y) > half_d) return false;
+    if (beyond_r2(m.x, m.y, thr2)) return false;
   }
   return true;
 }
