@@ -91,28 +91,19 @@ __device__ __forceinline__ bool beyond_r2(double x, double y, double thr2) { ret
 // ---------------------------------------------------------------------------------------------
 
 // np.interp on an ascending knot table (the interp1d of electrostatic_lens.py:209, evaluated at
-// :217): j with r_j <= r < r_{j+1}; r == r_j returns a_j; r == r_last returns a_last; r above the
-// table is flagged (the reference raises ValueError there) and returns a_last.
+// :217): j with r_j <= r < r_{j+1}; r == r_j returns a_j; r == r_last returns a_last; r below or
+// above the table is flagged (the reference raises ValueError there) and returns a_0 or a_last; a
+// NaN radius returns itself.
+// The common case runs without a branch: the segment is guessed from the first knot spacing (the
+// guess clamped to the table in floating point, so a NaN or infinite radius gives a valid index),
+// its record (r_j, a_j, slope_j) and r_{j+1} are read in one LDS round trip and the interpolation
+// is made; it stands when r_j <= r < r_{j+1}, which holds on every segment of a uniform grid except
+// where rounding moves a knot across the guessed boundary.  Everything else -- that case, radii off
+// the table, NaN -- takes one rarely entered branch that follows the rules above in order.
 __device__ __forceinline__ double rtable_eval(const Tables &T, const RDesc &D, double r, uint32_t &status) {
   const lds_double *tb = T.rt + 4 * D.off;
   const int n = D.n;
-  if (r < D.r0) {
-    status |= CTRAJ_ST_OOT;
-    return tb[1];
-  }
-  if (!(r < D.rlast)) {
-    if (r == D.rlast) return tb[4 * (n - 1) + 1];
-    if (r != r) return r;  // NaN propagates
-    status |= CTRAJ_ST_OOT;
-    return tb[4 * (n - 1) + 1];
-  }
-  // Guess the segment from the first knot spacing, read its record (r_j, a_j, slope_j) and r_{j+1}
-  // and interpolate at once: one LDS round trip where the guess holds, which is every segment of a
-  // uniform grid except where rounding moves a knot across the guessed boundary; otherwise step to
-  // the segment with r_j <= r < r_{j+1} (unique for strictly increasing knots, np.interp's segment)
-  // and interpolate there.
-  int j = (int)((r - D.r0) * D.invh);
-  j = j < 0 ? 0 : (j > n - 2 ? n - 2 : j);
+  int j = (int)__builtin_fmin(__builtin_fmax((r - D.r0) * D.invh, 0.0), (double)(n - 2));
   const lds_dbl2 *rec = (const lds_dbl2 *)(tb + 4 * j);
   const dbl2_t ra = rec[0];
   const double slope = tb[4 * j + 2], rn = tb[4 * j + 4];
@@ -121,27 +112,40 @@ __device__ __forceinline__ double rtable_eval(const Tables &T, const RDesc &D, d
   asm volatile("" : : "v"(ra.x), "v"(ra.y), "v"(slope), "v"(rn));
   double res = r == ra.x ? ra.y : slope * (r - ra.x) + ra.y;
   if (!((ra.x <= r) & (r < rn))) {
-    while (j > 0 && r < tb[4 * j]) --j;
-    while (j < n - 2 && r >= tb[4 * (j + 1)]) ++j;
-    const double rj = tb[4 * j], aj = tb[4 * j + 1];
-    res = r == rj ? aj : tb[4 * j + 2] * (r - rj) + aj;
+    if (r < D.r0) {
+      status |= CTRAJ_ST_OOT;
+      res = tb[1];
+    } else if (!(r < D.rlast)) {
+      if (r == D.rlast) {
+        res = tb[4 * (n - 1) + 1];
+      } else if (r != r) {
+        res = r;  // NaN propagates
+      } else {
+        status |= CTRAJ_ST_OOT;
+        res = tb[4 * (n - 1) + 1];
+      }
+    } else {  // inside the table, one knot off the guess
+      while (j > 0 && r < tb[4 * j]) --j;
+      while (j < n - 2 && r >= tb[4 * (j + 1)]) ++j;
+      const double rj = tb[4 * j], aj = tb[4 * j + 1];
+      res = r == rj ? aj : tb[4 * j + 2] * (r - rj) + aj;
+    }
   }
   return res;
 }
 
 // ElectrostaticLens.lens_acceleration (electrostatic_lens.py:168-228) with a tabulated a_r(r):
-// a = (a_r x / r, a_r y / r - g, 0), and (0, -g, 0) on axis.
+// a = (a_r x / r, a_r y / r - g, 0), and (0, -g, 0) on axis.  The quotients are formed whatever r
+// and replaced by 0 on axis (a select, not a branch).
 __device__ __forceinline__ void force_rtable(const Tables &T, const RDesc &D, double g, double x, double y,
                                              double &ax, double &ay, uint32_t &status) {
   const double r = rho_of(x, y);
   const double ar = rtable_eval(T, D, r, status);
-  double fx = 0.0, fy = 0.0;
-  if (r != 0.0) {
-    fx = (ar * x) / r;
-    fy = (ar * y) / r;
-  }
-  ax = fx;
-  ay = fy - g;
+  double fx = (ar * x) / r, fy = (ar * y) / r;
+  asm volatile("" : "+v"(fx), "+v"(fy));
+  const bool axis = r == 0.0;
+  ax = axis ? 0.0 : fx;
+  ay = (axis ? 0.0 : fy) - g;
 }
 
 // One lens state's W' spline in LDS.
