@@ -437,6 +437,36 @@ FLOP_PER_REF_STEP = 142  # reference-mode lens step: 4 a(r) evaluations (radius 
 #                          each), the reference's stage arithmetic on x, y, z and velocities, time, bore radius
 
 
+def bench_drop_in(ctx, n: int = 10_000_000) -> dict:
+    """The drop-in TrajectorySimulator.run_simulation end to end (wall clock, one call each after a
+    small warm-up) on the headline workload: molecules drawn on the host by numpy's global RNG in the
+    reference's order (then uploaded), against drawn on the device (device_seed)."""
+    from trajectories import configs
+    from trajectories.trajectory_simulator import TrajectorySimulator
+
+    bl = configs.config1()
+    xdist, vdist = configs.source()
+    keep = ["Detected"]
+    TrajectorySimulator().run_simulation(bl, "warm", vdist=vdist, xdist=xdist, N_traj=100_000,
+                                         apertures_of_interest=keep, ctx=ctx, device_seed=0)
+    np.random.seed(1)
+    t0 = time.time()
+    host = TrajectorySimulator()
+    host.run_simulation(bl, "host", vdist=vdist, xdist=xdist, N_traj=n, apertures_of_interest=keep, ctx=ctx)
+    t_host = time.time() - t0
+    t0 = time.time()
+    dev = TrajectorySimulator()
+    dev.run_simulation(bl, "dev", vdist=vdist, xdist=xdist, N_traj=n, apertures_of_interest=keep, ctx=ctx,
+                       device_seed=1)
+    t_dev = time.time() - t0
+    return {"workload": "TrajectorySimulator.run_simulation, ES-lens beamline, benchmark source, "
+                        "apertures_of_interest=['Detected'], wall clock of one call",
+            "molecules": n, "host_draw_s": t_host, "host_draw_molecules_per_s": n / t_host,
+            "device_seed_s": t_dev, "device_seed_molecules_per_s": n / t_dev,
+            "detected_host_draw": host.counter.counter_dict.get("Detected", 0),
+            "detected_device_seed": dev.counter.counter_dict.get("Detected", 0)}
+
+
 def bench_reference_mode(ctx, n: int = 1_000_000, reps: int = 10, cpu_seconds: float = 10.0, no_cpu: bool = False):
     """The reference's example beamline (examples/lens_simulation_beamline.py:18-72) with its own lens
     model: integrator 'ref' (N = rint(L/dz) steps of dt = dz/vz), linear-field a(r) table; the
@@ -529,7 +559,8 @@ def main():
         return
     if args.extras_only:
         print(json.dumps({"ballistic": bench_ballistic(ctx),
-                          "reference_mode": bench_reference_mode(ctx, no_cpu=args.no_cpu)}), flush=True)
+                          "reference_mode": bench_reference_mode(ctx, no_cpu=args.no_cpu),
+                          "run_simulation": bench_drop_in(ctx)}), flush=True)
         return
     if args.batch_only:
         print(json.dumps(bench_batch(ctx)), flush=True)
@@ -711,6 +742,7 @@ def main():
     if args.config == "lens" and not args.no_extras:
         line["ballistic"] = bench_ballistic(ctx)
         line["reference_mode"] = bench_reference_mode(ctx, cpu_seconds=args.cpu_seconds * 2 / 3, no_cpu=args.no_cpu)
+        line["run_simulation"] = bench_drop_in(ctx)
     if not args.no_cpu:
         cb = cpu_baseline(bl, X, V, args.cpu_seconds)
         line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample", "cpu_model",

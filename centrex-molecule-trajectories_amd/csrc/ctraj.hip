@@ -947,6 +947,18 @@ __global__ void __launch_bounds__(256) k_draw_u8(uint64_t seed, uint64_t offset,
   }
 }
 
+__global__ void __launch_bounds__(256) k_draw_disc(uint64_t seed, uint64_t offset, int64_t n, double radius,
+                                                    double z, double *x, double *y, double *zo) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double px, py;
+    disc_point(seed, (uint64_t)i + offset, radius, px, py);
+    x[i] = px;
+    y[i] = py;
+    if (zo) zo[i] = z;
+  }
+}
+
 struct GatherArgs {
   const double *src[8];
   double *dst[8];
@@ -1923,6 +1935,19 @@ int32_t ctraj_draw_uniform_u8(ctraj_ctx *c, uint64_t seed, uint64_t offset, int6
   HIPCHK(c, hipSetDevice(c->device));
   const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->n_cu * 16);
   hipLaunchKernelGGL(k_draw_u8, dim3(grid), dim3(256), 0, c->stream, seed, offset, n, (uint32_t)n_values, out);
+  HIPCHK(c, hipGetLastError());
+  return CTRAJ_OK;
+}
+
+int32_t ctraj_draw_disc(ctraj_ctx *c, uint64_t seed, uint64_t offset, int64_t n, double radius, double z, double *x,
+                        double *y, double *z_out) {
+  if (!c || ((!x || !y) && n > 0)) return fail(c, CTRAJ_EINVAL, "null argument");
+  if (n < 0) return fail(c, CTRAJ_EINVAL, "n < 0");
+  if (!(radius >= 0.0) || !std::isfinite(radius)) return fail(c, CTRAJ_EINVAL, "radius must be finite and >= 0");
+  if (n == 0) return CTRAJ_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->n_cu * 16);
+  hipLaunchKernelGGL(k_draw_disc, dim3(grid), dim3(256), 0, c->stream, seed, offset, n, radius, z, x, y, z_out);
   HIPCHK(c, hipGetLastError());
   return CTRAJ_OK;
 }

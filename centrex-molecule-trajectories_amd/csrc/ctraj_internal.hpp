@@ -137,4 +137,20 @@ __device__ inline void gaussian_pair(uint64_t seed, uint64_t i, uint32_t p, doub
   z1 = r * sin(th);
 }
 
+// One point of the uniform disc source for molecule i: counter (i_lo, i_hi, 'DISC', 'CTRA'), key
+// (seed_lo, seed_hi), uniforms as in gaussian_pair; theta = 2 pi u1, then r = radius sqrt(u2)
+// (CeNTREXPositionDistribution.draw's order, distributions.py:112-119).
+__device__ inline void disc_point(uint64_t seed, uint64_t i, double radius, double &x, double &y) {
+  uint32_t c[4] = {(uint32_t)i, (uint32_t)(i >> 32), 0x43534944u, 0x43545241u};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const uint64_t m1 = ((uint64_t)c[0] << 21) | (c[1] >> 11);
+  const uint64_t m2 = ((uint64_t)c[2] << 21) | (c[3] >> 11);
+  const double u1 = ((double)m1 + 0.5) * 0x1p-53;
+  const double u2 = ((double)m2 + 0.5) * 0x1p-53;
+  const double th = 6.283185307179586 * u1;
+  const double r = sqrt(u2) * radius;
+  x = r * cos(th);
+  y = r * sin(th);
+}
+
 }  // namespace ctraj

@@ -207,6 +207,8 @@ SIGNATURES = [
     ("ctraj_batch_destroy", C.c_int32, [C.c_void_p]),
     ("ctraj_batch_split_path", C.c_int32, [C.c_void_p]),
     ("ctraj_draw_uniform_u8", C.c_int32, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int64, C.c_int32, C.c_void_p]),
+    ("ctraj_draw_disc", C.c_int32, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int64, C.c_double, C.c_double,
+                                    C.c_void_p, C.c_void_p, C.c_void_p]),
     ("ctraj_draw_gaussian_at", C.c_int32,
      [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("ctraj_draw_uniform_u8_at", C.c_int32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),

@@ -36,10 +36,13 @@ class DeviceArray:
             self.ctx.sync()
         return self
 
-    def to_host(self, out: np.ndarray | None = None) -> np.ndarray:
+    def to_host(self, out: np.ndarray | None = None, count: int | None = None) -> np.ndarray:
+        """Copy the buffer (or its first ``count`` elements) to the host."""
+        k = self.n if count is None else max(0, min(int(count), self.n))
         if out is None:
-            out = np.empty(self.n, dtype=self.dtype)
-        self.ctx.memcpy(out.ctypes.data, self.ptr, self.nbytes, N.D2H)
+            out = np.empty(k, dtype=self.dtype)
+        if k:
+            self.ctx.memcpy(out.ctypes.data, self.ptr, k * self.dtype.itemsize, N.D2H)
         self.ctx.sync()
         return out
 

@@ -24,20 +24,31 @@ class Distribution(ABC):
 
         save_distribution(self, filepath, run_name)
 
-    # device form: (mean[3], sigma[3]) of independent Gaussians, or None if not Gaussian
+    # device forms: (mean[3], sigma[3]) of independent Gaussians, or None if not Gaussian
     def gaussian_params(self):
         return None
 
+    # (radius, z) of a uniform disc at fixed z, or None if not such a disc
+    def disc_params(self):
+        return None
+
     def draw_device(self, n: int, seed: int, offset: int = 0, ctx=None):
-        """Draw n samples on the device; returns three DeviceArrays (rows of draw(n))."""
+        """Draw n samples on the device from the counter-based streams of key ``seed`` (molecule
+        index + ``offset`` as the counter, so any chunking of a run draws the same molecules);
+        returns three DeviceArrays (rows of draw(n)).  The values follow the distribution, not the
+        host draw's numpy sequence."""
         from .device import get_context
-        from .engine import draw_gaussian
+        from .engine import draw_disc, draw_gaussian
 
         p = self.gaussian_params()
-        if p is None:
-            raise NotImplementedError(f"{type(self).__name__} has no device sampler")
-        mean, sigma = p
-        return draw_gaussian(ctx or get_context(), n, mean, sigma, seed, offset)
+        if p is not None:
+            mean, sigma = p
+            return draw_gaussian(ctx or get_context(), n, mean, sigma, seed, offset)
+        q = self.disc_params()
+        if q is not None:
+            radius, z = q
+            return draw_disc(ctx or get_context(), n, radius, z, seed, offset)
+        raise NotImplementedError(f"{type(self).__name__} has no device sampler")
 
 
 @dataclass
@@ -93,6 +104,9 @@ class CeNTREXPositionDistribution(Distribution):
         theta = uniform.rvs(size=n) * 2 * np.pi
         r = np.sqrt(uniform.rvs(size=n)) * self.d / 2
         return np.array((r * np.cos(theta), r * np.sin(theta), np.full(n, self.z)))
+
+    def disc_params(self):
+        return (self.d / 2, float(self.z))
 
 
 @dataclass

@@ -110,17 +110,29 @@ class Engine:
 
     def propagate(self, x, y, z, vx, vy, vz, t=None, a0=None, state=None, *, outputs: bool = True,
                   want_accel: bool = False, aoi_labels=(), index_base: int = 0, block: int = 0, flags: int = 0,
-                  refill: int = 0, noalloc: bool = False) -> dict:
+                  refill: int = 0, noalloc: bool = False, per_molecule: bool = True) -> dict:
         """Propagate host arrays; returns labels, tallies, work counters and (optionally) final states."""
-        ctx = self.ctx
         n, inp, a0d, st = self._upload_inputs(x, y, z, vx, vy, vz, t, a0, state)
+        return self.propagate_device(n, inp, a0d, st, outputs=outputs, want_accel=want_accel, aoi_labels=aoi_labels,
+                                     index_base=index_base, block=block, flags=flags, refill=refill, noalloc=noalloc,
+                                     per_molecule=per_molecule)
+
+    def propagate_device(self, n: int, inp: dict, a0=None, state=None, *, outputs: bool = True,
+                         want_accel: bool = False, aoi_labels=(), index_base: int = 0, block: int = 0, flags: int = 0,
+                         refill: int = 0, noalloc: bool = False, per_molecule: bool = True) -> dict:
+        """propagate() on inputs already in device buffers (``inp``: x, y, z, vx, vy, vz[, t] DeviceArrays
+        of at least n entries; ``state`` a device uint8 array or None).  ``per_molecule=False`` asks
+        for no per-molecule label and status (the result then has neither; the work counters still
+        count out-of-table and step-capped molecules)."""
+        ctx = self.ctx
+        a0d, st = a0, state
         res = {"n": n, "labels": list(self.cb.labels)}
         nl, ns = self.cb.n_labels, self.cb.n_states
         tally = ctx.alloc(ns * nl, np.int64).fill(0)
         first = ctx.alloc(nl, np.int64).fill(np.iinfo(np.int64).max)
         work = ctx.alloc(N.WORK_N, np.uint64).fill(0)
-        label = ctx.alloc(max(n, 1), np.int32)
-        status = ctx.alloc(max(n, 1), np.uint8)
+        label = ctx.alloc(max(n, 1), np.int32) if per_molecule else None
+        status = ctx.alloc(max(n, 1), np.uint8) if per_molecule else None
         out = {k: ctx.alloc(max(n, 1)) for k in FIELDS} if outputs else None
         out_a = [ctx.alloc(max(n, 1)) for _ in range(3)] if want_accel else None
         mask = 0
@@ -143,17 +155,18 @@ class Engine:
         res["tally"] = tally.to_host().reshape(ns, nl)
         res["first_index"] = first.to_host()
         res["work"] = work.to_host()
-        res["label"] = label.to_host()[:n]
-        res["status"] = status.to_host()[:n]
+        if per_molecule:
+            res["label"] = label.to_host(count=n)
+            res["status"] = status.to_host(count=n)
         if outputs:
             res["out"] = {k: v.to_host()[:n] for k, v in out.items()}
         if want_accel:
             res["out_a"] = [v.to_host()[:n] for v in out_a]
         if mask:
             cnt = int(aoi["count"].to_host()[0])
-            res["aoi_index"] = aoi["index"].to_host()[:cnt]
-            res["aoi_label"] = aoi["label"].to_host()[:cnt]
-            res["aoi_out"] = {k: v.to_host()[:cnt] for k, v in aoi["out"].items()}
+            res["aoi_index"] = aoi["index"].to_host(count=cnt)
+            res["aoi_label"] = aoi["label"].to_host(count=cnt)
+            res["aoi_out"] = {k: v.to_host(count=cnt) for k, v in aoi["out"].items()}
         return res
 
     def _record(self, mode, x, y, z, vx, vy, vz, t, a0, state, cap, z_start=0.0, z_step=0.0):
@@ -223,6 +236,17 @@ def draw_gaussian(ctx: Context, n: int, mean, sigma, seed: int, offset: int = 0,
     N.check("ctraj_draw_gaussian",
             N.lib().ctraj_draw_gaussian(ctx.handle, C.c_uint64(seed), C.c_uint64(offset), int(n), nd,
                                         mean.ctypes.data, sigma.ctypes.data, C.cast(ptrs, C.c_void_p)),
+            ctx.handle)
+    return out
+
+
+def draw_disc(ctx: Context, n: int, radius: float, z: float, seed: int, offset: int = 0, out=None) -> list:
+    """Device uniform-disc source (CeNTREXPositionDistribution.draw's form, Philox4x32-10): [x, y, z]."""
+    if out is None:
+        out = [ctx.alloc(max(int(n), 1)) for _ in range(3)]
+    N.check("ctraj_draw_disc",
+            N.lib().ctraj_draw_disc(ctx.handle, C.c_uint64(seed), C.c_uint64(offset), int(n), float(radius), float(z),
+                                    N.addr(out[0]), N.addr(out[1]), N.addr(out[2]) if len(out) > 2 else None),
             ctx.handle)
     return out
 

@@ -157,12 +157,20 @@ class TrajectorySimulator:
     def run_simulation(self, beamline: Beamline, run_name: str, vdist=CeNTREXVelocityDistribution(),
                        xdist=CeNTREXPositionDistribution(), N_traj: int = 1000, apertures_of_interest=[],
                        n_jobs=1, *, ctx=None, chunk: int = 1 << 22, block: int = 0, states=None,
-                       raise_out_of_table: bool = True):
+                       raise_out_of_table: bool = True, device_seed: int | None = None):
         """Simulate ``int(N_traj/(100*n_jobs))*100*n_jobs`` molecules (trajectory_simulator.py:34-103).
 
         ``n_jobs`` only sets the reference's chunking (and hence which molecules are drawn); the
-        work runs on the GPU.  ``states`` (extension) gives each molecule a lens state id."""
+        work runs on the GPU.  ``states`` (extension) gives each molecule a lens state id.
+        ``device_seed`` (extension) draws the molecules on the device from counter-based streams
+        (``simulate_device``) instead of replaying numpy's global RNG on the host: the same
+        distributions, other samples, and no host draw or upload."""
         N_loops, N, total = reference_chunking(N_traj, n_jobs)
+        if device_seed is not None:
+            res = simulate_device(beamline, xdist, vdist, total, device_seed, apertures_of_interest, ctx=ctx,
+                                  chunk=chunk, block=block, states=states, raise_out_of_table=raise_out_of_table)
+            self._store(run_name, beamline, xdist, vdist, res)
+            return
         vs, xs = [], []
         for _ in range(N_loops):
             vs.append(np.asarray(vdist.draw(N), dtype=np.float64).reshape(3, N))
@@ -172,6 +180,9 @@ class TrajectorySimulator:
         res = simulate_arrays(beamline, X, V, apertures_of_interest, ctx=ctx, chunk=chunk, block=block,
                               states=states, raise_out_of_table=raise_out_of_table)
         assert res.n == total
+        self._store(run_name, beamline, xdist, vdist, res)
+
+    def _store(self, run_name, beamline, xdist, vdist, res):
         self.counter = res.counter
         self.result = SimulationResult(self.counter, beamline, xdist, vdist, res.molecules)
         self.result.run = res
@@ -213,17 +224,32 @@ def simulate_arrays(beamline: Beamline, X: np.ndarray, V: np.ndarray, apertures_
             break
         r = eng.propagate(X[0, s:e], X[1, s:e], X[2, s:e], V[0, s:e], V[1, s:e], V[2, s:e],
                           None if t0 is None else t0[s:e], state=None if states is None else states[s:e],
-                          outputs=False, aoi_labels=apertures_of_interest, index_base=s, block=block)
+                          outputs=False, aoi_labels=apertures_of_interest, index_base=s, block=block,
+                          per_molecule=False)
         tally += r["tally"]
         first = np.minimum(first, r["first_index"])
         work += r["work"]
-        n_oot += int(np.count_nonzero(r["status"] & N.ST_OOT))
-        n_capped += int(np.count_nonzero(r["status"] & N.ST_CAPPED))
+        n_oot += int(r["work"][N.WORK_OOT])
+        n_capped += int(r["work"][N.WORK_CAPPED])
         if "aoi_index" in r:
             o = np.argsort(r["aoi_index"], kind="stable")
             aoi_idx.append(r["aoi_index"][o])
             aoi_lab.append(r["aoi_label"][o])
             aoi_fin.append({k: v[o] for k, v in r["aoi_out"].items()})
+    idx = np.concatenate(aoi_idx) if aoi_idx else np.zeros(0, np.int64)
+    init = np.zeros((7, idx.size))
+    if idx.size:
+        init[0:3] = X[:, idx]
+        init[3:6] = V[:, idx]
+        init[6] = 0.0 if t0 is None else t0[idx]
+    return _run_output(eng, beamline, n, tally, first, work, n_oot, n_capped, aoi_idx, aoi_lab, aoi_fin, init,
+                       states, raise_out_of_table)
+
+
+def _run_output(eng, beamline, n, tally, first, work, n_oot, n_capped, aoi_idx, aoi_lab, aoi_fin, init, states,
+                raise_out_of_table) -> RunOutput:
+    """The RunOutput of a chunked run: the out-of-table error or step-cap warning, the Counter, and the
+    kept molecules (global indices aoi_idx, initial conditions init [7][k]) as a MoleculeList."""
     if n_oot and raise_out_of_table:
         raise ValueError(f"{n_oot} molecules evaluated the lens force outside its table "
                          "(the reference's interp1d raises ValueError here)")
@@ -234,17 +260,72 @@ def simulate_arrays(beamline: Beamline, X: np.ndarray, V: np.ndarray, apertures_
     lab = np.concatenate(aoi_lab) if aoi_lab else np.zeros(0, np.int32)
     fin = {k: np.concatenate([d[k] for d in aoi_fin]) for k in aoi_fin[0]} if aoi_fin else {}
     counter = Counter.from_tallies(eng.cb.labels, tally.sum(axis=0), first)
-    init = np.zeros((7, idx.size))
-    if idx.size:
-        init[0:3] = X[:, idx]
-        init[3:6] = V[:, idx]
-        init[6] = 0.0 if t0 is None else t0[idx]
-    vz = V[2, idx] if idx.size else np.array([1.0])
+    vz = init[5] if idx.size else np.array([1.0])
     vzmin = float(vz[vz > 0].min()) if np.any(vz > 0) else 1.0
     cap = 10 + sum(el.rows_needed(vzmin) if hasattr(el, "rows_needed") else el.N_steps() for el in beamline.elements)
     kept_states = None if states is None else np.asarray(states, dtype=np.uint8)[idx]
     mols = MoleculeList(eng, init, lab, cap, states=kept_states)
     return RunOutput(n, list(eng.cb.labels), tally, first, counter, idx, lab, fin, mols, work, n_oot, n_capped)
+
+
+def device_stream_keys(seed: int) -> tuple[int, int]:
+    """Keys of the position and velocity streams of a device-drawn run: 2 seed and 2 seed + 1 (mod 2^64),
+    so that the two never share a stream even where both are Gaussian."""
+    k = (2 * int(seed)) % (1 << 64)
+    return k, (k + 1) % (1 << 64)
+
+
+def simulate_device(beamline: Beamline, xdist: Distribution, vdist: Distribution, n: int, seed: int,
+                    apertures_of_interest=(), *, ctx=None, chunk: int = 1 << 22, block: int = 0, states=None,
+                    raise_out_of_table=True, engine: Engine | None = None) -> RunOutput:
+    """Propagate n molecules drawn on the device: positions by ``xdist.draw_device`` and velocities by
+    ``vdist.draw_device`` under the keys of ``device_stream_keys(seed)``, molecule i taking counter i in
+    both, so the molecules do not depend on ``chunk``.  The inputs never leave the device; the kept
+    molecules' initial conditions are gathered there.  Results are those of ``simulate_arrays`` on the
+    same draws copied to the host."""
+    from .engine import FIELDS, gather
+
+    eng = engine or Engine(beamline, ctx, n_states=None if states is None else int(np.max(states)) + 1)
+    ctx = eng.ctx
+    kx, kv = device_stream_keys(seed)
+    nl, ns = eng.cb.n_labels, eng.cb.n_states
+    tally = np.zeros((ns, nl), np.int64)
+    first = np.full(nl, np.iinfo(np.int64).max, np.int64)
+    work = np.zeros(8, np.uint64)
+    aoi_idx, aoi_lab, aoi_fin, inits = [], [], [], []
+    n_oot = n_capped = 0
+    for s in range(0, n, chunk):
+        m = min(chunk, n - s)
+        src = [*xdist.draw_device(m, kx, offset=s, ctx=ctx), *vdist.draw_device(m, kv, offset=s, ctx=ctx)]
+        st = None
+        if states is not None:
+            sh = np.ascontiguousarray(states[s:s + m], dtype=np.uint8)
+            if m and int(sh.max()) >= ns:
+                raise ValueError("state id >= n_states")
+            st = ctx.to_device(sh)
+        r = eng.propagate_device(m, dict(zip(FIELDS[:6], src)), None, st, outputs=False,
+                                 aoi_labels=apertures_of_interest, index_base=s, block=block, per_molecule=False)
+        tally += r["tally"]
+        first = np.minimum(first, r["first_index"])
+        work += r["work"]
+        n_oot += int(r["work"][N.WORK_OOT])
+        n_capped += int(r["work"][N.WORK_CAPPED])
+        if "aoi_index" in r:
+            o = np.argsort(r["aoi_index"], kind="stable")
+            idx = np.ascontiguousarray(r["aoi_index"][o], dtype=np.int64)
+            aoi_idx.append(idx)
+            aoi_lab.append(r["aoi_label"][o])
+            aoi_fin.append({k: v[o] for k, v in r["aoi_out"].items()})
+            if idx.size:
+                dst = [ctx.alloc(idx.size) for _ in range(6)]
+                gather(ctx, idx.size, ctx.to_device(idx), s, m, src, dst)
+                inits.append(np.array([d.to_host()[: idx.size] for d in dst]))
+    k = sum(a.size for a in aoi_idx)
+    init = np.zeros((7, k))
+    if k:
+        init[0:6] = np.concatenate(inits, axis=1)
+    return _run_output(eng, beamline, n, tally, first, work, n_oot, n_capped, aoi_idx, aoi_lab, aoi_fin, init,
+                       states, raise_out_of_table)
 
 
 @dataclass

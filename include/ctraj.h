@@ -256,6 +256,13 @@ int32_t ctraj_draw_gaussian(ctraj_ctx *ctx, uint64_t seed, uint64_t offset, int6
 int32_t ctraj_draw_uniform_u8(ctraj_ctx *ctx, uint64_t seed, uint64_t offset, int64_t n, int32_t n_values,
                               uint8_t *out);
 
+/* CeNTREXPositionDistribution.draw (distributions.py:112-119) on the device: molecule i (+ offset) at
+ * (r cos theta, r sin theta, z) with theta = 2 pi u1 and r = radius sqrt(u2), u1 and u2 the 53-bit uniforms of
+ * Philox4x32-10 counter (i_lo, i_hi, 'DISC' = 0x43534944, 'CTRA') and key (seed_lo, seed_hi), formed as in
+ * ctraj_draw_gaussian.  x, y, z_out are device [n]; a NULL z_out skips z.  radius must be finite and >= 0. */
+int32_t ctraj_draw_disc(ctraj_ctx *ctx, uint64_t seed, uint64_t offset, int64_t n, double radius, double z,
+                        double *x, double *y, double *z_out);
+
 /* index-list forms of the two draws: output slot j holds molecule index[j] (instead of j + offset) of the same
  * streams, so any subset of a large run can be regenerated on the device (device index[n], out as above) */
 int32_t ctraj_draw_gaussian_at(ctraj_ctx *ctx, uint64_t seed, const int64_t *index, int64_t n, int32_t n_dims,

@@ -569,6 +569,26 @@ void orc_draw_uniform_u8(uint64_t seed, uint64_t offset, int64_t n, int n_values
   }
 }
 
+/* ctraj_draw_disc: CeNTREXPositionDistribution.draw (distributions.py:112-119) from counter
+ * (i_lo, i_hi, 'DISC', 'CTRA'): theta = 2 pi u1, r = radius sqrt(u2), uniforms as in orc_draw_gaussian */
+void orc_draw_disc(uint64_t seed, uint64_t offset, int64_t n, double radius, double z, double *x, double *y,
+                   double *zo) {
+  for (int64_t i = 0; i < n; ++i) {
+    const uint64_t k = (uint64_t)i + offset;
+    uint32_t c[4] = {(uint32_t)k, (uint32_t)(k >> 32), 0x43534944u, 0x43545241u};
+    orc_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint64_t m1 = ((uint64_t)c[0] << 21) | (c[1] >> 11);
+    const uint64_t m2 = ((uint64_t)c[2] << 21) | (c[3] >> 11);
+    const double u1 = ((double)m1 + 0.5) * 0x1p-53;
+    const double u2 = ((double)m2 + 0.5) * 0x1p-53;
+    const double th = 6.283185307179586 * u1;
+    const double r = sqrt(u2) * radius;
+    x[i] = r * cos(th);
+    y[i] = r * sin(th);
+    if (zo) zo[i] = z;
+  }
+}
+
 int orc_abi_version(void) { return CTRAJ_ABI_VERSION; }
 
 /*

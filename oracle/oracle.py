@@ -45,6 +45,8 @@ def lib():
         L.orc_draw_gaussian.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_int, vp, vp, vp, vp]
         L.orc_draw_uniform_u8.restype = None
         L.orc_draw_uniform_u8.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_int, vp]
+        L.orc_draw_disc.restype = None
+        L.orc_draw_disc.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_double, C.c_double, vp, vp, vp]
         L.orc_smooth.restype = C.c_int
         L.orc_smooth.argtypes = [vp, C.c_int, C.c_double, vp, C.c_int, C.c_int64, vp, vp, vp, vp, vp, C.c_double,
                                  C.c_double, C.c_double, C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, vp, C.c_int]
@@ -121,6 +123,14 @@ def draw_gaussian(n, mean, sigma, seed, offset=0, raw=False):
     lib().orc_draw_gaussian(C.c_uint64(seed), C.c_uint64(offset), int(n), nd, mean.ctypes.data, sigma.ctypes.data,
                             _ptrs(outs), None if rawbuf is None else rawbuf.ctypes.data)
     return (outs, rawbuf) if raw else outs
+
+
+def draw_disc(n, radius, z, seed, offset=0):
+    """[x, y, z] of ctraj_draw_disc's uniform-disc source."""
+    outs = [np.empty(n) for _ in range(3)]
+    lib().orc_draw_disc(C.c_uint64(seed), C.c_uint64(offset), int(n), float(radius), float(z), outs[0].ctypes.data,
+                        outs[1].ctypes.data, outs[2].ctypes.data)
+    return outs
 
 
 def draw_uniform_u8(n, n_values, seed, offset=0):

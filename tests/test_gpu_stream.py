@@ -109,3 +109,62 @@ def test_molecule_major_zgrid_is_the_field_major_transposed(ctx):
     same = (mm == want) | (np.isnan(mm) & np.isnan(want))
     assert same.all()
     assert np.isfinite(mm).any() and np.isnan(mm).any()
+
+
+def test_disc_source_matches_oracle(ctx):
+    """ctraj_draw_disc (CeNTREXPositionDistribution on the device) against the oracle's restatement:
+    the Philox words are the same, so the points agree to the last bits that the two math libraries'
+    sqrt/cos/sin may differ in."""
+    from trajectories.engine import draw_disc
+
+    n, R, z0 = 100_003, 0.01, 0.00635
+    dev = [a.to_host()[:n] for a in draw_disc(ctx, n, R, z0, seed=2024, offset=99)]
+    ref = O.draw_disc(n, R, z0, 2024, offset=99)
+    for d in range(2):
+        np.testing.assert_allclose(dev[d], ref[d], rtol=1e-13, atol=1e-13 * R)
+    np.testing.assert_array_equal(dev[2], ref[2])
+    assert np.all(dev[0] ** 2 + dev[1] ** 2 <= R * R * (1 + 1e-15))
+
+
+def test_run_simulation_on_device_draws_equals_the_host_arrays_path(ctx):
+    """run_simulation(device_seed=...) draws and propagates on the device; the same molecules, drawn
+    with the same keys and copied to the host, give the same Counter, kept molecules and trajectories
+    through simulate_arrays, whatever the chunk size."""
+    from trajectories.trajectory_simulator import TrajectorySimulator, device_stream_keys, simulate_arrays
+
+    from trajectories.distributions import CeNTREXPositionDistribution
+
+    bl = configs.config1()
+    xdist, vdist = CeNTREXPositionDistribution(), configs.source()[1]  # the reference's default disc source
+    n_traj = 250_000
+    sim = TrajectorySimulator()
+    keep = ["Detected", "Aperture 2"]
+    sim.run_simulation(bl, "dev", vdist=vdist, xdist=xdist, N_traj=n_traj, apertures_of_interest=keep,
+                       device_seed=31, chunk=1 << 16)
+    res = sim.result
+    n = res.run.n
+    assert n == n_traj
+    kx, kv = device_stream_keys(31)
+    X = np.array([a.to_host()[:n] for a in xdist.draw_device(n, kx, ctx=ctx)])
+    V = np.array([a.to_host()[:n] for a in vdist.draw_device(n, kv, ctx=ctx)])
+    host = simulate_arrays(bl, X, V, keep, ctx=ctx)
+    assert list(res.counter.counter_dict.items()) == list(host.counter.counter_dict.items())
+    np.testing.assert_array_equal(res.run.tally, host.tally)
+    np.testing.assert_array_equal(res.run.aoi_index, host.aoi_index)
+    np.testing.assert_array_equal(res.run.aoi_label, host.aoi_label)
+    assert len(res.molecules) == len(host.molecules) > 20
+    np.testing.assert_array_equal(res.molecules._init, host.molecules._init)
+    for j in (0, len(res.molecules) // 2, len(res.molecules) - 1):
+        a, b = res.molecules[j], host.molecules[j]
+        assert a.aperture_hit == b.aperture_hit
+        np.testing.assert_array_equal(a.trajectory.x, b.trajectory.x)
+        np.testing.assert_array_equal(a.trajectory.v, b.trajectory.v)
+    # another chunking draws the same molecules
+    sim2 = TrajectorySimulator()
+    sim2.run_simulation(bl, "dev2", vdist=vdist, xdist=xdist, N_traj=n_traj, apertures_of_interest=keep,
+                        device_seed=31, chunk=100_000)
+    np.testing.assert_array_equal(sim2.result.run.tally, res.run.tally)
+    np.testing.assert_array_equal(sim2.result.run.aoi_index, res.run.aoi_index)
+    # the source follows the distributions: vz ~ N(184, 16), positions inside the 2 cm source disc
+    assert abs(V[2].mean() - vdist.vz) < 0.2 and abs(V[2].std() - vdist.sigmaz) < 0.2
+    assert np.all(X[0] ** 2 + X[1] ** 2 <= (xdist.d / 2) ** 2 * (1 + 1e-15)) and np.all(X[2] == xdist.z)

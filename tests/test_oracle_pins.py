@@ -330,6 +330,32 @@ def test_gaussian_source_moments():
     assert abs(outs[1].mean() - 184.0) < 0.1
 
 
+def test_disc_source_from_its_philox_words_and_uniform():
+    """The oracle's uniform-disc source (ctraj_draw_disc's restatement): each point is theta = 2 pi u1,
+    r = R sqrt(u2) from the Philox words of counter (i, 0, 'DISC', 'CTRA'), and the points are uniform
+    over the disc (r^2/R^2 and theta/2pi uniform, independent)."""
+    R, z0, seed, off = 0.01, 0.00635, 0x1234_5678_9ABC_DEF0, 5
+    x, y, z = O.draw_disc(200_000, R, z0, seed, offset=off)
+    assert np.all(z == z0)
+    for i in (0, 1, 77, 199_999):
+        c = O.philox(np.array([i + off, 0, 0x43534944, 0x43545241], dtype=np.uint32),
+                     (seed & 0xFFFFFFFF, seed >> 32)).astype(np.uint64)
+        u1 = (float((c[0] << np.uint64(21)) | (c[1] >> np.uint64(11))) + 0.5) * 2.0 ** -53
+        u2 = (float((c[2] << np.uint64(21)) | (c[3] >> np.uint64(11))) + 0.5) * 2.0 ** -53
+        r, th = np.sqrt(u2) * R, 2 * np.pi * u1
+        assert x[i] == pytest.approx(r * np.cos(th), rel=1e-14, abs=1e-18)
+        assert y[i] == pytest.approx(r * np.sin(th), rel=1e-14, abs=1e-18)
+    q = (x * x + y * y) / R ** 2
+    a = (np.arctan2(y, x) / (2 * np.pi)) % 1.0
+    assert np.all(q <= 1.0)
+    for u in (q, a):
+        assert abs(u.mean() - 0.5) < 0.003 and abs(u.var() - 1 / 12) < 0.002
+    assert abs(np.corrcoef(q, a)[0, 1]) < 0.01
+    # the offset shifts the counter: molecule i of offset k is molecule i + k of offset 0
+    x0, y0, _ = O.draw_disc(10, R, z0, seed, offset=off + 3)
+    np.testing.assert_array_equal(x0[:5], x[3:8])
+
+
 def test_pyref_matches_oracle_reference_mode():
     """The numpy restatement (the reference's arithmetic) and the C oracle agree to rounding on a
     reference-mode beamline (apertures, reference lens with its a(r) table, field plates)."""
