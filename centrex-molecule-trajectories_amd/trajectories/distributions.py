@@ -32,22 +32,22 @@ class Distribution(ABC):
     def disc_params(self):
         return None
 
-    def draw_device(self, n: int, seed: int, offset: int = 0, ctx=None):
+    def draw_device(self, n: int, seed: int, offset: int = 0, ctx=None, out=None):
         """Draw n samples on the device from the counter-based streams of key ``seed`` (molecule
         index + ``offset`` as the counter, so any chunking of a run draws the same molecules);
-        returns three DeviceArrays (rows of draw(n)).  The values follow the distribution, not the
-        host draw's numpy sequence."""
+        returns three DeviceArrays (rows of draw(n); ``out``, three arrays of at least n, to fill
+        existing ones).  The values follow the distribution, not the host draw's numpy sequence."""
         from .device import get_context
         from .engine import draw_disc, draw_gaussian
 
         p = self.gaussian_params()
         if p is not None:
             mean, sigma = p
-            return draw_gaussian(ctx or get_context(), n, mean, sigma, seed, offset)
+            return draw_gaussian(ctx or get_context(), n, mean, sigma, seed, offset, out=out)
         q = self.disc_params()
         if q is not None:
             radius, z = q
-            return draw_disc(ctx or get_context(), n, radius, z, seed, offset)
+            return draw_disc(ctx or get_context(), n, radius, z, seed, offset, out=out)
         raise NotImplementedError(f"{type(self).__name__} has no device sampler")
 
 

@@ -53,6 +53,22 @@ class Engine:
             self.cb = compile_elements(beamline.elements, beamline.g, n_states=n_states,
                                        max_lens_steps=max_lens_steps)
         bind(self.ctx, self.cb)
+        self._scratch = {}
+
+    def _buf(self, name: str, n: int, dtype=np.float64) -> DeviceArray:
+        """A device buffer of at least n elements kept for the host-array calls of this engine (each
+        call copies its results to the host before it returns, so the next call may reuse it)."""
+        b = self._scratch.get(name)
+        if b is None or b.n < n or b.dtype != np.dtype(dtype):
+            b = self.ctx.alloc(max(int(n), 1), dtype)
+            self._scratch[name] = b
+        return b
+
+    def free_scratch(self) -> None:
+        """Release the buffers kept by _buf (a chunked run calls this when it is done)."""
+        for b in self._scratch.values():
+            b.free()
+        self._scratch.clear()
 
     @property
     def labels(self):
@@ -128,23 +144,23 @@ class Engine:
         a0d, st = a0, state
         res = {"n": n, "labels": list(self.cb.labels)}
         nl, ns = self.cb.n_labels, self.cb.n_states
-        tally = ctx.alloc(ns * nl, np.int64).fill(0)
-        first = ctx.alloc(nl, np.int64).fill(np.iinfo(np.int64).max)
-        work = ctx.alloc(N.WORK_N, np.uint64).fill(0)
-        label = ctx.alloc(max(n, 1), np.int32) if per_molecule else None
-        status = ctx.alloc(max(n, 1), np.uint8) if per_molecule else None
-        out = {k: ctx.alloc(max(n, 1)) for k in FIELDS} if outputs else None
-        out_a = [ctx.alloc(max(n, 1)) for _ in range(3)] if want_accel else None
+        tally = self._buf("tally", ns * nl, np.int64).fill(0)
+        first = self._buf("first", nl, np.int64).fill(np.iinfo(np.int64).max)
+        work = self._buf("work", N.WORK_N, np.uint64).fill(0)
+        label = self._buf("label", n, np.int32) if per_molecule else None
+        status = self._buf("status", n, np.uint8) if per_molecule else None
+        out = {k: self._buf("out_" + k, n) for k in FIELDS} if outputs else None
+        out_a = [self._buf(f"out_a{k}", n) for k in range(3)] if want_accel else None
         mask = 0
         for name in aoi_labels:
             if name in self.cb.labels:
                 mask |= 1 << self.cb.labels.index(name)
         aoi = {}
         if mask:
-            aoi["count"] = ctx.alloc(1, np.int64).fill(0)
-            aoi["index"] = ctx.alloc(max(n, 1), np.int64)
-            aoi["label"] = ctx.alloc(max(n, 1), np.int32)
-            aoi["out"] = {k: ctx.alloc(max(n, 1)) for k in FIELDS}
+            aoi["count"] = self._buf("aoi_count", 1, np.int64).fill(0)
+            aoi["index"] = self._buf("aoi_index", n, np.int64)
+            aoi["label"] = self._buf("aoi_label", n, np.int32)
+            aoi["out"] = {k: self._buf("aoi_out_" + k, n) for k in FIELDS}
         if n:
             self.launch(inp, out=out, out_a=out_a, label=label, status=status, tally=tally, first_index=first,
                         aoi_mask=mask, aoi_cap=n if mask else 0, aoi_count=aoi.get("count"),
@@ -152,18 +168,18 @@ class Engine:
                         work=work, a0=a0d, state=st,
                         n=n, index_base=index_base, block=block, flags=flags, refill=refill, noalloc=noalloc)
         ctx.sync()
-        res["tally"] = tally.to_host().reshape(ns, nl)
-        res["first_index"] = first.to_host()
-        res["work"] = work.to_host()
+        res["tally"] = tally.to_host(count=ns * nl).reshape(ns, nl)
+        res["first_index"] = first.to_host(count=nl)
+        res["work"] = work.to_host(count=N.WORK_N)
         if per_molecule:
             res["label"] = label.to_host(count=n)
             res["status"] = status.to_host(count=n)
         if outputs:
-            res["out"] = {k: v.to_host()[:n] for k, v in out.items()}
+            res["out"] = {k: v.to_host(count=n) for k, v in out.items()}
         if want_accel:
-            res["out_a"] = [v.to_host()[:n] for v in out_a]
+            res["out_a"] = [v.to_host(count=n) for v in out_a]
         if mask:
-            cnt = int(aoi["count"].to_host()[0])
+            cnt = int(aoi["count"].to_host(count=1)[0])
             res["aoi_index"] = aoi["index"].to_host(count=cnt)
             res["aoi_label"] = aoi["label"].to_host(count=cnt)
             res["aoi_out"] = {k: v.to_host(count=cnt) for k, v in aoi["out"].items()}

@@ -236,6 +236,7 @@ def simulate_arrays(beamline: Beamline, X: np.ndarray, V: np.ndarray, apertures_
             aoi_idx.append(r["aoi_index"][o])
             aoi_lab.append(r["aoi_label"][o])
             aoi_fin.append({k: v[o] for k, v in r["aoi_out"].items()})
+    eng.free_scratch()
     idx = np.concatenate(aoi_idx) if aoi_idx else np.zeros(0, np.int64)
     init = np.zeros((7, idx.size))
     if idx.size:
@@ -294,9 +295,11 @@ def simulate_device(beamline: Beamline, xdist: Distribution, vdist: Distribution
     work = np.zeros(8, np.uint64)
     aoi_idx, aoi_lab, aoi_fin, inits = [], [], [], []
     n_oot = n_capped = 0
+    bufs = [ctx.alloc(max(1, min(chunk, n))) for _ in range(6)]
     for s in range(0, n, chunk):
         m = min(chunk, n - s)
-        src = [*xdist.draw_device(m, kx, offset=s, ctx=ctx), *vdist.draw_device(m, kv, offset=s, ctx=ctx)]
+        src = [*xdist.draw_device(m, kx, offset=s, ctx=ctx, out=bufs[:3]),
+               *vdist.draw_device(m, kv, offset=s, ctx=ctx, out=bufs[3:])]
         st = None
         if states is not None:
             sh = np.ascontiguousarray(states[s:s + m], dtype=np.uint8)
@@ -320,6 +323,9 @@ def simulate_device(beamline: Beamline, xdist: Distribution, vdist: Distribution
                 dst = [ctx.alloc(idx.size) for _ in range(6)]
                 gather(ctx, idx.size, ctx.to_device(idx), s, m, src, dst)
                 inits.append(np.array([d.to_host()[: idx.size] for d in dst]))
+    eng.free_scratch()
+    for b in bufs:
+        b.free()
     k = sum(a.size for a in aoi_idx)
     init = np.zeros((7, k))
     if k:
