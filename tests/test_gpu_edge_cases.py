@@ -177,6 +177,11 @@ def test_invalid_tables_and_arguments_return_einval_before_any_launch(ctx):
     off2 = ctx.to_device(np.array([0, 3, 6], np.int64))
     S.row_off = off2.ptr
     expect_einval(L.ctraj_sample_rows(ctx.handle, C.byref(S)), "row offsets past n_rows")
+    xs, ys = ctx.alloc(4), ctx.alloc(4)
+    for radius in (-1.0, float("nan"), float("inf")):
+        expect_einval(L.ctraj_draw_disc(ctx.handle, 1, 0, 4, radius, 0.0, xs.ptr, ys.ptr, None), f"disc radius {radius}")
+    expect_einval(L.ctraj_draw_disc(ctx.handle, 1, 0, -1, 0.01, 0.0, xs.ptr, ys.ptr, None), "disc n < 0")
+    expect_einval(L.ctraj_draw_disc(ctx.handle, 1, 0, 4, 0.01, 0.0, None, ys.ptr, None), "disc without x")
     # the valid beamline set before is still in force
     ctx.bound = None
     X, V = configs.draw_source(3000, seed=61)
