@@ -168,3 +168,27 @@ def test_run_simulation_on_device_draws_equals_the_host_arrays_path(ctx):
     # the source follows the distributions: vz ~ N(184, 16), positions inside the 2 cm source disc
     assert abs(V[2].mean() - vdist.vz) < 0.2 and abs(V[2].std() - vdist.sigmaz) < 0.2
     assert np.all(X[0] ** 2 + X[1] ** 2 <= (xdist.d / 2) ** 2 * (1 + 1e-15)) and np.all(X[2] == xdist.z)
+
+
+def test_device_drawn_run_with_lens_states_equals_the_host_arrays_path(ctx):
+    """simulate_device with per-molecule lens states (config 4's four tables): the same tallies per
+    state and the same kept molecules as simulate_arrays on the same draws and states."""
+    from trajectories.trajectory_simulator import device_stream_keys, simulate_arrays, simulate_device
+
+    bl = configs.config4()
+    xdist, vdist = configs.source()
+    n = 120_000
+    states = np.random.RandomState(8).randint(0, 4, n).astype(np.uint8)
+    keep = ["Detected", "Inside lens"]
+    dev = simulate_device(bl, xdist, vdist, n, 5, keep, ctx=ctx, chunk=50_000, states=states)
+    kx, kv = device_stream_keys(5)
+    X = np.array([a.to_host()[:n] for a in xdist.draw_device(n, kx, ctx=ctx)])
+    V = np.array([a.to_host()[:n] for a in vdist.draw_device(n, kv, ctx=ctx)])
+    host = simulate_arrays(bl, X, V, keep, ctx=ctx, states=states)
+    np.testing.assert_array_equal(dev.tally, host.tally)
+    assert np.count_nonzero(dev.tally.sum(axis=1)) == 4
+    np.testing.assert_array_equal(dev.aoi_index, host.aoi_index)
+    np.testing.assert_array_equal(dev.molecules._init, host.molecules._init)
+    np.testing.assert_array_equal(dev.molecules._states, host.molecules._states)
+    for k in dev.aoi_final:
+        np.testing.assert_array_equal(dev.aoi_final[k], host.aoi_final[k])
