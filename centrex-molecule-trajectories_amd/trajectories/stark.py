@@ -8,6 +8,11 @@ provided instead:
 * :class:`TwoLevelStark` -- the closed form W(E) = sign * (sqrt((h nu/2)^2 + (a mu E)^2) - h nu/2).
   ``sign=-1`` is the literal form of the benchmark configuration (a high-field seeker).  It also
   works as the lens ``state`` for the reference's own a(r)-table construction.
+* :class:`HamiltonianStark` -- the reference's own construction (stark_potential.py:9-90) for a
+  Hamiltonian H(E) = H0 + E HE given as matrices: a sweep of diagonalisations over |E| with the
+  eigenvectors relabelled adiabatically (reorder_evecs) and the state picked by its overlap with a
+  given state vector.  With the TlF Hamiltonian (which needs centrex_TlF to build) this is the
+  reference's table; with any other Hamiltonian it is the same algorithm.
 * :class:`WSpline` -- a not-a-knot cubic spline of W on a uniform |E| grid, stored per segment as
   the coefficients of W'(E) = (A s + B) s + C in the segment coordinate s = E/dE - k.  This is the
   table the device stages in LDS.
@@ -17,6 +22,7 @@ from __future__ import annotations
 import numpy as np
 
 H_PLANCK = 6.62607015e-34  # J s
+HBAR = H_PLANCK / (2.0 * np.pi)  # J s
 C_LIGHT = 299792458.0  # m/s
 DEBYE = 1e-21 / C_LIGHT  # C m
 
@@ -48,6 +54,59 @@ class TwoLevelStark:
     def __repr__(self):
         return (f"TwoLevelStark(nu={self.nu!r}, mu_debye={self.mu_debye!r}, sign={self.sign!r}, "
                 f"dipole_scale={self.dipole_scale!r})")
+
+
+def reorder_evecs(V_in: np.ndarray, E_in: np.ndarray, V_ref: np.ndarray):
+    """Relabel one diagonalisation adiabatically (stark_potential.py:70-90): eigenvector i of V_in
+    (columns) is matched to the column of V_ref it overlaps most, and the eigenpairs are put in
+    the order of their matches, so that index k keeps labelling the same state along a sweep."""
+    overlap = np.abs(V_in.conj().T @ V_ref)
+    index = np.argsort(np.argmax(overlap, axis=1))
+    return E_in[index], V_in[:, index]
+
+
+def closest_vector_index(state_vector: np.ndarray, V: np.ndarray) -> int:
+    """The column of V (eigenvectors) with the largest overlap |<v|state>| with the state vector."""
+    return int(np.argmax(np.abs(V.conj().T @ np.asarray(state_vector))))
+
+
+class HamiltonianStark:
+    """Stark potential of one eigenstate of H(E) = H0 + E HE, E along z in V/cm, H in rad/s, by the
+    reference's sweep (stark_potential.py:9-67): the eigenvectors at ``E_ref`` (100 V/cm in the
+    reference) are the initial labels and the state is the one closest to ``state_vector`` there;
+    each field of a sweep is diagonalised (numpy's eigh, as the reference) and relabelled against the
+    previous one; the potential is the state's eigenvalue times hbar (J).
+
+    ``H0`` may include a small magnetic term to split degeneracies, as the reference's H(E, B) does.
+    ``stark_potential(E_V_per_cm)`` takes the fields in sweep order (the labels follow the path);
+    ``energy(E_V_per_m)`` is the same for :meth:`WSpline.from_stark`."""
+
+    def __init__(self, H0, HE, state_vector, E_ref: float = 100.0):
+        self.H0 = np.asarray(H0)
+        self.HE = np.asarray(HE)
+        if self.H0.ndim != 2 or self.H0.shape[0] != self.H0.shape[1] or self.HE.shape != self.H0.shape:
+            raise ValueError("H0 and HE must be square matrices of one shape")
+        self.E_ref = float(E_ref)
+        self.state_vector = np.asarray(state_vector)
+        _, self.V_ref0 = np.linalg.eigh(self.H0 + self.E_ref * self.HE)
+        self.index = closest_vector_index(self.state_vector, self.V_ref0)
+
+    def stark_potential(self, E_V_per_cm):
+        Ezs = np.atleast_1d(np.asarray(E_V_per_cm, dtype=np.float64))
+        D, V = np.linalg.eigh(self.H0[None, :, :] + Ezs[:, None, None] * self.HE[None, :, :])
+        out = np.empty(Ezs.size)
+        V_ref = self.V_ref0
+        for i in range(Ezs.size):
+            energies, V_ref = reorder_evecs(V[i], D[i], V_ref)
+            out[i] = energies[self.index]
+        out *= HBAR
+        return out.reshape(np.shape(E_V_per_cm))
+
+    def energy(self, E):
+        return self.stark_potential(np.asarray(E, dtype=np.float64) / 100.0)
+
+    def __repr__(self):
+        return f"HamiltonianStark(dim={self.H0.shape[0]}, state_index={self.index}, E_ref={self.E_ref!r})"
 
 
 def not_a_knot_second_derivatives(y: np.ndarray, h: float) -> np.ndarray:

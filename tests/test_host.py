@@ -223,3 +223,58 @@ def test_library_is_newer_than_its_sources():
     srcs = list((ROOT / "centrex-molecule-trajectories_amd" / "csrc").glob("*")) + [ROOT / "include" / "ctraj.h"]
     newest = max(p.stat().st_mtime for p in srcs)
     assert so.exists() and so.stat().st_mtime >= newest - 1.0
+
+
+def _two_level_hamiltonian(nu=6.67e9, mu_debye=4.23):
+    from trajectories.stark import HBAR
+
+    w = 2 * np.pi * nu
+    c = mu_debye * DEBYE * 100.0 / HBAR  # rad/s per V/cm
+    return np.diag([w / 2, -w / 2]), np.array([[0.0, c], [c, 0.0]])
+
+
+@pytest.mark.parametrize("sign", [+1.0, -1.0])
+def test_hamiltonian_stark_sweep_reproduces_the_two_level_closed_form(sign):
+    """The reference's adiabatic sweep (stark_potential.py:9-90) on the two-level Hamiltonian
+    H(E) = diag(+w/2, -w/2) + E mu (off-diagonal): the tracked state's energy minus its zero-field
+    value is the closed form of TwoLevelStark (upper state: sign +1, lower: -1)."""
+    from trajectories.stark import HamiltonianStark
+
+    H0, HE = _two_level_hamiltonian()
+    st = HamiltonianStark(H0, HE, [1.0, 0.0] if sign > 0 else [0.0, 1.0])
+    Ezs = np.linspace(0.0, 70_000.0, 701)  # V/cm, as the reference sweeps
+    V = st.stark_potential(Ezs)
+    want = TwoLevelStark(sign=sign).energy(Ezs * 100.0)
+    np.testing.assert_allclose(V - V[0], want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
+    np.testing.assert_allclose(st.energy(Ezs * 100.0), V, rtol=0, atol=0)
+
+
+def test_hamiltonian_stark_follows_diabatic_levels_through_crossings():
+    """Uncoupled levels that cross: the overlap relabelling (reorder_evecs) keeps each level on its
+    own straight line through every crossing, where sorted eigenvalues would switch lines."""
+    from trajectories.stark import HBAR, HamiltonianStark
+
+    H0 = np.diag([0.0, 10.0, 30.0])
+    HE = np.diag([1.0, 0.5, -0.2])
+    Ezs = np.linspace(0.3, 99.7, 200)
+    for i in range(3):
+        st = HamiltonianStark(H0, HE, np.eye(3)[i])
+        np.testing.assert_allclose(st.stark_potential(Ezs) / HBAR, H0[i, i] + HE[i, i] * Ezs, rtol=1e-12, atol=1e-12)
+    sorted_low = np.linalg.eigvalsh(H0[None] + Ezs[:, None, None] * HE[None])[:, 0]
+    assert not np.allclose(sorted_low, H0[0, 0] + HE[0, 0] * Ezs)  # the lowest level is not one line
+
+
+def test_lens_tables_from_the_hamiltonian_sweep_equal_the_two_level_ones():
+    """A lens whose state is the two-level Hamiltonian sweep builds the a(r) table (electrostatic_lens.py:
+    193-206) and the W' spline of the TwoLevelStark closed form, to the eigensolver's rounding."""
+    from trajectories.stark import HamiltonianStark
+
+    H0, HE = _two_level_hamiltonian()
+    ham = HamiltonianStark(H0, HE, [1.0, 0.0])
+    r1, a1 = ElectrostaticLens("L", 0.8, 0.6, state=ham).radial_table()
+    r2, a2 = ElectrostaticLens("L", 0.8, 0.6, state=TwoLevelStark(sign=1.0)).radial_table()
+    np.testing.assert_array_equal(r1, r2)
+    np.testing.assert_allclose(a1, a2, rtol=1e-7, atol=1e-9 * np.abs(a2).max())
+    s1 = WSpline.from_stark(ham, 100e3, 201)
+    s2 = WSpline.from_stark(TwoLevelStark(sign=1.0), 100e3, 201)
+    np.testing.assert_allclose(s1.coef[:, :3], s2.coef[:, :3], rtol=1e-6, atol=1e-9 * np.abs(s2.coef).max())
