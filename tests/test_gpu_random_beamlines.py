@@ -1,0 +1,90 @@
+"""GPU parity on random beamlines: every walker against the CPU oracle, bit for bit.
+
+Each case draws a beamline from a fixed seed -- circular and rectangular apertures and field
+plates at random positions and sizes, thin or with length, around one electrostatic lens in either
+model (configuration mode: hexapole or linear field, W' spline of either Stark sign, random V, bore,
+length and dt; reference mode: the a(r) table built from the two-level form, random dz) -- and a
+source of random spread, then compares final phase space, stored acceleration, labels, status bits,
+tallies and work counters of the default split path, the single-kernel lane walker and the
+one-molecule-per-thread walker with the oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from trajectories import _native as N
+from trajectories.beamline import Beamline
+from trajectories.beamline_elements import CircularAperture, ElectrostaticLens, FieldPlates, RectangularAperture
+from trajectories.engine import Engine
+from trajectories.stark import TwoLevelStark, WSpline
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "y", "z", "vx", "vy", "vz", "t")
+
+
+def random_beamline(seed: int):
+    rs = np.random.RandomState(seed)
+    z = 0.05
+    els = []
+
+    def aperture(name):
+        nonlocal z
+        kind = rs.randint(3)
+        L = 0.0 if rs.rand() < 0.5 else rs.uniform(1e-3, 0.05)
+        if kind == 0:
+            el = CircularAperture(name, z, L, x0=rs.uniform(-1e-3, 1e-3), y0=rs.uniform(-1e-3, 1e-3),
+                                  d=rs.uniform(4e-3, 30e-3))
+        elif kind == 1:
+            el = RectangularAperture(name, z, L, x0=rs.uniform(-1e-3, 1e-3), y0=rs.uniform(-1e-3, 1e-3),
+                                     w=rs.uniform(4e-3, 30e-3), h=rs.uniform(4e-3, 30e-3))
+        else:
+            el = FieldPlates(name, z, rs.uniform(0.05, 0.6), x0=rs.uniform(-1e-3, 1e-3), w=rs.uniform(6e-3, 30e-3))
+        z = el.z0 + el.L + rs.uniform(0.01, 0.5)
+        return el
+
+    for k in range(rs.randint(0, 3)):
+        els.append(aperture(f"before {k}"))
+    ref = rs.rand() < 0.4
+    L = rs.uniform(0.2, 0.8)
+    if ref:
+        lens = ElectrostaticLens("ES lens", z, L, d=rs.uniform(0.02, 0.05), V=rs.uniform(5e3, 40e3),
+                                 dz=rs.uniform(0.5e-3, 2e-3), state=TwoLevelStark(sign=rs.choice([-1.0, 1.0])))
+    else:
+        stark = WSpline.from_stark(TwoLevelStark(sign=rs.choice([-1.0, 1.0]), dipole_scale=rs.uniform(0.5, 1.0)),
+                                   100e3, 1001)
+        lens = ElectrostaticLens("ES lens", z, L, d=rs.uniform(0.02, 0.05), V=rs.uniform(5e3, 40e3), stark=stark,
+                                 field=rs.choice(["hexapole", "linear"]), integrator="rk4",
+                                 dt=rs.uniform(5e-6, 20e-6))
+    els.append(lens)
+    z = lens.z1 + rs.uniform(0.05, 0.5)
+    for k in range(rs.randint(1, 4)):
+        els.append(aperture(f"after {k}"))
+    order = rs.permutation(len(els))  # the Beamline sorts them by z0 (stable)
+    bl = Beamline([els[i] for i in order], 9.80665)
+    n = 12_000 if ref else 20_000
+    sx, sv = rs.uniform(1e-3, 5e-3), rs.uniform(1.0, 20.0)
+    X = np.vstack([rs.normal(0, sx, n), rs.normal(0, sx, n), np.zeros(n)])
+    V = np.vstack([rs.normal(0, sv, n), rs.normal(0, sv, n), rs.normal(184.0, 20.0, n)])
+    return bl, X, V, ref
+
+
+@pytest.mark.parametrize("seed", list(range(40)))
+def test_random_beamline_every_walker_equals_the_oracle(seed):
+    bl, X, V, ref = random_beamline(1000 + seed)
+    eng = Engine(bl)
+    orc = O.propagate(eng.cb, X[0], X[1], X[2], V[0], V[1], V[2])
+    labels = np.bincount(orc["label"], minlength=len(eng.labels))
+    assert labels[eng.cb.label_detected] + labels[eng.labels.index("Inside lens")] > 0, "the lens is reached"
+    for flags in (0, N.RUN_LANES, N.RUN_SIMPLE, N.RUN_QUEUE_ORDER):
+        dev = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], want_accel=True, flags=flags)
+        tag = f"seed {seed} ({'reference' if ref else 'configuration'} mode) flags={flags}"
+        np.testing.assert_array_equal(dev["label"], orc["label"], err_msg=tag)
+        np.testing.assert_array_equal(dev["status"], orc["status"], err_msg=tag)
+        for k in FIELDS:
+            np.testing.assert_array_equal(dev["out"][k], orc["out"][k], err_msg=f"{tag} {k}")
+        for q in range(3):
+            np.testing.assert_array_equal(dev["out_a"][q], orc["out_a"][q], err_msg=f"{tag} a{q}")
+        np.testing.assert_array_equal(dev["tally"][0], labels, err_msg=tag)
+        assert int(dev["work"][N.WORK_FACES]) == int(orc["work"][0]), tag
+        assert int(dev["work"][N.WORK_STEPS]) == int(orc["work"][1]), tag
