@@ -1,14 +1,15 @@
 // batch.hip -- many beamline configurations in one persistent, graph-replayed launch (gfx950).
 //
 // The molecules of all configurations form one index space [0, total): configuration c owns
-// [begin_c, begin_c + n_c).  Wavefronts claim `chunk` indices at a time from one global queue
-// (an atomic counter reset by a fill kernel inside the captured graph), so long and short
-// configurations are load-balanced by the same queue.  The lane-refill walker of the ordinary
-// propagation is used with per-lane configuration state: a lane finds its configuration by a
-// binary search of the begin offsets (LDS), draws its molecule from that configuration's Philox
-// stream (the ctraj_draw_gaussian stream), walks the configuration's elements before the lens,
-// integrates the lens with the configuration's field constants held in its own registers, walks
-// the elements after the lens and tallies (configuration, label) in a workgroup LDS histogram.
+// [begin_c, begin_c + n_c); a molecule finds its configuration by a binary search of the begin
+// offsets (LDS) and draws itself from that configuration's Philox stream (the ctraj_draw_gaussian
+// stream).  By default the scan runs as split kernels per slice of at most 2^27 molecules, like the
+// single-run path: k_batch_pre draws, walks each molecule to and through its configuration's lens
+// entrance and queues the lens molecules; k_batch_integ integrates the queue with lane refill and
+// the configuration's lens constants per lane; k_batch_post walks the molecules that left their
+// lens through the rest of their configuration.  k_batch_lanes does all of it in one lane-refill
+// kernel, wavefronts claiming `chunk` indices at a time from one global counter (reset by a fill
+// kernel inside the captured graph).  Tallies are (configuration, label) histograms in LDS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

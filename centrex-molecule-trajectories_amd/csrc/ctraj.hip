@@ -1,12 +1,18 @@
 // ctraj.hip -- kernels and C ABI of libctraj.so (gfx950).
 //
 // Kernels:
-//   k_propagate  fused beamline walker: one thread per molecule, grid-stride over a SoA store,
-//                lens tables in LDS, wave-aggregated tallies and aoi compaction.
-//   k_record     the same walker re-run for selected molecules with a row / z-grid sink.
-//   k_draw       Philox4x32-10 + Box-Muller Gaussian source.
-//   k_batch      persistent walker over (configuration, chunk) work items (batch.hip).
-//   k_smooth_*   differentiable flux and its adjoint (adjoint.hip).
+//   k_propagate        streaming beamline walker, one thread per molecule, grid-stride over a SoA
+//                      store: the whole beamline (FULL), or the split path's walk to and through the
+//                      lens entrance that queues lens molecules (PRE) and its walk after the lens
+//                      (POST); wave-aggregated tallies and kept-molecule compaction.
+//   k_ord_scan/scatter the split path's claim list, longest predicted lens time first (ctraj_order.hpp).
+//   k_lens             the split path's lens integration: persistent wavefronts with lane refill.
+//   k_propagate_lanes  the single-kernel lane-refill walker (one lens).
+//   k_record           the walker re-run for selected molecules with a row / z-grid sink.
+//   k_draw*            Philox4x32-10 device sources (Gaussian, uniform ids, uniform disc).
+//   k_gather_*         kept-molecule bookkeeping.
+// Elsewhere: the batched scan (batch.hip), the smoothed flux and its adjoint (adjoint.hip), the
+// radix sort (sort.hip), the resampling of stored rows (resample.hip).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
