@@ -163,3 +163,62 @@ def test_batch_beyond_one_slice_equals_per_configuration_propagation(ctx):
     t1, w1, _ = scan.run(T(block=256, chunk=128, grid=0, refill=16, use_graph=0, fused=1))
     np.testing.assert_array_equal(t1, tally)
     np.testing.assert_array_equal(w1[:4], work[:4])
+
+
+def random_configurations(seed: int, n_cfg: int = 24):
+    """Configurations with rectangular apertures and field plates besides circular ones, lenses of
+    random voltage, position, bore, length and dt (hexapole spline, RK4), one of them without a lens."""
+    from trajectories.beamline_elements import ElectrostaticLens, FieldPlates, RectangularAperture
+
+    rs = np.random.RandomState(seed)
+    stark = configs.stark_table()
+    bls = []
+    for c in range(n_cfg):
+        z, els = 0.05, []
+
+        def aperture(name):
+            nonlocal z
+            kind = rs.randint(3)
+            L = 0.0 if rs.rand() < 0.5 else rs.uniform(1e-3, 0.05)
+            if kind == 0:
+                el = CircularAperture(name, z, L, d=rs.uniform(4e-3, 30e-3))
+            elif kind == 1:
+                el = RectangularAperture(name, z, L, x0=rs.uniform(-1e-3, 1e-3), y0=rs.uniform(-1e-3, 1e-3),
+                                         w=rs.uniform(4e-3, 30e-3), h=rs.uniform(4e-3, 30e-3))
+            else:
+                el = FieldPlates(name, z, rs.uniform(0.05, 0.6), x0=rs.uniform(-1e-3, 1e-3), w=rs.uniform(6e-3, 30e-3))
+            z = el.z0 + el.L + rs.uniform(0.01, 0.4)
+            return el
+
+        for k in range(rs.randint(0, 3)):
+            els.append(aperture(f"before {k}"))
+        if c != 3:
+            lens = ElectrostaticLens("ES lens", z, rs.uniform(0.2, 0.8), d=rs.uniform(0.02, 0.05),
+                                     V=rs.uniform(5e3, 40e3), stark=stark, field="hexapole", integrator="rk4",
+                                     dt=10e-6)
+            els.append(lens)
+            z = lens.z1 + rs.uniform(0.05, 0.4)
+        for k in range(rs.randint(1, 4)):
+            els.append(aperture(f"after {k}"))
+        bls.append(Beamline(els, configs.G))
+    n = rs.randint(0, 30_000, n_cfg)
+    n[5] = 0
+    seeds = rs.randint(0, 2 ** 31, n_cfg).astype(np.uint64)
+    return bls, n, seeds
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_random_configurations_with_all_element_kinds_equal_per_configuration_runs(ctx, fused):
+    """The general (all element kinds) instantiations of the batched kernels, split and fused:
+    per-configuration tallies equal ctraj_propagate and the oracle on the same device draws."""
+    bls, n, seeds = random_configurations(77)
+    scan = BatchScan(bls, n, seeds, configs.SOURCE_MEAN, configs.SOURCE_SIGMA, ctx=ctx)
+    T = scan.tune.__class__
+    tally, work, ms = scan.run(T(block=256 if fused else 0, chunk=128, grid=0, refill=8 if not fused else 16,
+                                 use_graph=0, fused=int(fused)))
+    assert tally.sum() == n.sum()
+    dev, orc = per_config_reference(ctx, scan, bls, n, seeds)
+    for c in range(len(bls)):
+        got = as_dict(scan, tally, c)
+        assert got == {k: v for k, v in dev[c].items() if v}, (c, got, dev[c])
+        assert got == {k: v for k, v in orc[c].items() if v}, (c, got, orc[c])
