@@ -6,7 +6,8 @@ model (configuration mode: hexapole or linear field, W' spline of either Stark s
 length and dt; reference mode: the a(r) table built from the two-level form, random dz) -- and a
 source of random spread, then compares final phase space, stored acceleration, labels, status bits,
 tallies and work counters of the default split path, the single-kernel lane walker and the
-one-molecule-per-thread walker with the oracle.
+one-molecule-per-thread walker with the oracle; on every fourth seed it also compares the record
+kernel's trajectory rows and z-grid samples.
 """
 import numpy as np
 import pytest
@@ -88,3 +89,26 @@ def test_random_beamline_every_walker_equals_the_oracle(seed):
         np.testing.assert_array_equal(dev["tally"][0], labels, err_msg=tag)
         assert int(dev["work"][N.WORK_FACES]) == int(orc["work"][0]), tag
         assert int(dev["work"][N.WORK_STEPS]) == int(orc["work"][1]), tag
+
+
+@pytest.mark.parametrize("seed", list(range(0, 40, 4)))
+def test_random_beamline_trajectory_rows_and_zgrid_equal_the_oracle(seed):
+    """The record kernel on random beamlines: every trajectory row (the rows the reference's
+    Trajectory would hold) and the 1-mm z-grid samples of the post-processing rule equal the
+    oracle's, bit for bit."""
+    bl, X, V, ref = random_beamline(1000 + seed)
+    n = 800
+    X, V = X[:, :n], V[:, :n]
+    eng = Engine(bl)
+    rows, nrows, lab = eng.record_rows(X[0], X[1], X[2], V[0], V[1], V[2], cap=96)  # grows to the longest
+    orc = O.propagate(eng.cb, X[0], X[1], X[2], V[0], V[1], V[2], rows_cap=rows.shape[1])
+    np.testing.assert_array_equal(lab, orc["label"])
+    np.testing.assert_array_equal(nrows, orc["n_rows"])
+    np.testing.assert_array_equal(rows, orc["rows"])
+    K = int(np.ceil(max(el.z1 for el in bl.elements) / 1e-3)) + 50
+    samp, written, lab2 = eng.zgrid(X[0], X[1], X[2], V[0], V[1], V[2], z_start=0.0, z_step=1e-3, n_samples=K)
+    np.testing.assert_array_equal(lab2, orc["label"])
+    for i in range(0, n, 37):
+        ref_s = O.zgrid_from_rows(orc["rows"][i], int(orc["n_rows"][i]), 0.0, 1e-3, K)
+        np.testing.assert_array_equal(samp[:, :, i], ref_s, err_msg=f"seed {seed} molecule {i}")
+        assert written[i] == np.count_nonzero(~np.isnan(ref_s[0]))
