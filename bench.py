@@ -668,8 +668,10 @@ def main():
         roof = {"bound": "mfma", "engine": ENGINE, "achieved": ach_lens, "peak": PEAK_FP64, "unit": "TFLOP/s",
                 "frac": ach_lens / PEAK_FP64, "traffic": traffic}
         # the other two kernels stream HBM: PRE reads 48 B of inputs per molecule and queues 64 B per
-        # molecule entering the lens; the lens kernel rewrites each entry in place; POST reads the
-        # entries back and writes each kept molecule (8 B index + 7 fields)
+        # molecule entering the lens; the lens kernel rewrites each entry in place, counts the
+        # molecules the bore stops and lists the slots of those that leave the lens (lens tallies: a
+        # hexapole lens, no per-molecule outputs); POST reads the listed entries (64 B + 4 B of list
+        # each) and writes each kept molecule (8 B index + 7 fields)
         n_enter = n - counts.get("Aperture 1", 0) - counts.get("Lens entrance", 0)
         # and the claim list: a bucket byte written and read back and a 4-byte slot per queued molecule
         wbytes = 48.0 * n + 64.0 * n_enter + 6.0 * n_enter
@@ -678,9 +680,10 @@ def main():
                       "ms_avg": walk_avg, "bytes_per_launch": wbytes, "molecules_queued": n_enter,
                       "GBps": wbytes / (walk_avg * 1e-3) / 1e9, "hbm_frac": wbytes / (walk_avg * 1e-3) / 1e9 / PEAK_HBM,
                       "faces": faces - faces_lens}
-        pbytes = 64.0 * n_enter + 64.0 * n_det
-        post_phase = {"kernel": "k_propagate<POST> (elements after the lens, tallies and kept molecules of the lens "
-                                "molecules)", "ms_avg": float(np.mean(post_ms)), "bytes_per_launch": pbytes,
+        n_left = n_enter - counts.get("Inside lens", 0)
+        pbytes = 68.0 * n_left + 64.0 * n_det
+        post_phase = {"kernel": "k_propagate<POST> (elements after the lens, tallies and kept molecules of the "
+                                "molecules that leave the lens; k_lens tallies those it stops)", "ms_avg": float(np.mean(post_ms)), "bytes_per_launch": pbytes,
                       "GBps": pbytes / (float(np.mean(post_ms)) * 1e-3) / 1e9,
                       "hbm_frac": pbytes / (float(np.mean(post_ms)) * 1e-3) / 1e9 / PEAK_HBM, "faces": faces_lens}
     elif lens:

@@ -84,7 +84,9 @@ struct InMol {
 // refill and writes each molecule's state where the lens left it back into its queue entry, with
 // the outcome (stopped by the bore, or left the lens at z1); POST walks the molecules that left the
 // lens through the remaining elements and records every lens molecule.  A molecule is recorded
-// (outputs, tallies, kept-molecule compaction) by PRE if it stops before the lens, else by POST.
+// (outputs, tallies, kept-molecule compaction) by PRE if it stops before the lens, else by POST --
+// except, with lens tallies (PipeArgs::lt), a molecule stopped in a hexapole lens, which k_lens
+// counts itself; POST then reads only the molecules k_lens listed as leaving the lens.
 enum : int { PHASE_FULL = 0, PHASE_PRE = 1, PHASE_POST = 2 };
 
 // Lens step constants: the half and sixth of the step (as rk4_step forms them) and, for a folding
@@ -113,7 +115,16 @@ struct PipeArgs {
   uint32_t *perm;              // queue slots in claim order
   float ord_k2[CTRAJ_MAX_STATES];  // per state: the acceleration per unit radius at the W' table's end
   int ord_ref;                 // the lens integrates in the reference's steps of dz (else of dt)
+  // lens tallies (null: the lens kernel writes every molecule back and POST reads every slot): the
+  // lens kernel counts the molecules stopped in the lens into this block, which POST adds to the run's
+  // tallies under the lens's "inside" label, and lists the slots of those that leave the lens, the
+  // only ones POST then walks.  Words (zeroed before the lens kernel): [LT_N] listed slots,
+  // [LT_STOP + s] stopped molecules of state s, [LT_OOT] / [LT_CAP] out-of-table / capped ones among
+  // them, [LT_FIRST] ~(slice index) of the first of them (0: none); the list from word LT_LIST.
+  uint32_t *lt;
 };
+enum : int { LT_N = 0, LT_STOP = 1, LT_OOT = LT_STOP + CTRAJ_MAX_STATES, LT_CAP, LT_FIRST, LT_LIST = 64 };
+constexpr int POST_ITEMS = 4;  // POST with lens tallies: listed slots per thread
 
 // Outcome of a lens molecule, written by k_lens into its queue entry's key (queue_put_done).
 enum : int { LENS_QUEUED = 0, LENS_STOPPED = 1, LENS_LEFT = 2 };
@@ -135,21 +146,28 @@ __device__ __forceinline__ void load_mol(const ctraj_run &R, const KBeam &B, int
   q.idx = i;
 }
 
-// POST: queue slot e (entry k of PRE wavefront w's region); slots past the region's count are empty.
-__device__ __forceinline__ void load_slot(const PipeArgs &P, int64_t e, InMol &q) {
-  const int64_t w = e / P.region;
-  const bool filled = e - w * P.region < (int64_t)P.wcount[w];
+// POST: the molecule of queue slot e as the lens kernel left it.
+__device__ __forceinline__ void load_entry(const PipeArgs &P, int64_t e, InMol &q) {
+  MState m;
+  int64_t key;
+  queue_get_done(P.q, e, m, key);
+  q.x = m.x; q.y = m.y; q.z = m.z; q.vx = m.vx; q.vy = m.vy; q.vz = m.vz; q.t = m.t;
+  q.idx = key & QKEY_INDEX;
+  q.status = (int)((key >> 40) & 0xff);
+  q.outcome = (int)((key >> 48) & 0xff);
+  q.st = (int)((uint64_t)key >> 56);
+}
+
+// POST's item j: with lens tallies the j-th listed slot, else queue slot j (entry k of PRE wavefront
+// w's region; slots past the region's count are empty).
+__device__ __forceinline__ void load_slot(const PipeArgs &P, int64_t j, InMol &q) {
   q.outcome = LENS_QUEUED;
-  if (filled) {
-    MState m;
-    int64_t key;
-    queue_get_done(P.q, e, m, key);
-    q.x = m.x; q.y = m.y; q.z = m.z; q.vx = m.vx; q.vy = m.vy; q.vz = m.vz; q.t = m.t;
-    q.idx = key & QKEY_INDEX;
-    q.status = (int)((key >> 40) & 0xff);
-    q.outcome = (int)((key >> 48) & 0xff);
-    q.st = (int)((uint64_t)key >> 56);
+  if (P.lt) {
+    load_entry(P, (int64_t)P.lt[LT_LIST + j], q);
+    return;
   }
+  const int64_t w = j / P.region;
+  if (j - w * P.region < (int64_t)P.wcount[w]) load_entry(P, j, q);
 }
 
 // Inputs of a run that passes positions and velocities only (no time, stored acceleration or state
@@ -168,6 +186,21 @@ __device__ __forceinline__ void load_min(const ctraj_run &R, int64_t i, InMol &q
 // ahead (and the constant time, acceleration and state are not carried per molecule).
 template <int FORCE, int INTEG, int PHASE, int KM, bool OUT, bool MIN = false>
 __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R, int aoi_lds_off, PipeArgs P) {
+  // POST: the listed slots (lens tallies), else every slot of the queue
+  const int64_t n_items =
+      PHASE == PHASE_POST ? (P.lt ? (int64_t)P.lt[LT_N] : (int64_t)P.n_pre_waves * P.region) : R.n;
+  // POST with lens tallies: the grid is sized for every slot of the queue, but the listed slots are
+  // usually a few per cent of them; only as many workgroups as give each thread about POST_ITEMS
+  // slots take part (the others leave at once; the first also merges the lens kernel's tallies
+  // below), which keeps the same-address atomics of the workgroups' final flushes few
+  int64_t n_wg = gridDim.x;
+  if constexpr (PHASE == PHASE_POST) {
+    if (P.lt) {
+      const int64_t per_wg = (int64_t)blockDim.x * POST_ITEMS;
+      n_wg = std::min<int64_t>(gridDim.x, std::max<int64_t>(1, (n_items + per_wg - 1) / per_wg));
+      if (blockIdx.x >= n_wg) return;
+    }
+  }
   extern __shared__ __attribute__((aligned(16))) double lds[];
   __shared__ uint32_t s_hist[CTRAJ_MAX_STATES * CTRAJ_MAX_LABELS];
   __shared__ unsigned long long s_first[CTRAJ_MAX_LABELS];
@@ -197,10 +230,9 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
   bool aoi_full = false;  // the slice holds more than AOI_WAVE - 64 entries: flush before staging more
   __syncthreads();
 
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t stride = n_wg * blockDim.x;
   WalkCounters wc{0u, 0u};
   uint32_t n_oot = 0, n_cap = 0, faces_post = 0;
-  const int64_t n_items = PHASE == PHASE_POST ? (int64_t)P.n_pre_waves * P.region : R.n;
   const int64_t gwave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   int64_t qn = 0;  // PRE: entries this wavefront has queued
   // PRE: the lens entrance, read once with scalar loads (a vector load of it in the loop would wait
@@ -219,7 +251,20 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
     ord_cap = P.ord_ref ? (float)L.n_steps : (float)B.cap;
   }
   int lens_inside = 0;
-  if constexpr (PHASE == PHASE_POST) lens_inside = elem_at(B.elems, P.lens_e).label_inside;
+  if constexpr (PHASE == PHASE_POST) {
+    lens_inside = elem_at(B.elems, P.lens_e).label_inside;
+    if (P.lt && blockIdx.x == 0 && threadIdx.x == 0) {
+      // lens tallies: the molecules the lens kernel counted (stopped in the lens, not walked here)
+      for (int s = 0; s < B.n_states; ++s)
+        if (R.tally && P.lt[LT_STOP + s])
+          atomicAdd((unsigned long long *)&R.tally[s * B.n_labels + lens_inside], (unsigned long long)P.lt[LT_STOP + s]);
+      if (R.work && P.lt[LT_OOT]) atomicAdd((unsigned long long *)&R.work[2], (unsigned long long)P.lt[LT_OOT]);
+      if (R.work && P.lt[LT_CAP]) atomicAdd((unsigned long long *)&R.work[3], (unsigned long long)P.lt[LT_CAP]);
+      if (R.first_index && P.lt[LT_FIRST])
+        atomicMin((unsigned long long *)&R.first_index[lens_inside],
+                  (unsigned long long)(R.index_base + (int64_t)~P.lt[LT_FIRST]));
+    }
+  }
   const int64_t j0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t w0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
   // one grid-stride iteration: molecule j (inputs in q, loaded earlier), then q is refilled
@@ -384,9 +429,10 @@ __global__ void __launch_bounds__(256) k_propagate(KBeam B, KTab KT, ctraj_run R
   if (R.tally)
     for (int b = threadIdx.x; b < nbins; b += blockDim.x)
       if (s_hist[b]) atomicAdd((unsigned long long *)&R.tally[b], (unsigned long long)s_hist[b]);
-  if (R.first_index)
+  if (R.first_index)  // the stored minimum only decreases: a read of it that is not above ours spares the atomic
     for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x)
-      if (s_first[b] != (~0ull >> 1)) atomicMin((unsigned long long *)&R.first_index[b], s_first[b]);
+      if (s_first[b] != (~0ull >> 1) && (int64_t)s_first[b] < __atomic_load_n(&R.first_index[b], __ATOMIC_RELAXED))
+        atomicMin((unsigned long long *)&R.first_index[b], s_first[b]);
   if (R.work && threadIdx.x < 5 && s_work[threadIdx.x])
     atomicAdd((unsigned long long *)&R.work[threadIdx.x], s_work[threadIdx.x]);
   if constexpr (PHASE == PHASE_PRE) {
@@ -649,9 +695,10 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
   if (R.tally)
     for (int b = threadIdx.x; b < nbins; b += blockDim.x)
       if (s_hist[b]) atomicAdd((unsigned long long *)&R.tally[b], (unsigned long long)s_hist[b]);
-  if (R.first_index)
+  if (R.first_index)  // the stored minimum only decreases: a read of it that is not above ours spares the atomic
     for (int b = threadIdx.x; b < B.n_labels; b += blockDim.x)
-      if (s_first[b] != (~0ull >> 1)) atomicMin((unsigned long long *)&R.first_index[b], s_first[b]);
+      if (s_first[b] != (~0ull >> 1) && (int64_t)s_first[b] < __atomic_load_n(&R.first_index[b], __ATOMIC_RELAXED))
+        atomicMin((unsigned long long *)&R.first_index[b], s_first[b]);
   if (R.work && threadIdx.x < 4 && s_work[threadIdx.x])
     atomicAdd((unsigned long long *)&R.work[threadIdx.x], s_work[threadIdx.x]);
 }
@@ -679,20 +726,27 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
 // walk().
 // ---------------------------------------------------------------------------------------------
 
-// The hexapole RK4 instantiations are held to at most 80 VGPRs (six wavefronts per SIMD), which they
-// fit without spilling; they compile to 64, so the host's 1024-thread workgroups run two per CU
-// beside two copies of one state's 24 KB table, eight wavefronts per SIMD.  The others need more
-// registers than that.
+// The hexapole RK4 instantiations are held to 64 VGPRs (eight wavefronts per SIMD), so the host's
+// 1024-thread workgroups run two per CU beside two copies of one state's 24 KB table.  They fit
+// without spilling vector registers; the two scalar registers they spill are reloaded after the
+// lens loop only.  Unheld, the scheduler spends up to 72 on the lens end's partial step (the step
+// loop itself needs 64), which would leave one workgroup per CU.  The others need more registers
+// than that.
 constexpr int REM_EMPTY = -1, REM_STOPPED = -2;
 
 // ORD: the wavefronts claim the list P.perm (longest-lived first) instead of the regions in order.
 template <int FORCE, int INTEG, bool ORD>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
-    FORCE == CTRAJ_FORCE_WSPLINE_HEX && INTEG == CTRAJ_INTEG_RK4 ? 6 : 1)))
+    FORCE == CTRAJ_FORCE_WSPLINE_HEX && INTEG == CTRAJ_INTEG_RK4 ? 8 : 1)))
 k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long *work) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
+  // lens tallies (P.lt) are built into the hexapole RK4 instantiations only; the others would need
+  // more registers for them
+  constexpr bool LT = FORCE == CTRAJ_FORCE_WSPLINE_HEX && INTEG == CTRAJ_INTEG_RK4;
   __shared__ unsigned long long s_work[2];
+  __shared__ uint32_t s_lt[LT ? LT_FIRST + 1 : 1];  // lens tallies: this workgroup's words of P.lt
   if (threadIdx.x < 2) s_work[threadIdx.x] = 0;
+  if (LT && threadIdx.x <= LT_FIRST) s_lt[threadIdx.x] = 0;
   const Tables T = stage_tables(lds, KT);
   __syncthreads();
   // eight claim counters: counter g serves the regions of one XCD's PRE workgroups (n_chunks over
@@ -765,8 +819,26 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
               outcome = LENS_LEFT;
             }
           }
-          queue_put_done(P.q, slot, m.x, m.y, m.z, m.vx, m.vy, m.vz, m.t,
-                         key | ((int64_t)status << 40) | ((int64_t)outcome << 48));
+          if (LT && P.lt && outcome == LENS_STOPPED) {
+            // lens tallies: the molecule is counted here and needs no further walk
+            atomicAdd(&s_lt[LT_STOP + (int)((uint64_t)key >> 56)], 1u);
+            atomicMax(&s_lt[LT_FIRST], ~(uint32_t)key);  // slice index < 2^32
+            if (status) {
+              if (status & CTRAJ_ST_OOT) atomicAdd(&s_lt[LT_OOT], 1u);
+              if (status & CTRAJ_ST_CAPPED) atomicAdd(&s_lt[LT_CAP], 1u);
+            }
+          } else {
+            queue_put_done(P.q, slot, m.x, m.y, m.z, m.vx, m.vy, m.vz, m.t,
+                           key | ((int64_t)status << 40) | ((int64_t)outcome << 48));
+            if (LT && P.lt) {  // list the slot for POST: one atomic per wavefront, by its first lane here
+              const uint64_t lm = __builtin_amdgcn_ballot_w64(true);
+              const int rank = lanes_below(lm);
+              uint32_t base = 0;
+              if (rank == 0) base = atomicAdd(&P.lt[LT_N], (uint32_t)__popcll(lm));
+              base = __builtin_amdgcn_readfirstlane(base);
+              P.lt[LT_LIST + base + (uint32_t)rank] = (uint32_t)slot;
+            }
+          }
           rem = REM_EMPTY;
         }
         const uint64_t need = __builtin_amdgcn_ballot_w64(rem == REM_EMPTY);
@@ -841,13 +913,17 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
     }
   }
 
-  if (work) {
-    if (lane == 0) {
+  if (work || (LT && P.lt)) {  // kernel arguments: every thread takes the same branch
+    if (work && lane == 0) {
       if (steps_w) atomicAdd(&s_work[1], (unsigned long long)steps_w);
       if (faces_w) atomicAdd(&s_work[0], (unsigned long long)faces_w);
     }
     __syncthreads();
-    if (threadIdx.x < 2 && s_work[threadIdx.x]) atomicAdd(&work[threadIdx.x], s_work[threadIdx.x]);
+    if (work && threadIdx.x < 2 && s_work[threadIdx.x]) atomicAdd(&work[threadIdx.x], s_work[threadIdx.x]);
+    if (LT && P.lt && threadIdx.x >= LT_STOP && threadIdx.x <= LT_FIRST && s_lt[threadIdx.x]) {
+      if (threadIdx.x == LT_FIRST) atomicMax(&P.lt[LT_FIRST], s_lt[LT_FIRST]);
+      else atomicAdd(&P.lt[threadIdx.x], s_lt[threadIdx.x]);
+    }
   }
 }
 
@@ -1414,6 +1490,12 @@ static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *sdev) {
     LensFn kl = lens_kernel(c->force, c->integ, ordered);
     if (!kl) return fail(c, CTRAJ_EINVAL, "no kernel for this force/integrator");
     const bool out = wants_outputs(R);
+    // lens tallies: for a hexapole lens in RK4 steps, without per-molecule outputs, and unless the
+    // molecules stopped in the lens are kept (apertures of interest), the lens kernel tallies those
+    // itself and POST walks only the molecules that leave the lens
+    const int lab_in = c->h_elems[lens_e].label_inside;
+    const bool lens_tally = c->force == CTRAJ_FORCE_WSPLINE_HEX && c->integ == CTRAJ_INTEG_RK4 && !out &&
+                            !(lab_in >= 0 && lab_in < 64 && ((R->aoi_mask >> lab_in) & 1ull));
     // PRE: walk to the lens and through its entrance; each wavefront queues the molecules that enter
     // the lens into its own region (no atomics)
     const int blk = 256;
@@ -1438,7 +1520,8 @@ static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *sdev) {
     int64_t slice = n_first;
     PipeArgs P{};
     if (sdev) {
-      const size_t q_entries = (CTRAJ_NOALLOC_SCRATCH_BYTES - SCR_QUEUE) / 64;
+      // 64 B of queue entry and 4 B of listed slot per entry, and the lens tally words
+      const size_t q_entries = (CTRAJ_NOALLOC_SCRATCH_BYTES - SCR_QUEUE - 4 * LT_LIST) / 68 & ~(size_t)63;
       const int g_cap = (int)std::min<size_t>(q_entries / blk, SCR_WCOUNT_MAX / (blk / 64)) & ~7;
       if (R->grid > g_cap)
         return fail(c, CTRAJ_EINVAL, "grid of " + std::to_string(R->grid) + " blocks exceeds the " +
@@ -1449,15 +1532,19 @@ static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *sdev) {
       P.q = (longlong2 *)(sdev + SCR_QUEUE);
       P.claim = (unsigned long long *)(sdev + SCR_CLAIM);
       P.wcount = (uint32_t *)(sdev + SCR_WCOUNT);
+      if (lens_tally) P.lt = (uint32_t *)(sdev + SCR_QUEUE + q_entries * 64);
     } else {
       const int64_t threads = (int64_t)gpre * blk;
       const size_t cap = (size_t)threads * (size_t)((slice + threads - 1) / threads);
-      if ((rc = ensure_buf(c, c->pipe, cap * 64 + 256)) != CTRAJ_OK) return rc;
+      // the queue, then (lens tallies) the tally words and the listed slots
+      if ((rc = ensure_buf(c, c->pipe, cap * 64 + 256 + (lens_tally ? 4 * (LT_LIST + cap) : 0))) != CTRAJ_OK)
+        return rc;
       if ((rc = ensure_buf(c, c->pipe_cnt, 8 * 256 + (size_t)gpre * (blk / 64) * sizeof(uint32_t))) != CTRAJ_OK)
         return rc;
       P.q = (longlong2 *)c->pipe.p;
       P.claim = (unsigned long long *)c->pipe_cnt.p;
       P.wcount = (uint32_t *)((char *)c->pipe_cnt.p + 8 * 256);
+      if (lens_tally) P.lt = (uint32_t *)((char *)c->pipe.p + cap * 64 + 256);
     }
     const int64_t threads = (int64_t)gpre * blk;
     P.n_pre_waves = gpre * (blk / 64);
@@ -1485,6 +1572,7 @@ static int32_t propagate_impl(ctraj_ctx *c, const ctraj_run *R, char *sdev) {
       const ctraj_run S = slice_of(*R, s0, len);
       P.region = (len + threads - 1) / threads * 64;
       if ((rc = ctraj_fill_i64(c, (int64_t *)P.claim, 0, 8 * 32)) != CTRAJ_OK) return rc;
+      if (P.lt && (rc = ctraj_fill_i64(c, (int64_t *)P.lt, 0, LT_LIST / 2)) != CTRAJ_OK) return rc;
       if (pev && s0 == 0) HIPCHK(c, hipEventRecord(pev[0], c->stream));
       hipLaunchKernelGGL(kpre, dim3(gpre), dim3(blk), lds_aoi, c->stream, B, kt, S, 0, P);
       HIPCHK(c, hipGetLastError());

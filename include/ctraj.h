@@ -183,7 +183,8 @@ int32_t ctraj_propagate(ctraj_ctx *ctx, const ctraj_run *run);
 /* ctraj_propagate for callers that cannot allocate device memory (code under HIP graph capture, serving loops,
  * harnesses that forbid allocation).  No counterpart in the reference.  Same run, same results, bit for bit.
  * Every device buffer its kernels use beyond the caller's inputs and outputs -- the lens-entry queue, the queue's
- * claim and count words, the lane walker's molecule counter and its copy of the run -- lies in a library-owned
+ * claim and count words, the lens tally words and list of molecules leaving the lens, the lane walker's molecule
+ * counter and its copy of the run -- lies in a library-owned
  * array of CTRAJ_NOALLOC_SCRATCH_BYTES per device, reserved when the library is loaded on a device and held until
  * the process ends; a beamline with a lens is processed in slices of at most as many molecules as that queue holds
  * (about a million).  No hipMalloc, hipFree or event creation happens in the call: CTRAJ_RUN_PHASE_TIMES, and a

@@ -7,7 +7,9 @@ length and dt; reference mode: the a(r) table built from the two-level form, ran
 source of random spread, then compares final phase space, stored acceleration, labels, status bits,
 tallies and work counters of the default split path, the single-kernel lane walker and the
 one-molecule-per-thread walker with the oracle; on every fourth seed it also compares the record
-kernel's trajectory rows and z-grid samples.
+kernel's trajectory rows and z-grid samples.  Runs without per-molecule outputs (tallies, first
+indices, work counters and kept molecules only) are compared on every seed and on a four-state
+beamline.
 """
 import numpy as np
 import pytest
@@ -112,3 +114,60 @@ def test_random_beamline_trajectory_rows_and_zgrid_equal_the_oracle(seed):
         ref_s = O.zgrid_from_rows(orc["rows"][i], int(orc["n_rows"][i]), 0.0, 1e-3, K)
         np.testing.assert_array_equal(samp[:, :, i], ref_s, err_msg=f"seed {seed} molecule {i}")
         assert written[i] == np.count_nonzero(~np.isnan(ref_s[0]))
+
+
+def check_tallies_only_run(eng, dev, orc, state, index_base, tag):
+    """A run without per-molecule outputs, keeping only the detected molecules, against the oracle:
+    tallies per state, first index per label (offset by index_base), work counters with the
+    out-of-table and step-capped counts, and the kept molecules' indices and final phase space."""
+    lab = orc["label"]
+    nl = len(eng.labels)
+    ns = dev["tally"].shape[0]
+    for s in range(ns):
+        m = state == s
+        np.testing.assert_array_equal(dev["tally"][s], np.bincount(lab[m], minlength=nl), err_msg=f"{tag} state {s}")
+    for i in range(nl):
+        hit = np.nonzero(lab == i)[0]
+        want = index_base + hit[0] if hit.size else np.iinfo(np.int64).max
+        assert int(dev["first_index"][i]) == want, (tag, eng.labels[i])
+    assert int(dev["work"][N.WORK_FACES]) == int(orc["work"][0]), tag
+    assert int(dev["work"][N.WORK_STEPS]) == int(orc["work"][1]), tag
+    assert int(dev["work"][N.WORK_OOT]) == int(np.count_nonzero(orc["status"] & 1)), tag
+    assert int(dev["work"][N.WORK_CAPPED]) == int(np.count_nonzero(orc["status"] & 2)), tag
+    det = np.nonzero(lab == eng.cb.label_detected)[0]
+    o = np.argsort(dev["aoi_index"])
+    np.testing.assert_array_equal(dev["aoi_index"][o], index_base + det, err_msg=tag)
+    for k in FIELDS:
+        np.testing.assert_array_equal(dev["aoi_out"][k][o], orc["out"][k][det], err_msg=f"{tag} kept {k}")
+
+
+@pytest.mark.parametrize("seed", list(range(40)))
+def test_random_beamline_tallies_only_run_equals_the_oracle(seed):
+    """No per-molecule outputs: for a hexapole lens in RK4 steps the lens kernel tallies the
+    molecules stopped in the lens itself and the walk after the lens reads only those that leave it;
+    either way the tallies, first indices, work counters and kept molecules equal the oracle's."""
+    bl, X, V, ref = random_beamline(1000 + seed)
+    eng = Engine(bl)
+    orc = O.propagate(eng.cb, X[0], X[1], X[2], V[0], V[1], V[2])
+    base = 7_000_000_000
+    for flags in (0, N.RUN_QUEUE_ORDER):
+        dev = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], outputs=False, per_molecule=False,
+                            aoi_labels=("Detected",), index_base=base, flags=flags)
+        check_tallies_only_run(eng, dev, orc, np.zeros(X.shape[1], np.uint8), base,
+                               f"seed {seed} ({'reference' if ref else 'configuration'} mode) flags={flags}")
+
+
+def test_four_state_tallies_only_run_equals_the_oracle():
+    """The lens kernel's own tallies of stopped molecules are kept per lens state."""
+    from trajectories import configs
+
+    bl = configs.config4()
+    n = 200_000
+    X, V = configs.draw_source(n, seed=45)
+    st = np.random.RandomState(46).randint(0, 4, n).astype(np.uint8)
+    eng = Engine(bl, n_states=4)
+    orc = O.propagate(eng.cb, X[0], X[1], X[2], V[0], V[1], V[2], state=st)
+    dev = eng.propagate(X[0], X[1], X[2], V[0], V[1], V[2], state=st, outputs=False, per_molecule=False,
+                        aoi_labels=("Detected",), index_base=123)
+    assert dev["tally"].shape[0] == 4
+    check_tallies_only_run(eng, dev, orc, st, 123, "config 4, four states")
