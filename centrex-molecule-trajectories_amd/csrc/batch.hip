@@ -69,7 +69,13 @@ struct BPipe {
   int64_t region;
   int n_pre_waves;
   int64_t s0, s1;               // the slice of batch molecule indices [s0, s1) this PRE launch walks
+  // the lens kernel counts the molecules the bore stops into the tallies itself and lists the queue
+  // slots of those that leave their lens, the only ones phase 3 reads: [0] their count (zeroed
+  // before the lens kernel), the list from word BLT_LIST
+  uint32_t *lt;
 };
+constexpr int BLT_LIST = 64;
+constexpr int BPOST_ITEMS = 4;  // phase 3: listed slots per thread
 
 struct BLane {
   int c, e0, ne, lens, label_inside;
@@ -363,7 +369,10 @@ k_batch_integ(BArgs A, BPipe P, KTab KT, double Hh, double H6) {
   if (threadIdx.x == 0) s_steps = 0;
   const Tables T = stage_tables(lds, KT);
   BLensC *s_cfg = reinterpret_cast<BLensC *>(lds + A.off_begin);
+  // per configuration, the molecules the bore stopped; then the out-of-table and capped ones among them
+  uint32_t *s_stop = reinterpret_cast<uint32_t *>(lds + A.off_hist);
   for (int i = threadIdx.x; i < A.n_cfg; i += blockDim.x) s_cfg[i] = A.lensk[i];
+  for (int i = threadIdx.x; i < A.n_cfg + 2; i += blockDim.x) s_stop[i] = 0;
   __syncthreads();
 
   const WsRef wsb = ws_of(T, 0);
@@ -404,8 +413,23 @@ k_batch_integ(BArgs A, BPipe P, KTab KT, double Hh, double H6) {
             r2 = __builtin_fma(m.x, m.x, m.y * m.y);
             outcome = r2 > R2 ? BLENS_STOPPED : BLENS_LEFT;
           }
-          queue_put_done(P.q, slot, m.x, m.y, m.z, m.vx, m.vy, m.vz, m.t,
-                         key | ((int64_t)status << 40) | ((int64_t)outcome << 48));
+          if (outcome == BLENS_STOPPED) {  // counted here: the molecule needs no further walk
+            atomicAdd(&s_stop[c], 1u);
+            if (status) {
+              if (status & CTRAJ_ST_OOT) atomicAdd(&s_stop[A.n_cfg], 1u);
+              if (status & CTRAJ_ST_CAPPED) atomicAdd(&s_stop[A.n_cfg + 1], 1u);
+            }
+          } else {
+            queue_put_done(P.q, slot, m.x, m.y, m.z, m.vx, m.vy, m.vz, m.t,
+                           key | ((int64_t)status << 40) | ((int64_t)outcome << 48));
+            // list the slot for phase 3: one atomic per wavefront, by its first lane here
+            const uint64_t lm = __builtin_amdgcn_ballot_w64(true);
+            const int rank = lanes_below(lm);
+            uint32_t base = 0;
+            if (rank == 0) base = atomicAdd(&P.lt[0], (uint32_t)__popcll(lm));
+            base = __builtin_amdgcn_readfirstlane(base);
+            P.lt[BLT_LIST + base + (uint32_t)rank] = (uint32_t)slot;
+          }
           rem = BREM_EMPTY;
         }
         const uint64_t need = __builtin_amdgcn_ballot_w64(rem == BREM_EMPTY);
@@ -477,14 +501,27 @@ k_batch_integ(BArgs A, BPipe P, KTab KT, double Hh, double H6) {
   if (lane == 0 && steps_w) atomicAdd(&s_steps, (unsigned long long)steps_w);
   __syncthreads();
   if (threadIdx.x == 0 && s_steps) atomicAdd(&A.work[1], s_steps);
+  for (int i = threadIdx.x; i < A.n_cfg; i += blockDim.x)
+    if (s_stop[i])
+      atomicAdd((unsigned long long *)&A.tally[(int64_t)i * A.n_labels + s_cfg[i].label_inside],
+                (unsigned long long)s_stop[i]);
+  if (threadIdx.x < 2 && s_stop[A.n_cfg + threadIdx.x])
+    atomicAdd(&A.work[2 + threadIdx.x], (unsigned long long)s_stop[A.n_cfg + threadIdx.x]);
 }
 
-// Split scan, phase 3: every queue entry the lens kernel wrote back (grid-stride, wave-uniform trip
-// count): a molecule that left its lens walks its configuration's elements after the lens (the lens
-// exit row stores (0, -g, 0)), one stopped by the bore is recorded inside the lens; (configuration,
-// label) tallied in LDS.
+// Split scan, phase 3: the molecules the lens kernel listed as leaving their lens (grid-stride,
+// wave-uniform trip count) walk their configuration's elements after the lens (the lens exit row
+// stores (0, -g, 0)); (configuration, label) tallied in LDS.  Those the bore stopped were tallied by
+// the lens kernel.
 template <int KM>
 __global__ void __launch_bounds__(256) k_batch_post(BArgs A, BPipe P) {
+  // as many workgroups as give each thread about BPOST_ITEMS listed slots take part (the grid is
+  // sized for every slot of the queue; the listed ones are usually a few per cent of them), which
+  // keeps the workgroups' final tally atomics few
+  const int64_t n_items = (int64_t)P.lt[0];
+  const int64_t per_wg = (int64_t)blockDim.x * BPOST_ITEMS;
+  const int64_t n_wg = std::min<int64_t>(gridDim.x, std::max<int64_t>(1, (n_items + per_wg - 1) / per_wg));
+  if (blockIdx.x >= n_wg) return;
   extern __shared__ __attribute__((aligned(16))) double lds[];
   __shared__ unsigned long long s_work[4];
   uint32_t *s_hist = reinterpret_cast<uint32_t *>(lds);
@@ -495,35 +532,25 @@ __global__ void __launch_bounds__(256) k_batch_post(BArgs A, BPipe P) {
 
   const int lane = lane_id();
   const double g = A.g;
-  const int64_t n_items = (int64_t)P.n_pre_waves * P.region;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t stride = n_wg * blockDim.x;
   WalkCounters wc{0u, 0u};
   uint32_t n_oot = 0, n_cap = 0;
   for (int64_t wbase = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < n_items; wbase += stride) {
-    const int64_t e = wbase + lane;
-    bool fin = false;
+    const int64_t j = wbase + lane;
+    const bool fin = j < n_items;
     int c = 0, label = 0;
-    if (e < n_items) {
-      const int64_t w = e / P.region;
-      fin = e - w * P.region < (int64_t)P.wcount[w];
-    }
     if (fin) {
       MState m;
       int64_t key;
-      queue_get_done(P.q, e, m, key);
+      queue_get_done(P.q, (int64_t)P.lt[BLT_LIST + j], m, key);
       c = (int)(key & QKEY_INDEX);
       const uint32_t status = (uint32_t)((key >> 40) & 0xff);
-      const int outcome = (int)((key >> 48) & 0xff);
       const BLensC &K = A.lensk[c];
-      if (outcome == BLENS_LEFT) {
-        m.ax = 0.0;
-        m.ay = -g;
-        m.az = 0.0;
-        label = A.label_detected;
-        walk_plain<KM>(opaque(A.elems) + K.e0, K.lens + 1, K.ne, g, m, label, wc);
-      } else {
-        label = K.label_inside;
-      }
+      m.ax = 0.0;
+      m.ay = -g;
+      m.az = 0.0;
+      label = A.label_detected;
+      walk_plain<KM>(opaque(A.elems) + K.e0, K.lens + 1, K.ne, g, m, label, wc);
       n_oot += (status & CTRAJ_ST_OOT) ? 1u : 0u;
       n_cap += (status & CTRAJ_ST_CAPPED) ? 1u : 0u;
     }
@@ -618,6 +645,7 @@ static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refi
   P.wcount = (uint32_t *)((char *)b->d_pipe_cnt + 8 * 256);
   P.region = b->region;
   P.n_pre_waves = b->n_pre_waves;
+  P.lt = (uint32_t *)((char *)b->d_pipe + (size_t)b->n_pre_waves * (size_t)b->region * 64);
   // phase 1: LDS = configuration offsets + histogram
   const size_t lds_pre = (size_t)b->n_cfg * sizeof(int64_t) + hist_bytes;
   const void *kp = b->circ_only ? (const void *)k_batch_pre<KM_CIRC> : (const void *)k_batch_pre<KM_ALL>;
@@ -625,7 +653,8 @@ static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refi
   // phase 2: LDS = tables + lens constants
   const size_t cfg_dbl = (size_t)b->n_cfg * (sizeof(BLensC) / sizeof(double));
   A.off_begin = (int)tab;
-  const size_t lds_lens = (tab + cfg_dbl) * sizeof(double);
+  A.off_hist = (int)(tab + cfg_dbl);  // the lens kernel's stopped-molecule counts
+  const size_t lds_lens = (tab + cfg_dbl) * sizeof(double) + (size_t)(b->n_cfg + 2) * sizeof(uint32_t);
   const void *kl = b->uniform_dt ? (const void *)k_batch_integ<true> : (const void *)k_batch_integ<false>;
   A.Hs = b->dt;
   if ((rc = lds_limit(c, kl, lds_lens)) != CTRAJ_OK) return rc;
@@ -652,6 +681,7 @@ static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refi
     if (s0 > 0)
       hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, c->stream, (unsigned long long *)b->d_pipe_cnt,
                          (int64_t)8 * 32);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, (unsigned long long *)P.lt, (int64_t)BLT_LIST / 2);
     hipLaunchKernelGGL((PreK)kp, dim3(b->pre_grid), dim3(256), lds_pre, c->stream, A, P);
     HIPCHK(c, hipGetLastError());
     hipLaunchKernelGGL((LensK)kl, dim3(grid), dim3(block), lds_lens, c->stream, A, P, kt, Hh, H6);
@@ -780,7 +810,8 @@ int32_t ctraj_batch_create(ctraj_ctx *c, const ctraj_elem *elems, int32_t n_elem
     b->n_pre_waves = gpre * 4;
     b->slice = slice;
     b->region = (slice + threads - 1) / threads * 64;
-    const size_t qbytes = (size_t)b->n_pre_waves * (size_t)b->region * 64;
+    // the queue, then the lens kernel's count and list of the molecules that leave their lens
+    const size_t qbytes = (size_t)b->n_pre_waves * (size_t)b->region * (64 + 4) + 4 * BLT_LIST;
     if (hipMalloc((void **)&b->d_lensk, sizeof(BLensC) * n_cfgs) != hipSuccess ||
         hipMalloc(&b->d_pipe, qbytes) != hipSuccess ||
         hipMalloc(&b->d_pipe_cnt, 8 * 256 + (size_t)b->n_pre_waves * sizeof(uint32_t)) != hipSuccess) {
