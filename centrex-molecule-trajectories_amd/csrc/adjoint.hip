@@ -108,22 +108,21 @@ struct LensK {
 // does not fold every step takes the plain RK4 step.  The forward sweep and both recomputations call
 // this one function, so a recomputed state equals the forward state bit for bit.
 struct SmoothStep {
-  double h, hh, h6, khh, kh, kh6;
+  HexK hk;
   bool fold;
 };
 
 __device__ __forceinline__ SmoothStep smooth_step_k(const Force<CTRAJ_FORCE_WSPLINE_HEX> &F, double h) {
-  const double hh = 0.5 * h, h6 = h * kSixth;
-  return SmoothStep{h, hh, h6, F.ks * hh, F.ks * h, F.ks * h6, hex_folds(F.ks)};
+  return SmoothStep{hex_k(F.ks, h), hex_folds(F.ks)};
 }
 
 __device__ __forceinline__ void smooth_step(const Force<CTRAJ_FORCE_WSPLINE_HEX> &F, const Tables &T, const WsRef &wsb,
                                             const SmoothStep &S, bool full, double hp, double &x, double &y,
                                             double &vx, double &vy, double r2, uint32_t &st) {
   if (full && S.fold)
-    rk4_step_hex_k(wsb, T.umax, F.cu, F.gk, S.h, S.hh, S.h6, S.khh, S.kh, S.kh6, x, y, vx, vy, r2);
+    rk4_step_hex_k(wsb, T.umax, F.cu, F.gk, S.hk, x, y, vx, vy, r2);
   else
-    lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, full ? S.h : hp, x, y, vx, vy, r2, st);
+    lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, full ? S.hk.h : hp, x, y, vx, vy, r2, st);
 }
 
 // Reverse of one classical RK4 step (rk4_step) of size h from input state (x, y, vx, vy).

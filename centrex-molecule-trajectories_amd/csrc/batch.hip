@@ -354,16 +354,16 @@ __global__ void __launch_bounds__(256) k_batch_pre(BArgs A, BPipe P) {
 // constants per lane: the configuration's lens constants staged in LDS, its field constants and
 // step in the lane's registers).  A lane's rem is its full steps left (> 0), 0 at the lens end (one
 // partial step of L/vz - n dt to go), REM_STOPPED once the bore stopped it, REM_EMPTY when it holds
-// no molecule.  UH: every configuration's lens has the same dt, whose half and sixth come formed by
-// the host (Hh, H6), and folds its force scale (the step constants ks h/2, ks h, ks h/6 are formed per
-// molecule); otherwise each step chooses the folded or the plain RK4 step for its molecule's lens
-// (lens_step).  The arithmetic per molecule is the one of k_batch_lanes.
+// no molecule.  UH: every configuration's lens has the same dt and folds its force scale (the folded
+// step's constants, hex_k of the lens's ks and dt, are formed per molecule); otherwise each step
+// chooses the folded or the plain RK4 step for its molecule's lens (lens_step).  The arithmetic per
+// molecule is the one of k_batch_lanes.
 constexpr int BREM_EMPTY = -1, BREM_STOPPED = -2;
 enum : int { BLENS_STOPPED = 1, BLENS_LEFT = 2 };
 
 template <bool UH>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(UH ? 6 : 1)))
-k_batch_integ(BArgs A, BPipe P, KTab KT, double Hh, double H6) {
+k_batch_integ(BArgs A, BPipe P, KTab KT) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
   __shared__ unsigned long long s_steps;
   if (threadIdx.x == 0) s_steps = 0;
@@ -387,7 +387,8 @@ k_batch_integ(BArgs A, BPipe P, KTab KT, double Hh, double H6) {
 
   MState m;
   m.x = m.y = m.z = m.vx = m.vy = m.vz = m.t = 0.0;
-  double r2 = 0.0, dzs = 0.0, ks = 0.0, cu = 0.0, gk = 0.0, R2 = 0.0, Hs = A.Hs, khh = 0.0, kh = 0.0, kh6 = 0.0;
+  double r2 = 0.0, dzs = 0.0, ks = 0.0, cu = 0.0, gk = 0.0, R2 = 0.0, Hs = A.Hs;
+  HexK hk{};  // UH: the folded step's constants of the molecule's lens
   int rem = BREM_EMPTY, nfull = 0, c = 0;
   uint32_t status = 0;
   int64_t slot = 0, key = 0;
@@ -465,9 +466,7 @@ k_batch_integ(BArgs A, BPipe P, KTab KT, double Hh, double H6) {
             R2 = K.R2;
             if constexpr (!UH) Hs = K.Hs;
             if constexpr (UH) {
-              khh = ks * Hh;
-              kh = ks * Hs;
-              kh6 = ks * H6;
+              hk = hex_k(ks, Hs);
             }
             status = 0;
             nfull = rk4_full_steps(K.Ll, m.vz, Hs, A.cap, status);
@@ -486,7 +485,7 @@ k_batch_integ(BArgs A, BPipe P, KTab KT, double Hh, double H6) {
     steps_w += (uint32_t)__popcll(am);
     if (act) {
       if constexpr (UH) {
-        rk4_step_hex_k(wsb, T.umax, cu, gk, Hs, Hh, H6, khh, kh, kh6, m.x, m.y, m.vx, m.vy, r2);
+        rk4_step_hex_k(wsb, T.umax, cu, gk, hk, m.x, m.y, m.vx, m.vy, r2);
       } else {
         const Force<CTRAJ_FORCE_WSPLINE_HEX> F{ks, cu, g, gk};
         lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, Hs, m.x, m.y, m.vx, m.vy, r2, status);
@@ -671,9 +670,7 @@ static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refi
   hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<int64_t>((nt + 255) / 256, 1024)), dim3(256), 0, c->stream,
                      (unsigned long long *)b->d_tally, nt);
   using PreK = void (*)(BArgs, BPipe);
-  using LensK = void (*)(BArgs, BPipe, KTab, double, double);
-  // the RK4 half and sixth steps, rounded as rk4_step forms them
-  const double Hh = 0.5 * b->dt, H6 = b->dt * (1.0 / 6.0);
+  using LensK = void (*)(BArgs, BPipe, KTab);
   // slices of at most b->slice molecules reuse the queue; the tallies and work counters accumulate
   for (int64_t s0 = 0; s0 < b->total; s0 += b->slice) {
     P.s0 = s0;
@@ -684,7 +681,7 @@ static int batch_launch(ctraj_batch *b, int block, int chunk, int grid, int refi
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, (unsigned long long *)P.lt, (int64_t)BLT_LIST / 2);
     hipLaunchKernelGGL((PreK)kp, dim3(b->pre_grid), dim3(256), lds_pre, c->stream, A, P);
     HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL((LensK)kl, dim3(grid), dim3(block), lds_lens, c->stream, A, P, kt, Hh, H6);
+    hipLaunchKernelGGL((LensK)kl, dim3(grid), dim3(block), lds_lens, c->stream, A, P, kt);
     HIPCHK(c, hipGetLastError());
     hipLaunchKernelGGL((PreK)kq, dim3(gpost), dim3(256), hist_bytes, c->stream, A, P);
     HIPCHK(c, hipGetLastError());

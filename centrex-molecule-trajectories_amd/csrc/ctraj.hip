@@ -92,12 +92,13 @@ enum : int { PHASE_FULL = 0, PHASE_PRE = 1, PHASE_POST = 2 };
 // Lens step constants: the half and sixth of the step (as rk4_step forms them) and, for a folding
 // hexapole lens, ks times the half step, the step and the sixth, and gk = -g/ks (rk4_step_hex_k).
 struct StepK {
-  double hh, h6, khh, kh, kh6, gk;
+  double gk;
+  HexK hk;  // the folded hexapole step's constants, hex_k(ks, h)
+  HexK h1;  // the plain step's, hex_k(1, h)
 };
 
 __host__ __device__ inline StepK step_k(double h, double ks, double g) {
-  const double hh = 0.5 * h, h6 = h * (1.0 / 6.0);
-  return StepK{hh, h6, ks * hh, ks * h, ks * h6, hex_folds(ks) ? -g / ks : 0.0};
+  return StepK{hex_folds(ks) ? -g / ks : 0.0, hex_k(ks, h), hex_k(1.0, h)};
 }
 
 struct PipeArgs {
@@ -655,9 +656,7 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
         // branch-free step tail: every lane in the lens runs the same instructions
         const bool last = (s.i == s.nfull);
         if constexpr (FOLD) {
-          const double hh = 0.5 * s.h, h6 = s.h * kSixth;
-          rk4_step_hex_k(s.wsb, T.umax, F.cu, LA.gk, s.h, hh, h6, F.ks * hh, F.ks * s.h, F.ks * h6, m.x, m.y, m.vx,
-                         m.vy, s.r2);
+          rk4_step_hex_k(s.wsb, T.umax, F.cu, LA.gk, hex_k(F.ks, s.h), m.x, m.y, m.vx, m.vy, s.r2);
         } else {
           rk4_step<FORCE>(F, T, s.wsb, s.h, m.x, m.y, m.vx, m.vy, s.r2, s.status);
         }
@@ -802,9 +801,7 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
             if (!dead) {  // the end of the lens: one partial step of L/vz - n dt, so the lens ends at z1
               const double hp = Ll / m.vz - (double)nfull * Hs;
               if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX) {  // the host launches only folding lenses here
-                const double hh = 0.5 * hp, h6 = hp * kSixth;
-                rk4_step_hex_k(wsb, T.umax, F.cu, K.gk, hp, hh, h6, F.ks * hh, F.ks * hp, F.ks * h6, m.x, m.y, m.vx,
-                               m.vy, r2);
+                rk4_step_hex_k(wsb, T.umax, F.cu, K.gk, hex_k(F.ks, hp), m.x, m.y, m.vx, m.vy, r2);
               } else {
                 rk4_step<FORCE>(F, T, wsb, hp, m.x, m.y, m.vx, m.vy, r2, status);
               }
@@ -899,9 +896,9 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
     if (act) {
       if constexpr (INTEG == CTRAJ_INTEG_RK4) {
         if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX)  // the host launches only folding lenses here
-          rk4_step_hex_k(wsb, T.umax, F.cu, K.gk, Hs, K.hh, K.h6, K.khh, K.kh, K.kh6, m.x, m.y, m.vx, m.vy, r2);
+          rk4_step_hex_k(wsb, T.umax, F.cu, K.gk, K.hk, m.x, m.y, m.vx, m.vy, r2);
         else
-          rk4_step_hh<FORCE>(F, T, wsb, Hs, K.hh, K.h6, m.x, m.y, m.vx, m.vy, r2, status);
+          rk4_step_k<FORCE>(F, T, wsb, K.h1, m.x, m.y, m.vx, m.vy, r2, status);
         m.z = m.z + dzs;
         m.t = m.t + Hs;
         r2 = __builtin_fma(m.x, m.x, m.y * m.y);

@@ -346,68 +346,69 @@ __device__ __forceinline__ bool lens_ref(const ctraj_elem &E, const Tables &T, i
 // and device.
 constexpr double kSixth = 1.0 / 6.0;
 
-// Classical RK4 on the transverse state (x, y, vx, vy) with step h; vz is constant in the lens.
-// hh = h/2 and h6 = h*(1/6).  The stage slopes are folded into running sums
-//   S_x = ((v1 + 2 v2) + 2 v3) + v4,   S_v = ((a1 + 2 a2) + 2 a3) + a4
-// as soon as each stage is known, which keeps few FP64 values live.  Every FMA here is
-// mirrored by the CPU oracle.
-// rk4_step_hh takes hh and h6 precomputed (a kernel whose step size is uniform keeps them in
-// scalar registers); rk4_step forms them from h.
+// Classical RK4 on the transverse state (x, y, vx, vy) with step h; vz is constant in the lens.  The
+// force does not depend on the velocity, so the stage velocities enter only through their sums, and
+// with slopes s_k = a(x_k) / ks (ks = 1 for the plain step; the folded hexapole step below divides
+// the force scale out) the stage points and the result expand to
+//   x2 = x + h/2 v,  x3 = x2 + (h/2)(ks h/2) s1,  x4 = (x + h v) + h (ks h/2) s2,
+//   x' = (x + h v) + (h/6)(ks h) (s1 + s2 + s3),  v' = v + (ks h/6) (2 (s1 + s2 + s3) + (s4 - s1)),
+// the classical stages in exact arithmetic with 20 operations on the state where the stage
+// velocities take 28.  HexK holds the constants, formed by hex_k in this order wherever a step size
+// is chosen (host or lane); h/6 is formed as h * (1/6).  The CPU oracle mirrors every operation.
+struct HexK {
+  double h, hh, c3, c4, cx, kh6;  // h, h/2, (h/2)(ks h/2), h (ks h/2), (h/6)(ks h), ks h/6
+};
+
+__host__ __device__ __forceinline__ HexK hex_k(double ks, double h) {
+  const double hh = 0.5 * h, h6 = h * kSixth, khh = ks * hh, kh = ks * h;
+  return HexK{h, hh, hh * khh, h * khh, h6 * kh, ks * h6};
+}
+
+// The expanded step; slope(x, y, r2, sx, sy) gives the stage slope at (x, y), r2 = x^2 + y^2 (the
+// first stage's r2 is the caller's).
+template <class Slope>
+__device__ __forceinline__ void rk4_expanded(const HexK &K, const Slope &slope, double &x, double &y, double &vx,
+                                             double &vy, double r2) {
+  double s1x, s1y, s2x, s2y, s3x, s3y, s4x, s4y;
+  slope(x, y, r2, s1x, s1y);
+  const double x2 = __builtin_fma(K.hh, vx, x), y2 = __builtin_fma(K.hh, vy, y);
+  slope(x2, y2, __builtin_fma(x2, x2, y2 * y2), s2x, s2y);
+  const double x3 = __builtin_fma(K.c3, s1x, x2), y3 = __builtin_fma(K.c3, s1y, y2);
+  const double xv = __builtin_fma(K.h, vx, x), yv = __builtin_fma(K.h, vy, y);
+  slope(x3, y3, __builtin_fma(x3, x3, y3 * y3), s3x, s3y);
+  const double x4 = __builtin_fma(K.c4, s2x, xv), y4 = __builtin_fma(K.c4, s2y, yv);
+  double sumx = s1x + s2x, sumy = s1y + s2y;
+  sumx = sumx + s3x;
+  sumy = sumy + s3y;
+  slope(x4, y4, __builtin_fma(x4, x4, y4 * y4), s4x, s4y);
+  x = __builtin_fma(K.cx, sumx, xv);
+  y = __builtin_fma(K.cx, sumy, yv);
+  vx = __builtin_fma(K.kh6, __builtin_fma(2.0, sumx, s4x - s1x), vx);
+  vy = __builtin_fma(K.kh6, __builtin_fma(2.0, sumy, s4y - s1y), vy);
+}
+
+// The plain step (slopes = accelerations): K = hex_k(1.0, h); rk4_step forms it from h.
 template <int FORCE>
-__device__ __forceinline__ void rk4_step_hh(const Force<FORCE> &F, const Tables &T, const WsRef &wsb, double h,
-                                            double hh, double h6, double &x, double &y, double &vx, double &vy,
-                                            double r2, uint32_t &status) {
-  double ax, ay;
-  if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX) {
-    force_hex_r2(wsb, T.umax, F.ks, F.cu, F.g, x, y, r2, ax, ay);
-  } else {
-    F(T, wsb, x, y, ax, ay, status);
-  }
-  // stage 2 point
-  double xs = __builtin_fma(hh, vx, x), ys = __builtin_fma(hh, vy, y);
-  double ux = __builtin_fma(hh, ax, vx), uy = __builtin_fma(hh, ay, vy);
-  double svx = ax, svy = ay;
-  F(T, wsb, xs, ys, ax, ay, status);
-  double sx = __builtin_fma(2.0, ux, vx), sy = __builtin_fma(2.0, uy, vy);
-  svx = __builtin_fma(2.0, ax, svx);
-  svy = __builtin_fma(2.0, ay, svy);
-  // stage 3 point
-  xs = __builtin_fma(hh, ux, x);
-  ys = __builtin_fma(hh, uy, y);
-  ux = __builtin_fma(hh, ax, vx);
-  uy = __builtin_fma(hh, ay, vy);
-  F(T, wsb, xs, ys, ax, ay, status);
-  sx = __builtin_fma(2.0, ux, sx);
-  sy = __builtin_fma(2.0, uy, sy);
-  svx = __builtin_fma(2.0, ax, svx);
-  svy = __builtin_fma(2.0, ay, svy);
-  // stage 4 point
-  xs = __builtin_fma(h, ux, x);
-  ys = __builtin_fma(h, uy, y);
-  ux = __builtin_fma(h, ax, vx);
-  uy = __builtin_fma(h, ay, vy);
-  F(T, wsb, xs, ys, ax, ay, status);
-  sx = sx + ux;
-  sy = sy + uy;
-  svx = svx + ax;
-  svy = svy + ay;
-  x = __builtin_fma(h6, sx, x);
-  y = __builtin_fma(h6, sy, y);
-  vx = __builtin_fma(h6, svx, vx);
-  vy = __builtin_fma(h6, svy, vy);
+__device__ __forceinline__ void rk4_step_k(const Force<FORCE> &F, const Tables &T, const WsRef &wsb, const HexK &K,
+                                           double &x, double &y, double &vx, double &vy, double r2, uint32_t &status) {
+  rk4_expanded(
+      K,
+      [&](double px, double py, double pr2, double &ax, double &ay) {
+        if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX) force_hex_r2(wsb, T.umax, F.ks, F.cu, F.g, px, py, pr2, ax, ay);
+        else F(T, wsb, px, py, ax, ay, status);
+      },
+      x, y, vx, vy, r2);
 }
 
 template <int FORCE>
 __device__ __forceinline__ void rk4_step(const Force<FORCE> &F, const Tables &T, const WsRef &wsb, double h,
                                          double &x, double &y, double &vx, double &vy, double r2, uint32_t &status) {
-  rk4_step_hh<FORCE>(F, T, wsb, h, 0.5 * h, h * kSixth, x, y, vx, vy, r2, status);
+  rk4_step_k<FORCE>(F, T, wsb, hex_k(1.0, h), x, y, vx, vy, r2, status);
 }
 
 // The hexapole step with the force scale folded into the step constants.  The acceleration is
 // a = ks W'(cu r^2) (x, y) - g y_hat = ks s with the slope s = W' (x, y) + (0, gk), gk = -g/ks, so
-// the stages carry s and ks enters through the velocity-update constants khh = ks hh, kh = ks h and
-// kh6 = ks h6: one multiplication fewer per force evaluation than rk4_step.  The operation order
-// is rk4_step's otherwise; the CPU oracle mirrors every FMA.
+// the step takes K = hex_k(ks, h): one multiplication fewer per slope than the plain step.
 __device__ __forceinline__ void slope_hex_k(const WsRef &wsb, double umax, double cu, double gk, double x, double y,
                                             double r2, double &sx, double &sy) {
   const double w = wspline_eval(wsb, umax, r2 * cu);
@@ -415,43 +416,11 @@ __device__ __forceinline__ void slope_hex_k(const WsRef &wsb, double umax, doubl
   sy = __builtin_fma(w, y, gk);
 }
 
-__device__ __forceinline__ void rk4_step_hex_k(const WsRef &wsb, double umax, double cu, double gk, double h,
-                                               double hh, double h6, double khh, double kh, double kh6, double &x,
-                                               double &y, double &vx, double &vy, double r2) {
-  double ax, ay;
-  slope_hex_k(wsb, umax, cu, gk, x, y, r2, ax, ay);
-  // stage 2 point
-  double xs = __builtin_fma(hh, vx, x), ys = __builtin_fma(hh, vy, y);
-  double ux = __builtin_fma(khh, ax, vx), uy = __builtin_fma(khh, ay, vy);
-  double svx = ax, svy = ay;
-  slope_hex_k(wsb, umax, cu, gk, xs, ys, __builtin_fma(xs, xs, ys * ys), ax, ay);
-  double sx = __builtin_fma(2.0, ux, vx), sy = __builtin_fma(2.0, uy, vy);
-  svx = __builtin_fma(2.0, ax, svx);
-  svy = __builtin_fma(2.0, ay, svy);
-  // stage 3 point
-  xs = __builtin_fma(hh, ux, x);
-  ys = __builtin_fma(hh, uy, y);
-  ux = __builtin_fma(khh, ax, vx);
-  uy = __builtin_fma(khh, ay, vy);
-  slope_hex_k(wsb, umax, cu, gk, xs, ys, __builtin_fma(xs, xs, ys * ys), ax, ay);
-  sx = __builtin_fma(2.0, ux, sx);
-  sy = __builtin_fma(2.0, uy, sy);
-  svx = __builtin_fma(2.0, ax, svx);
-  svy = __builtin_fma(2.0, ay, svy);
-  // stage 4 point
-  xs = __builtin_fma(h, ux, x);
-  ys = __builtin_fma(h, uy, y);
-  ux = __builtin_fma(kh, ax, vx);
-  uy = __builtin_fma(kh, ay, vy);
-  slope_hex_k(wsb, umax, cu, gk, xs, ys, __builtin_fma(xs, xs, ys * ys), ax, ay);
-  sx = sx + ux;
-  sy = sy + uy;
-  svx = svx + ax;
-  svy = svy + ay;
-  x = __builtin_fma(h6, sx, x);
-  y = __builtin_fma(h6, sy, y);
-  vx = __builtin_fma(kh6, svx, vx);
-  vy = __builtin_fma(kh6, svy, vy);
+__device__ __forceinline__ void rk4_step_hex_k(const WsRef &wsb, double umax, double cu, double gk, const HexK &K,
+                                               double &x, double &y, double &vx, double &vy, double r2) {
+  rk4_expanded(
+      K, [&](double px, double py, double pr2, double &sx, double &sy) { slope_hex_k(wsb, umax, cu, gk, px, py, pr2, sx, sy); },
+      x, y, vx, vy, r2);
 }
 
 // One configuration-mode lens step of size h: the folded hexapole step where the lens folds,
@@ -461,8 +430,7 @@ __device__ __forceinline__ void lens_step(const Force<FORCE> &F, const Tables &T
                                           double &x, double &y, double &vx, double &vy, double r2, uint32_t &status) {
   if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX) {
     if (hex_folds(F.ks)) {
-      const double hh = 0.5 * h, h6 = h * kSixth;
-      rk4_step_hex_k(wsb, T.umax, F.cu, F.gk, h, hh, h6, F.ks * hh, F.ks * h, F.ks * h6, x, y, vx, vy, r2);
+      rk4_step_hex_k(wsb, T.umax, F.cu, F.gk, hex_k(F.ks, h), x, y, vx, vy, r2);
       return;
     }
   }

@@ -27,36 +27,66 @@ struct OrdList {
   int64_t region;          // entries per walk wavefront's region
 };
 
+// Fast single-precision helpers for the prediction (its error only moves a molecule between
+// neighbouring buckets): the hardware reciprocal and square root (1 ulp), and polynomial arc
+// functions.  acos: Abramowitz & Stegun 4.4.45, |error| <= 6.7e-5 on [0, 1], with
+// acos(-x) = pi - acos(x).  atan on [0, 1]: the odd degree-7 polynomial
+// a - 0.327622764 a^3 + 0.15931422 a^5 - 0.0464964749 a^7 (|error| <= 2.1e-4 rad, largest at a = 1),
+// extended to atan2 by the octant symmetries.  A bucket is 2 lens steps, some 5e-4 of a focusing
+// period at the lens's field.
+__device__ __forceinline__ float ord_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float ord_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+__device__ __forceinline__ float ord_acos(float x) {
+  const float ax = __builtin_fabsf(x);
+  const float p = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(-0.0187293f, ax, 0.0742610f), ax, -0.2121144f), ax,
+                                 1.5707288f);
+  const float r = ord_sqrt(1.0f - ax) * p;
+  return x < 0.0f ? 3.14159265f - r : r;
+}
+
+__device__ __forceinline__ float ord_atan2(float y, float x) {
+  const float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
+  const float mx = __builtin_fmaxf(ax, ay), mn = __builtin_fminf(ax, ay);
+  const float a = mx > 0.0f ? mn * ord_rcp(mx) : 0.0f, s = a * a;
+  float r = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(-0.0464964749f, s, 0.15931422f), s, -0.327622764f), s * a, a);
+  r = ay > ax ? 1.57079633f - r : r;
+  r = x < 0.0f ? 3.14159265f - r : r;
+  return y < 0.0f ? -r : r;
+}
+
 // Predicted lens steps of a molecule entering a hexapole lens at (x, y) with transverse velocity
 // (vx, vy) and axial speed vz, as the ordering bucket: the time until its radius reaches the bore
 // under the linear acceleration k2 r (the force where the W' table saturates; k2 > 0 pushes out,
 // r^2(t) = P cosh 2kt + Q sinh 2kt + S; k2 < 0 pulls in, r^2(t) = A + D cos(2wt - phi)), in steps,
-// at most the lens's full steps.  Single precision: the prediction orders work and is not a result.
+// at most the lens's full steps.  Single precision with fast approximations: the prediction orders
+// work and is not a result.
 __device__ __forceinline__ int ord_bucket(float k2, float x, float y, float vx, float vy, float vz, float bore2,
                                           float len, float h, float cap, int ref) {
   const float r0s = x * x + y * y, v0s = vx * vx + vy * vy, rv = x * vx + y * vy;
   float t = __builtin_inff();  // the molecule stays inside the bore
   if (k2 > 0.0f) {
-    const float k = __builtin_sqrtf(k2), iv = v0s / k2;
-    const float P = 0.5f * (r0s + iv), Q = rv / k, S = 0.5f * (r0s - iv);
+    const float k = ord_sqrt(k2), ik = ord_rcp(k), iv = v0s * ord_rcp(k2);
+    const float P = 0.5f * (r0s + iv), Q = rv * ik, S = 0.5f * (r0s - iv);
     // u = e^(2kt) solves (P + Q) u^2 + 2 (S - bore2) u + (P - Q) = 0 (larger root)
     const float a = P + Q, b = 2.0f * (S - bore2), c = P - Q;
-    const float u = (__builtin_sqrtf(__builtin_fmaxf(b * b - 4.0f * a * c, 0.0f)) - b) / (2.0f * a);
-    t = __logf(__builtin_fmaxf(u, 1.0f)) / (2.0f * k);
+    const float u = (ord_sqrt(__builtin_fmaxf(b * b - 4.0f * a * c, 0.0f)) - b) * ord_rcp(2.0f * a);
+    t = __logf(__builtin_fmaxf(u, 1.0f)) * (0.5f * ik);
   } else if (k2 < 0.0f) {
-    const float w = __builtin_sqrtf(-k2), iv = v0s / -k2;
-    const float A = 0.5f * (r0s + iv), Bc = 0.5f * (r0s - iv), C = rv / w;
-    const float D = __builtin_sqrtf(Bc * Bc + C * C);
+    const float w = ord_sqrt(-k2), iw = ord_rcp(w), iv = v0s * ord_rcp(-k2);
+    const float A = 0.5f * (r0s + iv), Bc = 0.5f * (r0s - iv), C = rv * iw;
+    const float D = ord_sqrt(Bc * Bc + C * C);
     if (A + D > bore2) {  // first upward crossing of bore2: 2wt = (phi - acos((bore2 - A) / D)) mod 2 pi
-      const float al = acosf(__builtin_fminf(__builtin_fmaxf((bore2 - A) / D, -1.0f), 1.0f));
-      float th = atan2f(C, Bc) - al;
+      const float al = ord_acos(__builtin_fminf(__builtin_fmaxf((bore2 - A) * ord_rcp(D), -1.0f), 1.0f));
+      float th = ord_atan2(C, Bc) - al;
       th -= 6.28318531f * __builtin_floorf(th * 0.159154943f);
-      t = th / (2.0f * w);
+      t = th * (0.5f * iw);
     }
   }
   // full steps: floor(L / (vz dt)) capped (configuration mode), n = rint(L/dz) (reference mode)
-  const float steps_max = ref ? cap : __builtin_fminf(__builtin_floorf(len / (vz * h)), cap);
-  float s = t * (ref ? vz / h : 1.0f / h);
+  const float ih = ord_rcp(h);
+  const float steps_max = ref ? cap : __builtin_fminf(__builtin_floorf(len * ord_rcp(vz * h)), cap);
+  float s = t * (ref ? vz * ih : ih);
   s = __builtin_fminf(s, steps_max);  // a NaN time gives the full steps
   s = s >= 0.0f ? s : 0.0f;           // and a NaN speed bucket 0
   return (int)__builtin_fminf(s * (1.0f / ORD_WIDTH), (float)(ORD_BUCKETS - 1));

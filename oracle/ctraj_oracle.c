@@ -195,47 +195,38 @@ static int lens_ref(const ctraj_elem *E, const OTab *T, int state, double g, Mol
   return 1;
 }
 
-/* classical RK4 on (x, y, vx, vy), running slope sums, h/6 formed as h*(1/6); same operation order
- * as the device */
+/* classical RK4 on (x, y, vx, vy); the force does not depend on the velocity, so the stages expand
+ * to x2 = x + h/2 v, x3 = x2 + (h/2)(h/2) a1, x4 = (x + h v) + h (h/2) a2,
+ * x' = (x + h v) + (h/6) h (a1 + a2 + a3), v' = v + (h/6)(2 (a1 + a2 + a3) + (a4 - a1)); h/6 formed as
+ * h*(1/6); the device's rk4_step (rk4_expanded with hex_k(1, h)), same operations in the same order */
 static void rk4_step(const OForce *F, const OTab *T, int state, double h, double *x, double *y, double *vx,
                      double *vy, uint32_t *st) {
   const double hh = 0.5 * h, h6 = h * (1.0 / 6.0);
-  double ax, ay;
-  force_eval(F, T, state, *x, *y, &ax, &ay, st);
-  double xs = fma(hh, *vx, *x), ys = fma(hh, *vy, *y);
-  double ux = fma(hh, ax, *vx), uy = fma(hh, ay, *vy);
-  double svx = ax, svy = ay;
-  force_eval(F, T, state, xs, ys, &ax, &ay, st);
-  double sx = fma(2.0, ux, *vx), sy = fma(2.0, uy, *vy);
-  svx = fma(2.0, ax, svx);
-  svy = fma(2.0, ay, svy);
-  xs = fma(hh, ux, *x);
-  ys = fma(hh, uy, *y);
-  ux = fma(hh, ax, *vx);
-  uy = fma(hh, ay, *vy);
-  force_eval(F, T, state, xs, ys, &ax, &ay, st);
-  sx = fma(2.0, ux, sx);
-  sy = fma(2.0, uy, sy);
-  svx = fma(2.0, ax, svx);
-  svy = fma(2.0, ay, svy);
-  xs = fma(h, ux, *x);
-  ys = fma(h, uy, *y);
-  ux = fma(h, ax, *vx);
-  uy = fma(h, ay, *vy);
-  force_eval(F, T, state, xs, ys, &ax, &ay, st);
-  sx = sx + ux;
-  sy = sy + uy;
-  svx = svx + ax;
-  svy = svy + ay;
-  *x = fma(h6, sx, *x);
-  *y = fma(h6, sy, *y);
-  *vx = fma(h6, svx, *vx);
-  *vy = fma(h6, svy, *vy);
+  const double c3 = hh * hh, c4 = h * hh, cx = h6 * h;
+  double a1x, a1y, a2x, a2y, a3x, a3y, a4x, a4y;
+  force_eval(F, T, state, *x, *y, &a1x, &a1y, st);
+  const double x2 = fma(hh, *vx, *x), y2 = fma(hh, *vy, *y);
+  force_eval(F, T, state, x2, y2, &a2x, &a2y, st);
+  const double x3 = fma(c3, a1x, x2), y3 = fma(c3, a1y, y2);
+  const double xv = fma(h, *vx, *x), yv = fma(h, *vy, *y);
+  force_eval(F, T, state, x3, y3, &a3x, &a3y, st);
+  const double x4 = fma(c4, a2x, xv), y4 = fma(c4, a2y, yv);
+  double sumx = a1x + a2x, sumy = a1y + a2y;
+  sumx = sumx + a3x;
+  sumy = sumy + a3y;
+  force_eval(F, T, state, x4, y4, &a4x, &a4y, st);
+  *x = fma(cx, sumx, xv);
+  *y = fma(cx, sumy, yv);
+  *vx = fma(h6, fma(2.0, sumx, a4x - a1x), *vx);
+  *vy = fma(h6, fma(2.0, sumy, a4y - a1y), *vy);
 }
 
 /* A hexapole lens folds its force scale into the step constants when |ks| lies in [2^-500, 2^500]:
- * a = ks W'(cu r^2) (x, y) - g y_hat = ks s with s = W' (x, y) + (0, gk), gk = -g/ks, so the stages
- * carry s and ks enters through ks hh, ks h, ks h6 (the device's rk4_step_hex_k, same operations). */
+ * a = ks W'(cu r^2) (x, y) - g y_hat = ks s with s = W' (x, y) + (0, gk), gk = -g/ks.  The force does
+ * not depend on the velocity, so the classical RK4 stages on (x, v) expand to
+ *   x2 = x + h/2 v,  x3 = x2 + (h/2)(ks h/2) s1,  x4 = (x + h v) + h (ks h/2) s2,
+ *   x' = (x + h v) + (h/6)(ks h)(s1 + s2 + s3),  v' = v + (ks h/6)(2 (s1 + s2 + s3) + (s4 - s1))
+ * (the device's hex_k and rk4_step_hex_k, same operations in the same order). */
 static int hex_folds(double ks) {
   const double a = fabs(ks);
   return a >= 0x1p-500 && a <= 0x1p500;
@@ -249,40 +240,26 @@ static void slope_hex_k(const OTab *T, int state, double cu, double gk, double x
 
 static void rk4_step_hex_k(const OForce *F, const OTab *T, int state, double h, double *x, double *y, double *vx,
                            double *vy) {
-  const double hh = 0.5 * h, h6 = h * (1.0 / 6.0);
-  const double khh = F->ks * hh, kh = F->ks * h, kh6 = F->ks * h6;
-  const double gk = -F->g / F->ks, cu = F->cu;
-  double ax, ay;
-  slope_hex_k(T, state, cu, gk, *x, *y, &ax, &ay);
-  double xs = fma(hh, *vx, *x), ys = fma(hh, *vy, *y);
-  double ux = fma(khh, ax, *vx), uy = fma(khh, ay, *vy);
-  double svx = ax, svy = ay;
-  slope_hex_k(T, state, cu, gk, xs, ys, &ax, &ay);
-  double sx = fma(2.0, ux, *vx), sy = fma(2.0, uy, *vy);
-  svx = fma(2.0, ax, svx);
-  svy = fma(2.0, ay, svy);
-  xs = fma(hh, ux, *x);
-  ys = fma(hh, uy, *y);
-  ux = fma(khh, ax, *vx);
-  uy = fma(khh, ay, *vy);
-  slope_hex_k(T, state, cu, gk, xs, ys, &ax, &ay);
-  sx = fma(2.0, ux, sx);
-  sy = fma(2.0, uy, sy);
-  svx = fma(2.0, ax, svx);
-  svy = fma(2.0, ay, svy);
-  xs = fma(h, ux, *x);
-  ys = fma(h, uy, *y);
-  ux = fma(kh, ax, *vx);
-  uy = fma(kh, ay, *vy);
-  slope_hex_k(T, state, cu, gk, xs, ys, &ax, &ay);
-  sx = sx + ux;
-  sy = sy + uy;
-  svx = svx + ax;
-  svy = svy + ay;
-  *x = fma(h6, sx, *x);
-  *y = fma(h6, sy, *y);
-  *vx = fma(kh6, svx, *vx);
-  *vy = fma(kh6, svy, *vy);
+  const double ks = F->ks;
+  const double hh = 0.5 * h, h6 = h * (1.0 / 6.0), khh = ks * hh, kh = ks * h;
+  const double c3 = hh * khh, c4 = h * khh, cx = h6 * kh, kh6 = ks * h6;
+  const double gk = -F->g / ks, cu = F->cu;
+  double s1x, s1y, s2x, s2y, s3x, s3y, s4x, s4y;
+  slope_hex_k(T, state, cu, gk, *x, *y, &s1x, &s1y);
+  const double x2 = fma(hh, *vx, *x), y2 = fma(hh, *vy, *y);
+  slope_hex_k(T, state, cu, gk, x2, y2, &s2x, &s2y);
+  const double x3 = fma(c3, s1x, x2), y3 = fma(c3, s1y, y2);
+  const double xv = fma(h, *vx, *x), yv = fma(h, *vy, *y);
+  slope_hex_k(T, state, cu, gk, x3, y3, &s3x, &s3y);
+  const double x4 = fma(c4, s2x, xv), y4 = fma(c4, s2y, yv);
+  double sumx = s1x + s2x, sumy = s1y + s2y;
+  sumx = sumx + s3x;
+  sumy = sumy + s3y;
+  slope_hex_k(T, state, cu, gk, x4, y4, &s4x, &s4y);
+  *x = fma(cx, sumx, xv);
+  *y = fma(cx, sumy, yv);
+  *vx = fma(kh6, fma(2.0, sumx, s4x - s1x), *vx);
+  *vy = fma(kh6, fma(2.0, sumy, s4y - s1y), *vy);
 }
 
 /* one configuration-mode lens step: the folded hexapole step where the lens folds, else rk4_step */
