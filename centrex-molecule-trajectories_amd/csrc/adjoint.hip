@@ -125,7 +125,9 @@ __device__ __forceinline__ void smooth_step(const Force<CTRAJ_FORCE_WSPLINE_HEX>
     lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, full ? S.hk.h : hp, x, y, vx, vy, r2, st);
 }
 
-// Reverse of one classical RK4 step (rk4_step) of size h from input state (x, y, vx, vy).
+// Reverse of one classical RK4 step of size h from input state (x, y, vx, vy), derived from the
+// stages with their velocities formed; the forward step (rk4_expanded) forms the same stages in
+// expanded form, equal in exact arithmetic, so this is its adjoint up to rounding.
 // In: (pbx, pby, vbx, vby) = adjoint of the step's output.  Out: adjoint of its input.
 // Vbar accumulates d/dV; with WANT_H the derivative with respect to h is returned.
 template <bool WANT_H>
