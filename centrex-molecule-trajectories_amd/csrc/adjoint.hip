@@ -136,7 +136,7 @@ __device__ __forceinline__ double rk4_adjoint(const LensK &L, double h, double x
   const double hh = 0.5 * h, h6 = h * kSixth, h3 = 2.0 * h6;
   const double g = L.g;
   double f1, d1, g1, f2, d2, g2, f3, d3, g3, f4, d4, g4;
-  // forward stages, bit-identical to rk4_step
+  // forward stages with their velocities (the classical form: rk4_step's expanded stages up to rounding)
   hex_eval_d(L.wsb, L.umax, L.ks, L.cu, L.kc2, L.invV, __builtin_fma(x, x, y * y), f1, d1, g1);
   const double a1x = f1 * x, a1y = __builtin_fma(f1, y, -g);
   const double p2x = __builtin_fma(hh, vx, x), p2y = __builtin_fma(hh, vy, y);
