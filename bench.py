@@ -30,9 +30,9 @@ import numpy as np  # noqa: E402
 # clamps, conversions and address arithmetic are not counted.
 FLOP_PER_FACE = 28  # time to the plane (sub, div: 2) + drift (dt^2, 3 x position 5, 3 x velocity 2, time: 23)
 #                     + squared-radius test (x*x + y*y: 3)
-FLOP_PER_RK4_STEP = 84  # folded hexapole RK4 step: 4 slope evaluations (r^2*cu 1, fract 1, spline 2 FMA, slope
+FLOP_PER_RK4_STEP = 82  # folded hexapole RK4 step: 4 slope evaluations (r^2*cu 1, fract 1, spline 2 FMA, slope
 #                         mul + FMA: 9 each, plus r^2 = 3 at stages 2-4), stage points 16, slope sums and update
-#                         18, z/t 2, bore test r^2 3
+#                         18, bore test r^2 3 (z and t are formed once per molecule from the step count)
 FLOP_ADJ_STEP = 204  # reverse of one RK4 step: 4 force+derivative evaluations, stage points, adjoint sums
 FLOP_SMOOTH_FACE = 23  # smoothed objective, per aperture face: drift, radius, sigmoid, product
 FLOP_SMOOTH_FACE_REV = 34  # its reverse: radius, sigmoid, factor derivative, drift adjoint

@@ -79,7 +79,7 @@ constexpr int BPOST_ITEMS = 4;  // phase 3: listed slots per thread
 
 struct BLane {
   int c, e0, ne, lens, label_inside;
-  double ks, cu, R2, z1, gk;
+  double ks, cu, R2, z1, gk, hs;  // hs: the lens's full step dt
 };
 
 __device__ __forceinline__ int find_cfg(const int64_t *s_begin, int n_cfg, int64_t gi) {
@@ -255,6 +255,7 @@ __global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
               b.R2 = L.p[1];
               b.z1 = L.z1;
               b.label_inside = L.label_inside;
+              b.hs = Hs;
               s.i = 0;
               s.nfull = rk4_full_steps(Ll, m.vz, Hs, A.cap, s.status);
               s.dzs = m.vz * Hs;
@@ -280,12 +281,18 @@ __global__ void __launch_bounds__(256) k_batch_lanes(BArgs A, KTab KT) {
       const Force<CTRAJ_FORCE_WSPLINE_HEX> F{b.ks, b.cu, g, b.gk};
       const bool last = (s.i == s.nfull);
       lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, s.h, m.x, m.y, m.vx, m.vy, s.r2, s.status);
-      m.z = last ? b.z1 : m.z + s.dzs;
-      m.t = m.t + s.h;
       s.r2 = __builtin_fma(m.x, m.x, m.y * m.y);
       ++s.i;
       s.h = (s.i == s.nfull) ? s.hp : s.h;
       const bool dead = s.r2 > b.R2;
+      // z and t stay at their entry values until the molecule finishes (the walker's definition)
+      if (last) {
+        m.z = b.z1;
+        m.t = __builtin_fma((double)s.nfull, b.hs, m.t) + s.hp;
+      } else if (dead) {
+        m.z = __builtin_fma((double)s.i, s.dzs, m.z);
+        m.t = __builtin_fma((double)s.i, b.hs, m.t);
+      }
       s.label = dead ? b.label_inside : s.label;
       s.fin = dead ? 2 : 0;
       s.phase = dead ? (int)PH_NEED : (last ? (int)PH_POST : (int)PH_LENS);
@@ -387,7 +394,9 @@ k_batch_integ(BArgs A, BPipe P, KTab KT) {
 
   MState m;
   m.x = m.y = m.z = m.vx = m.vy = m.vz = m.t = 0.0;
-  double r2 = 0.0, dzs = 0.0, ks = 0.0, cu = 0.0, gk = 0.0, R2 = 0.0, Hs = A.Hs;
+  // z and t stay at their entry values while the molecule steps: a stopped one is only counted, and
+  // one that leaves takes z1 and t_in + n dt + hp (the walker's definition)
+  double r2 = 0.0, ks = 0.0, cu = 0.0, gk = 0.0, R2 = 0.0, Hs = A.Hs;
   HexK hk{};  // UH: the folded step's constants of the molecule's lens
   int rem = BREM_EMPTY, nfull = 0, c = 0;
   uint32_t status = 0;
@@ -410,7 +419,7 @@ k_batch_integ(BArgs A, BPipe P, KTab KT) {
             const Force<CTRAJ_FORCE_WSPLINE_HEX> F{ks, cu, g, gk};
             lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, hp, m.x, m.y, m.vx, m.vy, r2, status);
             m.z = K.z1;
-            m.t = m.t + hp;
+            m.t = __builtin_fma((double)nfull, Hs, m.t) + hp;
             r2 = __builtin_fma(m.x, m.x, m.y * m.y);
             outcome = r2 > R2 ? BLENS_STOPPED : BLENS_LEFT;
           }
@@ -470,7 +479,6 @@ k_batch_integ(BArgs A, BPipe P, KTab KT) {
             }
             status = 0;
             nfull = rk4_full_steps(K.Ll, m.vz, Hs, A.cap, status);
-            dzs = m.vz * Hs;
             r2 = __builtin_fma(m.x, m.x, m.y * m.y);
             rem = nfull;
           }
@@ -490,8 +498,6 @@ k_batch_integ(BArgs A, BPipe P, KTab KT) {
         const Force<CTRAJ_FORCE_WSPLINE_HEX> F{ks, cu, g, gk};
         lens_step<CTRAJ_FORCE_WSPLINE_HEX>(F, T, wsb, Hs, m.x, m.y, m.vx, m.vy, r2, status);
       }
-      m.z = m.z + dzs;
-      m.t = m.t + Hs;
       r2 = __builtin_fma(m.x, m.x, m.y * m.y);
       rem = r2 > R2 ? BREM_STOPPED : rem - 1;
     }

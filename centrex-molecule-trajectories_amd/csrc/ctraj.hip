@@ -653,19 +653,25 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
     if (s.phase == PH_LENS) {
       MState &m = s.m;
       if constexpr (INTEG == CTRAJ_INTEG_RK4) {
-        // branch-free step tail: every lane in the lens runs the same instructions
+        // branch-free step tail (every lane in the lens runs the same instructions), except where
+        // a molecule finishes: z and t stay at their entry values until then
         const bool last = (s.i == s.nfull);
         if constexpr (FOLD) {
           rk4_step_hex_k(s.wsb, T.umax, F.cu, LA.gk, hex_k(F.ks, s.h), m.x, m.y, m.vx, m.vy, s.r2);
         } else {
           rk4_step<FORCE>(F, T, s.wsb, s.h, m.x, m.y, m.vx, m.vy, s.r2, s.status);
         }
-        m.z = last ? z1 : m.z + s.dzs;
-        m.t = m.t + s.h;
         s.r2 = __builtin_fma(m.x, m.x, m.y * m.y);
         ++s.i;
         s.h = (s.i == s.nfull) ? s.hp : s.h;
         const bool dead = s.r2 > R2;
+        if (last) {  // after the partial step
+          m.z = z1;
+          m.t = __builtin_fma((double)s.nfull, L0.p[2], m.t) + s.hp;
+        } else if (dead) {  // stopped by the bore after s.i full steps of the lens's dt
+          m.z = __builtin_fma((double)s.i, s.dzs, m.z);
+          m.t = __builtin_fma((double)s.i, L0.p[2], m.t);
+        }
         s.label = dead ? label_inside : s.label;
         s.fin = dead ? 2 : 0;
         s.phase = dead ? (int)PH_NEED : (last ? (int)PH_POST : (int)PH_LENS);
@@ -710,9 +716,11 @@ __global__ void __launch_bounds__(768) k_propagate_lanes(KBeam B, KTab KT, const
 // (configuration mode) or the exit hop with the stored a = l1 (reference mode) -- writes each
 // finished molecule's state back into its queue entry with its outcome (queue_put_done), and gives
 // the empty lanes the next queued molecules.  A lane's rem is its full steps left (> 0: it takes
-// steps; 0: it reached the end of the lens), REM_STOPPED once the bore stopped it, REM_EMPTY when it
-// holds no molecule, so one compare per iteration gives both the step's lane mask and the refill
-// count, and a step updates rem with one compare, one add and one select.  Finished molecules are walked
+// steps; 0: it reached the end of the lens), REM_EMPTY when it holds no molecule, and at most
+// REM_STOPPED once the bore stopped it: -2 - k after its k-th full step in configuration mode (the
+// lane forms z and t from k when it finishes, not step by step), REM_STOPPED in reference mode; so
+// one compare per iteration gives both the step's lane mask and the refill count, and a step updates
+// rem with one compare, one select and one subtraction.  Finished molecules are walked
 // through the remaining elements and recorded by POST, so the refill section carries no element
 // walk and no recording.  Wavefronts claim the queue in 64-entry chunks, region by region, from
 // eight counters: counter g serves the regions of the PRE workgroups dispatched to XCD g
@@ -771,13 +779,15 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
   int64_t next = 0, end = 0;
   bool exhausted = false;
 
-  // lane state: the molecule (z, t advance with the steps; vz is constant in the lens), its queue
-  // entry and key, rem (see above), its full-step count and state table
+  // lane state: the molecule (configuration mode: z and t stay at their entry values until the lane
+  // finishes; reference mode: they advance with the steps; vz is constant in the lens), its queue
+  // entry and key, rem (see above), its full-step count plus 3 (configuration mode) or the step dt
+  // (reference mode), and its state table
   MState m;
   m.x = m.y = m.z = m.vx = m.vy = m.vz = m.t = 0.0;
   m.ax = 0.0; m.ay = -B.g; m.az = 0.0;
-  double r2 = 0.0, dzs = 0.0, h = 0.0;
-  int rem = REM_EMPTY, nfull = 0;
+  double r2 = 0.0, h = 0.0;
+  int rem = REM_EMPTY, nfull3 = 0;
   uint32_t status = 0;
   int64_t slot = 0, key = 0;
   WsRef wsb = ws_of(T, 0);
@@ -790,8 +800,8 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
         // finish the molecules that hold a lane but take no more full steps: stopped by the bore, or
         // at the end of the lens, which takes one more step (configuration mode) or the exit hop
         // (reference mode)
-        const bool fin = rem == 0 || rem == REM_STOPPED;
-        const bool dead = rem == REM_STOPPED;
+        const bool fin = rem == 0 || rem <= REM_STOPPED;
+        const bool dead = rem <= REM_STOPPED;
         const uint64_t ends = __builtin_amdgcn_ballot_w64(rem == 0);
         if constexpr (INTEG == CTRAJ_INTEG_RK4) steps_w += (uint32_t)__popcll(ends);
         else faces_w += (uint32_t)__popcll(ends);
@@ -799,6 +809,7 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
           int outcome = LENS_STOPPED;
           if constexpr (INTEG == CTRAJ_INTEG_RK4) {
             if (!dead) {  // the end of the lens: one partial step of L/vz - n dt, so the lens ends at z1
+              const int nfull = nfull3 - 3;
               const double hp = Ll / m.vz - (double)nfull * Hs;
               if constexpr (FORCE == CTRAJ_FORCE_WSPLINE_HEX) {  // the host launches only folding lenses here
                 rk4_step_hex_k(wsb, T.umax, F.cu, K.gk, hex_k(F.ks, hp), m.x, m.y, m.vx, m.vy, r2);
@@ -806,9 +817,13 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
                 rk4_step<FORCE>(F, T, wsb, hp, m.x, m.y, m.vx, m.vy, r2, status);
               }
               m.z = z1;
-              m.t = m.t + hp;
+              m.t = __builtin_fma((double)nfull, Hs, m.t) + hp;
               r2 = __builtin_fma(m.x, m.x, m.y * m.y);
               outcome = r2 > R2 ? LENS_STOPPED : LENS_LEFT;
+            } else {  // stopped by the bore after k full steps: z and t from their entry values
+              const double k = (double)(REM_STOPPED - rem);
+              m.z = __builtin_fma(k, m.vz * Hs, m.z);
+              m.t = __builtin_fma(k, Hs, m.t);
             }
           } else {
             if (!dead) {  // exit hop with the stored a = l1 of the last step (electrostatic_lens.py:74-77)
@@ -875,14 +890,13 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
             wsb = ws_of(T, (int)((uint64_t)key >> 56));
             status = 0;
             if constexpr (INTEG == CTRAJ_INTEG_RK4) {
-              nfull = rk4_full_steps(Ll, m.vz, Hs, B.cap, status);
-              dzs = m.vz * Hs;
+              rem = rk4_full_steps(Ll, m.vz, Hs, B.cap, status);
+              nfull3 = rem + 3;
               r2 = __builtin_fma(m.x, m.x, m.y * m.y);
             } else {
-              nfull = n_ref;
+              rem = n_ref;
               h = Hs / m.vz;  // dt = dz / v_z (electrostatic_lens.py:88)
             }
-            rem = nfull;
           }
         }
         next += avail < (int64_t)__popcll(need) ? avail : (int64_t)__popcll(need);
@@ -899,10 +913,8 @@ k_lens(KBeam B, KTab KT, PipeArgs P, int refill_min, StepK K, unsigned long long
           rk4_step_hex_k(wsb, T.umax, F.cu, K.gk, K.hk, m.x, m.y, m.vx, m.vy, r2);
         else
           rk4_step_k<FORCE>(F, T, wsb, K.h1, m.x, m.y, m.vx, m.vy, r2, status);
-        m.z = m.z + dzs;
-        m.t = m.t + Hs;
         r2 = __builtin_fma(m.x, m.x, m.y * m.y);
-        rem = r2 > R2 ? REM_STOPPED : rem - 1;
+        rem = rem - (r2 > R2 ? nfull3 : 1);  // stopped after its k-th step: -2 - k
       } else {
         ref_step<FORCE>(F, T, wsb, h, m, status);  // the row stores a = l1
         rem = beyond_r2(m.x, m.y, R2) ? REM_STOPPED : rem - 1;  // R2: the threshold of d/2 (lens_ref)
@@ -1204,6 +1216,8 @@ int check_phys(ctraj_ctx *ctx, const ctraj_phys *p) {
   if (p->n_labels < 1 || p->n_labels > CTRAJ_MAX_LABELS) return fail(ctx, CTRAJ_EINVAL, "n_labels out of range");
   if (p->n_states < 1 || p->n_states > CTRAJ_MAX_STATES) return fail(ctx, CTRAJ_EINVAL, "n_states out of range");
   if (p->max_lens_steps < 0) return fail(ctx, CTRAJ_EINVAL, "max_lens_steps < 0");
+  // the lens kernels keep a stopped molecule's step count in its negative lane state (-2 - steps)
+  if (p->max_lens_steps > (1 << 30)) return fail(ctx, CTRAJ_EINVAL, "max_lens_steps > 2^30");
   return CTRAJ_OK;
 }
 

@@ -451,7 +451,9 @@ __device__ __forceinline__ int rk4_full_steps(double L, double vz, double dt, in
 
 // Config-mode lens interior: n = floor(L/(vz dt)) steps of dt, then one step of L/vz - n dt, so
 // the molecule leaves at z1.  Bore test x^2 + y^2 > (d/2)^2 after every step.  Rows store the
-// force at the row's position; the exit row stores (0, -g, 0).
+// force at the row's position; the exit row stores (0, -g, 0).  z and t after k full steps are
+// formed from the entry values, z_k = z_in + k (vz dt) and t_k = t_in + k dt (one FMA each, not
+// accumulated step by step), and after the partial step z = z1, t = (t_in + n dt) + hp.
 template <int FORCE, class Sink>
 __device__ __forceinline__ bool lens_rk4(const ctraj_elem &E, const Tables &T, int state, double g, int cap,
                                          MState &m, uint32_t &status, WalkCounters &wc, Sink &sink) {
@@ -463,19 +465,20 @@ __device__ __forceinline__ bool lens_rk4(const ctraj_elem &E, const Tables &T, i
   const double dzs = m.vz * h;
   const int nfull = rk4_full_steps(L, m.vz, h, cap, status);
   double x = m.x, y = m.y, vx = m.vx, vy = m.vy;
-  double z = m.z, t = m.t;
+  const double z_in = m.z, t_in = m.t;
+  double z = z_in, t = t_in;
   double r2 = __builtin_fma(x, x, y * y);
   bool alive = true;
   for (int i = 0; i <= nfull; ++i) {
     if (i < nfull) {
       lens_step<FORCE>(F, T, wsb, h, x, y, vx, vy, r2, status);
-      z = z + dzs;
-      t = t + h;
+      z = __builtin_fma((double)(i + 1), dzs, z_in);
+      t = __builtin_fma((double)(i + 1), h, t_in);
     } else {
       const double hp = L / m.vz - (double)nfull * h;
       lens_step<FORCE>(F, T, wsb, hp, x, y, vx, vy, r2, status);
       z = E.z1;
-      t = t + hp;
+      t = __builtin_fma((double)nfull, h, t_in) + hp;
     }
     ++wc.steps;
     r2 = __builtin_fma(x, x, y * y);

@@ -76,7 +76,7 @@ typedef struct ctraj_phys {
   int32_t n_labels;       /* distinct labels, <= CTRAJ_MAX_LABELS */
   int32_t label_detected; /* label of survivors (beamline.py:34-35) */
   int32_t n_states;       /* W-spline state tables (1..CTRAJ_MAX_STATES); molecule state ids must be < n_states */
-  int32_t max_lens_steps; /* safety cap on RK4 steps per lens per molecule (0 = 1<<20) */
+  int32_t max_lens_steps; /* safety cap on RK4 steps per lens per molecule (0 = 1<<20; at most 1<<30) */
 } ctraj_phys;
 
 /* structure-of-arrays phase space (device pointers) */

@@ -272,7 +272,9 @@ static void lens_step(const OForce *F, const OTab *T, int state, double h, doubl
 }
 
 /* configuration-mode lens interior: floor(L/(vz dt)) steps of dt, then L/vz - n dt; bore test
- * x^2 + y^2 > (d/2)^2 after each step; rows store the force at the row (exit row: (0,-g,0)) */
+ * x^2 + y^2 > (d/2)^2 after each step; rows store the force at the row (exit row: (0,-g,0));
+ * after k full steps z = z_in + k (vz dt), t = t_in + k dt (one fma each), after the partial step
+ * z = z1, t = (t_in + n dt) + hp */
 static int lens_rk4(const ctraj_elem *E, const OTab *T, int state, double g, int cap, Mol *m, uint32_t *st,
                     uint64_t *steps, Rows *R, double *edge) {
   OForce F = {E->force, E->table, E->p[4], E->p[5], g};
@@ -285,18 +287,19 @@ static int lens_rk4(const ctraj_elem *E, const OTab *T, int state, double g, int
     else if (q > (double)cap) { *st |= CTRAJ_ST_CAPPED; nfull = cap; }
     else nfull = (int)q;
   }
-  double x = m->x, y = m->y, vx = m->vx, vy = m->vy, z = m->z, t = m->t;
+  const double z_in = m->z, t_in = m->t;
+  double x = m->x, y = m->y, vx = m->vx, vy = m->vy, z = z_in, t = t_in;
   int alive = 1;
   for (int i = 0; i <= nfull; ++i) {
     if (i < nfull) {
       lens_step(&F, T, state, h, &x, &y, &vx, &vy, st);
-      z = z + dzs;
-      t = t + h;
+      z = fma((double)(i + 1), dzs, z_in);
+      t = fma((double)(i + 1), h, t_in);
     } else {
       const double hp = L / m->vz - (double)nfull * h;
       lens_step(&F, T, state, hp, &x, &y, &vx, &vy, st);
       z = E->z1;
-      t = t + hp;
+      t = fma((double)nfull, h, t_in) + hp;
     }
     (*steps)++;
     const double r2 = fma(x, x, y * y);
