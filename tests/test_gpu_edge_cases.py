@@ -146,7 +146,7 @@ def test_invalid_tables_and_arguments_return_einval_before_any_launch(ctx):
     e[lens_i].force = 7
     expect_einval(_set(ctx, e, cb.n_elems, _copy_phys(cb)), "bad force model")
     for field, bad in (("label_detected", -1), ("label_detected", 64), ("n_labels", 0), ("n_states", 0),
-                       ("n_states", 5), ("max_lens_steps", -1)):
+                       ("n_states", 5), ("max_lens_steps", -1), ("max_lens_steps", (1 << 30) + 1)):
         p = _copy_phys(cb)
         setattr(p, field, bad)
         expect_einval(_set(ctx, _copy_elems(cb), cb.n_elems, p), f"phys.{field}={bad}")
