@@ -177,7 +177,7 @@ typedef struct ctraj_run {
                           * 768 for the lane walker when the lens tables exceed 40 KB of LDS, else 256 */
   int32_t grid;          /* blocks (0 = enough for n, capped at 8 per CU x n_cu x 8) */
   int32_t flags;         /* CTRAJ_RUN_* */
-  int32_t refill;        /* lane refill: idle lanes per wavefront that trigger a refill (0 = 8 split path, 24 single kernel) */
+  int32_t refill;        /* lane refill: idle lanes per wavefront that trigger a refill (0 = 10 split path, 24 single kernel) */
 } ctraj_run;
 int32_t ctraj_propagate(ctraj_ctx *ctx, const ctraj_run *run);
 /* ctraj_propagate for callers that cannot allocate device memory (code under HIP graph capture, serving loops,
