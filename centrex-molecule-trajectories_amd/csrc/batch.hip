@@ -850,7 +850,7 @@ int32_t ctraj_batch_run(ctraj_batch *b, const ctraj_batch_tune *tune, int64_t *t
   const int block = tune && tune->block ? tune->block : (fused ? 256 : 1024);
   const int chunk = tune && tune->chunk ? tune->chunk : 128;
   const int grid = tune ? tune->grid : 0;
-  const int refill = tune && tune->refill ? tune->refill : (fused ? 16 : 8);
+  const int refill = tune && tune->refill ? tune->refill : 16;  // the tuner's usual pick on either path
   const bool use_graph = tune && tune->use_graph;
   if (block % 64 || block < 64 || block > (fused ? 256 : 1024))
     return fail(c, CTRAJ_EINVAL, fused ? "block must be 64, 128, 192 or 256" : "block must be a multiple of 64 up to 1024");

@@ -308,7 +308,7 @@ typedef struct ctraj_batch_tune {
                       * fused: 64, 128, 192 or 256 (0 = 256) */
   int32_t chunk;     /* fused: molecules a wavefront claims at a time from the global queue (0 = 128) */
   int32_t grid;      /* persistent blocks (0 = as many as are resident) */
-  int32_t refill;    /* refill a wavefront when at least this many lanes are idle, 1..64 (0 = 8, fused 16) */
+  int32_t refill;    /* refill a wavefront when at least this many lanes are idle, 1..64 (0 = 16) */
   int32_t use_graph; /* 1: capture the launches (counter and tally resets, kernels) once as a hipGraph and replay */
   int32_t fused;     /* 1: the single-kernel walker instead of the split kernels */
 } ctraj_batch_tune;
